@@ -1,0 +1,203 @@
+"""Benchmark of the SPH particle-interaction hot path on one MI355X.
+
+Workload: the ring collision at dx = 6.6e-5 m (nbox = 1213, 1,009,204
+particles), idx order shuffled from seed 0, N(0, 0.01 m/s) velocity jitter,
+E = 1e7 Pa, nu = 0.4, rho0 = 1010 kg/m^3, h = 1.3 dx, alpha = beta = 1,
+artificial stress eps = 0.3, RK4 with a CFL-0.3 time step.
+
+A step is one RK4 step of every particle (4 derivative evaluations,
+re-binning included).  Modes:
+  fp64   all arithmetic FP64 (the reference's arithmetic)
+  mixed  FP32 pair sums with compensated accumulation, FP64 per-particle
+         algebra and RK4 state
+`value` is the mixed-mode particle-steps/s; every mode's rate, the mixed
+mode's rel-L2 error against the FP64 mode after the accuracy horizon, the
+dominant kernel's roofline and the CPU baseline are in the same JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--accuracy-steps A]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M particles: FP64 vs mixed vs surrogate at ≤1e-3 rel L2 error"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one stage launch of the pair kernel, per particle:
+# read state8 + m, h (80 B) and write 8 derivatives (64 B).
+ALG_BYTES_PER_PARTICLE_STAGE = 144.0
+ALG_BYTES_PER_PARTICLE_STEP = 704.0
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+def run_mode(scene, precision, steps, warmup, log):
+    from mfree_iwf_amd import Solver
+    n = scene.particles.n
+    with Solver(precision=precision, use_graph=True) as sv:
+        sv.add_body(scene.particles, scene.params)
+        t0 = time.perf_counter()
+        sv.step(max(warmup, 1))
+        log(f"  warmup {warmup} steps: {time.perf_counter() - t0:.2f} s")
+        # timed region: HIP events around every pair-kernel launch on the
+        # context's stream (no graph while they are recorded)
+        sv.profile(True)
+        t0 = time.perf_counter()
+        sv.step(steps)  # returns after a stream synchronize
+        wall = time.perf_counter() - t0
+        prof = sv.profile_read()
+        sv.profile(False)
+        # the same number of steps replayed from the captured HIP graph
+        t0 = time.perf_counter()
+        sv.step(steps)
+        wall_graph = time.perf_counter() - t0
+        state = sv.state(0)
+    ms = prof["step_ms"] / max(prof["steps"], 1)
+    pair_ms = prof["pair_ms"] / max(prof["pair_launches"], 1)
+    return dict(
+        particle_steps_per_s=n * steps / (prof["step_ms"] / 1e3),
+        particle_steps_per_s_graph=n * steps / wall_graph,
+        ms_per_step=ms,
+        wall_s=wall,
+        pair_kernel_ms=pair_ms,
+        pair_launches=prof["pair_launches"],
+        state=state,
+    )
+
+
+def accuracy(scene, nsteps, log):
+    """Mixed vs FP64 GPU after nsteps RK4 steps from the same state."""
+    from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver
+    out = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        with Solver(precision=prec) as sv:
+            sv.add_body(scene.particles, scene.params)
+            t0 = time.perf_counter()
+            sv.step(nsteps)
+            out[name] = sv.state(0)
+            log(f"  accuracy run {name}: {nsteps} steps in {time.perf_counter() - t0:.2f} s")
+    a, b = out["mixed"], out["fp64"]
+    return dict(
+        steps=nsteps,
+        rel_l2_pos=rel_l2(np.concatenate([a.x, a.y]), np.concatenate([b.x, b.y])),
+        rel_l2_stress=rel_l2(np.concatenate([a.Sxx, a.Sxy, a.Syy]), np.concatenate([b.Sxx, b.Sxy, b.Syy])),
+        max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()),
+    )
+
+
+def cpu_baseline(scene, steps, log):
+    """The CPU oracle (reference-faithful FP64 restatement: per-stage sort,
+    Verlet lists, one loop per action) on one host core, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    t0 = time.perf_counter()
+    rc, _ = O.ul_step(scene.particles, scene.params, steps, order=O.ORDER_REFERENCE, threads=1)
+    dt = time.perf_counter() - t0
+    if rc != O.OK:
+        raise RuntimeError(f"oracle failed with status {rc}")
+    log(f"  cpu baseline: {steps} steps in {dt:.1f} s")
+    return dict(value=scene.particles.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{scene.particles.n} particles (the full workload), {steps} RK4 steps, "
+                       f"oracle in reference order on 1 thread of {platform.processor() or platform.machine()}",
+                seconds=dt)
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+
+    def log(msg):
+        if not args.quiet:
+            print(msg, file=sys.stderr, flush=True)
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the design for sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, scenes
+    t0 = time.perf_counter()
+    scene = scenes.rings_config2()
+    n = scene.particles.n
+    log(f"workload: {n} particles, dt = {scene.params.dt:.4e} s (setup {time.perf_counter() - t0:.1f} s)")
+
+    modes = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        log(f"mode {name}")
+        r = run_mode(scene, prec, args.steps, args.warmup, log)
+        r.pop("state")
+        modes[name] = r
+        log(f"  {r['particle_steps_per_s']:.4e} particle-steps/s ({r['ms_per_step']:.3f} ms/step), "
+            f"graph {r['particle_steps_per_s_graph']:.4e}, pair kernel {r['pair_kernel_ms']:.3f} ms")
+
+    acc = accuracy(scene, args.accuracy_steps, log) if args.accuracy_steps > 0 else None
+    if acc:
+        log(f"  mixed vs fp64 after {acc['steps']} steps: pos {acc['rel_l2_pos']:.3e} "
+            f"stress {acc['rel_l2_stress']:.3e}")
+
+    head = modes["mixed"]
+    achieved = ALG_BYTES_PER_PARTICLE_STAGE * n / (head["pair_kernel_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                traffic=None, kernel="k_ul_pairs32",
+                algorithmic_bytes_per_launch=ALG_BYTES_PER_PARTICLE_STAGE * n)
+    tr = traffic_from_profiles()
+    if tr and tr.get("kernel") == roof["kernel"] and tr.get("particles") == n:
+        roof["traffic"] = tr.get("hbm_bytes_per_launch")
+
+    cpu = None
+    if rank == 0 and not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, args.cpu_steps, log)
+
+    line = {
+        "metric": METRIC,
+        "value": head["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "ring collision, dx=6.6e-5 m, 1,009,204 particles, shuffled (seed 0), "
+                               "N(0, 0.01 m/s) jitter, mixed precision", "particles": n,
+                   "dt_s": scene.params.dt, "mode": "mixed"},
+        "modes": {k: {kk: v for kk, v in r.items()} for k, r in modes.items()},
+        "accuracy": acc,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/* mfree_iwf_amd -- MI355X (gfx950) implementation of mfree_iwf's SPH
+ * particle-interaction hot path, behind a C ABI.
+ *
+ * The reference exposes this path only as C++ templates (no FFI of its own):
+ *   body<T>(T** particles, n, simulation_data, dt, bc)   src/body.h:216-233
+ *   body<T>::add_action(void(*)(body<T>&))               src/body.h:219-221
+ *   body<T>::step() -> rk4<T>::step -> step_strong       src/body.h:223-225,
+ *                                                        src/rk4.h:71-138
+ *   compute_time_derivatives (UL / TL overloads)         src/body.h:110-148
+ *   grid<T> cell list -> Verlet lists                    src/grid.h:35-177
+ *   find_neighbors (TL init)                             src/benchmarks/neighbor_search.h:116-147
+ *   precomp_randles_libersky (TL init)                   src/precomp_shape_functions.h:231-290
+ *   boundary_conditions<T>                               src/boundary_conditions.h:27-51
+ *   simulation_time singleton (time, dt)                 src/simulation_time.cpp:20-38
+ * Each entry point below names the reference interface it replaces.  The
+ * C++ surface with the reference's own names (body<T>, rk4<T>, particle
+ * classes, actions) is the header-only adapter in include/mfree/, which is
+ * written on top of these functions.
+ *
+ * Conventions
+ *  - A context owns all device memory and one HIP stream (created on the
+ *    device given in the options).  Host buffers are caller-owned and used
+ *    synchronously.  A context is not thread-safe; distinct contexts may be
+ *    used from different threads.
+ *  - Particle arrays are SoA, in each body's idx order: element i of a
+ *    body's arrays is the particle with idx == i (the reference's drivers
+ *    create particle i with idx i, src/benchmarks/rings.cpp:44-52).
+ *  - Errors are returned as negative mfree_status codes; they replace the
+ *    reference's asserts (src/grid.h:114,167-168,
+ *    src/benchmarks/neighbor_search.h:142-143) and its FE_INVALID /
+ *    FE_OVERFLOW traps (src/mfree_iwf.cpp:40).
+ *  - There is no CPU fallback: without a usable gfx950 device every compute
+ *    entry point returns MFREE_ERR_HIP.
+ */
+#ifndef MFREE_IWF_AMD_H_
+#define MFREE_IWF_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFREE_ABI_VERSION 1
+
+typedef int32_t mfree_status;
+#define MFREE_OK 0
+#define MFREE_ERR_ARG (-1)          /* bad argument / call order */
+#define MFREE_ERR_HIP (-2)          /* HIP runtime error or no device */
+#define MFREE_ERR_NBH_OVERFLOW (-3) /* >= max_nbh neighbours (src/grid.h:167) */
+#define MFREE_ERR_NO_NEIGHBOR (-4)  /* particle without neighbour (src/grid.h:168) */
+#define MFREE_ERR_NONFINITE (-5)    /* NaN/Inf in a derivative (src/mfree_iwf.cpp:40) */
+#define MFREE_ERR_CELLS (-6)        /* num_cell >= n (src/grid.h:114) */
+#define MFREE_ERR_PIPELINE (-7)     /* action list not recognised */
+#define MFREE_ERR_CAPACITY (-9)     /* caller buffer too small */
+#define MFREE_ERR_UNSUPPORTED (-10) /* e.g. non-zero contact forces */
+
+/* Methods (selected in the reference by the particle type, src/body.h:60-148). */
+#define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
+#define MFREE_METHOD_TL_RL 1       /* particle_tl, tensile_tl action list */
+
+/* Arithmetic of the pair sums. */
+#define MFREE_PRECISION_FP64 0  /* everything FP64 (reference arithmetic) */
+#define MFREE_PRECISION_MIXED 1 /* FP32 pair sums with compensated accumulation,
+                                   FP64 per-particle algebra and RK4 state */
+
+/* Per-body constants: simulation_data (src/simulation_data.h:25-100) + dt.
+ * Mirrors physical_constants(nu, E, rho0), constants_monaghan(wdeltap,
+ * corr_exp, eps), constants_artificial_viscosity(alpha, beta, eta),
+ * correction_constants(..., xsph_eps) and rk4(n, dt). */
+typedef struct mfree_params {
+  double E, nu, rho0;
+  double wdeltap, corr_exp, mghn_eps;
+  double alpha, beta, eta;
+  double xsph_eps;
+  double dt;   /* time step of this body (rk4 ctor, src/rk4.h:242-243) */
+  double time; /* initial simulation time of this body */
+} mfree_params;
+
+typedef struct mfree_options {
+  int32_t method;           /* MFREE_METHOD_* (all bodies of a context) */
+  int32_t precision;        /* MFREE_PRECISION_* */
+  int32_t device;           /* HIP device ordinal */
+  uint32_t max_nbh;         /* reference MAX_NBH check (src/particle.h:31); 0 = off */
+  int32_t use_graph;        /* capture RK4 steps in a HIP graph */
+  int32_t deterministic;    /* 1: within-cell order by idx (bit-reproducible) */
+  double tl_h_search;       /* TL: h passed to find_neighbors; <= 0: per-particle h */
+} mfree_options;
+
+/* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
+ * (reference configuration) are read for TL bodies; fcx, fcy (contact
+ * forces) may be NULL and must otherwise be zero. */
+typedef struct mfree_particles {
+  double *x, *y, *X, *Y;
+  double *m, *rho, *h;
+  double *vx, *vy;
+  double *Sxx, *Sxy, *Syy;
+  double *fcx, *fcy;
+} mfree_particles;
+
+typedef struct mfree_ctx mfree_ctx;
+
+/* Library identity. */
+int32_t mfree_abi_version(void);
+const char* mfree_status_string(mfree_status s);
+/* The reference's kernel function cubic_spline (src/kernel.h:29,
+ * src/kernel.cpp:20-55), host side: out = {w, w_x, w_y}.  Drivers use it for
+ * the artificial-stress normalisation wdeltap = W(dx) (src/benchmarks/rings.cpp:74-75). */
+void mfree_cubic_spline(double xi, double yi, double xj, double yj, double h, double out[3]);
+
+/* Number of usable HIP devices (0 on a machine without a GPU). */
+int32_t mfree_device_count(void);
+void mfree_default_options(mfree_options* o);
+
+/* Context = the set of bodies a driver steps (src/mfree_iwf.cpp:45-89 keeps
+ * them in a std::vector<body<T>>).  Allocates nothing on the device yet. */
+mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out);
+void mfree_destroy(mfree_ctx* ctx);
+const char* mfree_last_error(const mfree_ctx* ctx);
+
+/* Replaces body<T>::body(particles, n, simulation_data, dt, bc)
+ * (src/body.h:216-233): registers a body and copies its particles in.
+ * Returns the body id (0, 1, ...) in *body_id.  Bodies can be added until
+ * the first compute call. */
+mfree_status mfree_add_body(mfree_ctx* ctx, const mfree_params* p, const mfree_particles* host,
+                            uint32_t n, uint32_t* body_id);
+
+/* Replaces boundary_conditions<T>::add_boundary_condition (src/boundary_conditions.h:32-34)
+ * with the TL strip functions of src/benchmarks/tensile.cpp:20-42:
+ * idx_left: x = X - v_bc*t, vx = -v_bc; idx_right: x = X + v_bc*t, vx = +v_bc;
+ * y = Y, vy = 0, vx_t = vy_t = 0.  TL bodies only. */
+mfree_status mfree_set_strip_bc(mfree_ctx* ctx, uint32_t body, const uint32_t* idx_left, uint32_t nl,
+                                const uint32_t* idx_right, uint32_t nr, double v_bc);
+
+/* Replaces find_neighbors + precomp_randles_libersky (src/benchmarks/tensile.cpp:271-272):
+ * builds the frozen TL neighbour lists and corrected kernel gradients on the
+ * device from the current positions.  Called implicitly by the first
+ * step of a TL context. */
+mfree_status mfree_tl_precompute(mfree_ctx* ctx);
+
+/* Overwrites / reads back a body's state (idx order).  Replaces direct
+ * access to *particles[i] (src/body.h:196-198, get_particles). */
+mfree_status mfree_set_state(mfree_ctx* ctx, uint32_t body, const mfree_particles* host);
+mfree_status mfree_get_state(mfree_ctx* ctx, uint32_t body, mfree_particles* host);
+mfree_status mfree_get_time(mfree_ctx* ctx, uint32_t body, double* time);
+
+/* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for
+ * every body: nsteps RK4 steps.  Each body advances its own time by its dt
+ * per step. */
+mfree_status mfree_step(mfree_ctx* ctx, uint32_t nsteps);
+
+/* One derivative evaluation of every body's current state without
+ * stepping: compute_time_derivatives (src/body.h:110-148).  Parity helper.
+ * d8 (n_total*8, bodies concatenated, idx order): x_t, y_t, vx_t, vy_t,
+ * rho_t, Sxx_t, Sxy_t, Syy_t.  aux may be NULL; otherwise n_total*16:
+ * UL: quad_weight, p, Rxx, Rxy, Ryy, Sxx_x, Sxy_y, Sxy_x, Syy_y, vx_x, vx_y,
+ *     vy_x, vy_y, num_nbh, 0, 0
+ * TL: Hxx, Hxy, Hyx, Hyy, Fdotxx, Fdotxy, Fdotyx, Fdotyy, Pxx, Pxy, Pyx, Pyy,
+ *     Pxx_x, Pxy_x, Pyx_y, Pyy_y */
+mfree_status mfree_eval_derivatives(mfree_ctx* ctx, double* d8, double* aux);
+
+/* The Verlet lists the reference would build for a body's current state
+ * (src/grid.h:117-177, predicate r^2 <= 4h_i^2 + 1e-6), in idx space,
+ * rows by idx, each row in the reference's emission order.  row_ptr has
+ * n+1 entries; col has cap entries; *npairs receives the total (also when
+ * cap is too small, then MFREE_ERR_CAPACITY).  For TL bodies: the frozen
+ * lists (find_neighbors order). */
+mfree_status mfree_get_pairs(mfree_ctx* ctx, uint32_t body, uint32_t* row_ptr, uint32_t* col, uint64_t cap,
+                             uint64_t* npairs);
+
+/* Timing of the dominant (pair) kernel, measured with HIP events on the
+ * context's stream while enabled (HIP graphs are not used while enabled). */
+mfree_status mfree_profile_enable(mfree_ctx* ctx, int32_t on);
+mfree_status mfree_profile_read(mfree_ctx* ctx, double* pair_ms_total, uint64_t* pair_launches,
+                                double* step_ms_total, uint64_t* steps);
+
+/* Total particle count and per-body counts. */
+uint64_t mfree_num_particles(const mfree_ctx* ctx);
+uint32_t mfree_num_bodies(const mfree_ctx* ctx);
+
+/* ---------------------------------------------------------------------
+ * Scene generators: parametrised re-creations of the reference drivers.
+ */
+
+/* rings_monaghan (src/benchmarks/rings.cpp:20-99): square lattice of
+ * nbox x nbox points at -ro + i*dx, dx = 2*ro/(nbox-1), kept if
+ * ri <= |p| < ro, each emitted twice, interleaved (left ring at x - centre:
+ * even idx; right ring at x + centre: odd idx).  rho = rho0, h = hdx*dx,
+ * m = dx*dx*rho0, vx = +v0 if x < 0 else -v0.  Optionally shuffled
+ * (SplitMix64(seed) + Fisher-Yates; idx = new position) and with
+ * N(0, jitter)-distributed velocity jitter (Box-Muller, same stream). */
+typedef struct mfree_rings_spec {
+  double ri, ro, centre;
+  uint32_t nbox;
+  double hdx, rho0, v0;
+  int32_t shuffle;
+  uint64_t seed;
+  double jitter;
+} mfree_rings_spec;
+
+/* Returns the particle count (0 on bad spec). */
+uint32_t mfree_scene_rings_count(const mfree_rings_spec* spec);
+/* Fills caller arrays (capacity cap); X, Y get the initial positions.
+ * Returns MFREE_ERR_CAPACITY if cap is too small. */
+mfree_status mfree_scene_rings(const mfree_rings_spec* spec, mfree_particles* out, uint32_t cap,
+                               uint32_t* n_out);
+
+/* Plate with a central hole for the TL method (geometry of the tension
+ * benchmark; lattice like src/benchmarks/tensile.cpp:232-257): points
+ * (i*dx, j*dx), i < nx = round(L/dx)+1, j < ny = round(W/dx)+1, dropped if
+ * closer than r_hole to the plate centre.  rho = rho0, h = hdx*dx,
+ * m = dx*dx*rho0, X = x, Y = y.  Strips: i < strip (left), i >= nx-strip
+ * (right) -> idx lists. */
+typedef struct mfree_plate_spec {
+  double L, W, r_hole, dx, hdx, rho0;
+  uint32_t strip;
+} mfree_plate_spec;
+
+uint32_t mfree_scene_plate_count(const mfree_plate_spec* spec, uint32_t* n_left, uint32_t* n_right);
+mfree_status mfree_scene_plate(const mfree_plate_spec* spec, mfree_particles* out, uint32_t cap,
+                               uint32_t* idx_left, uint32_t* idx_right, uint32_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFREE_IWF_AMD_H_ */

@@ -1,0 +1,300 @@
+// Cell-list construction on gfx950: the reference's grid (src/grid.h) as a
+// counting sort in HBM.
+//
+//   bbox      two-level min/max reduction per body (src/grid.h:83-115)
+//   geometry  exactly the reference's nudge / cell side / ceil (src/grid.h:100-114)
+//   hash      ix*ny + iy with (unsigned) truncation (src/grid.h:35-41),
+//             offset by the body's cell range -> one key space for all bodies
+//   count     atomic histogram of keys, rank of each particle in its cell
+//   scan      exclusive prefix sum -> cell_start (src/grid.h:48-81: the
+//             reference's "first index per cell, empty cells fixed from the
+//             right" table is exactly this exclusive scan)
+//   scatter   perm[cell_start[key] + rank] = particle
+//   cell sort within each cell, particles ordered by idx, so the order no
+//             longer depends on atomic timing (deterministic sums)
+#include "kernels.h"
+
+#include <float.h>
+
+namespace mfree {
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+__device__ inline T wave_reduce_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <typename T>
+__device__ inline T wave_reduce_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One workgroup per chunk: partial {min x, min y, max x, max y, max h}.
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict__ ch,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ h,
+                                                         double* __restrict__ part) {
+  const Chunk c = ch[blockIdx.x];
+  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
+  for (uint32_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+    double xi = x[i], yi = y[i];
+    mnx = fmin(xi, mnx);
+    mny = fmin(yi, mny);
+    mxx = fmax(xi, mxx);
+    mxy = fmax(yi, mxy);
+    mxh = fmax(h[i], mxh);
+  }
+  mnx = wave_reduce_min(mnx);
+  mny = wave_reduce_min(mny);
+  mxx = wave_reduce_max(mxx);
+  mxy = wave_reduce_max(mxy);
+  mxh = wave_reduce_max(mxh);
+  __shared__ double sh[5][kBlock / 64];
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    sh[0][w] = mnx;
+    sh[1][w] = mny;
+    sh[2][w] = mxx;
+    sh[3][w] = mxy;
+    sh[4][w] = mxh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; k++) {
+      sh[0][0] = fmin(sh[0][0], sh[0][k]);
+      sh[1][0] = fmin(sh[1][0], sh[1][k]);
+      sh[2][0] = fmax(sh[2][0], sh[2][k]);
+      sh[3][0] = fmax(sh[3][0], sh[3][k]);
+      sh[4][0] = fmax(sh[4][0], sh[4][k]);
+    }
+    double* o = part + 5 * (size_t)blockIdx.x;
+    for (int k = 0; k < 5; k++) o[k] = sh[k][0];
+  }
+}
+
+// One thread per body: combine the chunk partials and derive the cell
+// geometry with the reference's arithmetic (src/grid.h:100-114).
+__global__ void k_bbox_final(uint32_t nscene, const uint32_t* __restrict__ chunk_first,
+                             const SceneConst* __restrict__ sc, const double* __restrict__ part,
+                             double kernel_width, SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscene) return;
+  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
+  for (uint32_t c = chunk_first[s]; c < chunk_first[s + 1]; c++) {
+    const double* p = part + 5 * (size_t)c;
+    mnx = fmin(p[0], mnx);
+    mny = fmin(p[1], mny);
+    mxx = fmax(p[2], mxx);
+    mxy = fmax(p[3], mxy);
+    mxh = fmax(p[4], mxh);
+  }
+  mnx -= 1e-6;
+  mny -= 1e-6;
+  mxx += 1e-6;
+  mxy += 1e-6;
+  double dx = mxh * kernel_width;
+  double lx = mxx - mnx;
+  double ly = mxy - mny;
+  double fnx = ceil(lx / dx), fny = ceil(ly / dx);
+  SceneGeom g;
+  g.bbmin_x = mnx;
+  g.bbmin_y = mny;
+  g.cell = dx;
+  g.err = 0;
+  // num_cell < n is asserted by the reference (src/grid.h:114); the same
+  // bound keeps every key inside the body's cell range here.
+  double ncell = fnx * fny;
+  if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n)) {
+    g.nx = 1;
+    g.ny = 1;
+    g.num_cell = 1;
+    g.err = EB_CELLS;
+    atomicOr(err, EB_CELLS);
+  } else {
+    g.nx = (unsigned int)fnx;
+    g.ny = (unsigned int)fny;
+    g.num_cell = g.nx * g.ny;
+  }
+  geom[s] = g;
+}
+
+// Per particle (storage order): key = cell_offset + ix*ny + iy, and its rank
+// within the cell from an atomic histogram.
+__global__ __launch_bounds__(kBlock) void k_hash_count(uint32_t n, const uint32_t* __restrict__ scene,
+                                                       const double* __restrict__ x, const double* __restrict__ y,
+                                                       const SceneConst* __restrict__ sc,
+                                                       const SceneGeom* __restrict__ geom,
+                                                       uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                       uint32_t* __restrict__ count, uint32_t* __restrict__ err) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = scene[i];
+  const SceneGeom g = geom[s];
+  double fx = (x[i] - g.bbmin_x) / g.cell;
+  double fy = (y[i] - g.bbmin_y) / g.cell;
+  uint32_t ix, iy;
+  bool ok = (fx >= 0.0) && (fy >= 0.0) && (fx < (double)g.nx) && (fy < (double)g.ny);
+  if (ok) {
+    ix = (unsigned int)fx;
+    iy = (unsigned int)fy;
+  } else {
+    ix = 0;
+    iy = 0;
+    if (!g.err) atomicOr(err, EB_CELLS);
+  }
+  uint32_t k = sc[s].cell_offset + ix * g.ny + iy;
+  key[i] = k;
+  rank[i] = atomicAdd(&count[k], 1u);
+}
+
+// ---- exclusive scan over u32 (cell counts -> cell starts) ---------------
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  int l = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* excl) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  uint32_t inc = wave_incl_scan(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int k = 0; k < kBlock / 64; k++) {
+    if (k < w) base += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  *excl = base + inc - v;
+  return tot;
+}
+
+// Tile sums: each workgroup sums kScanTile elements.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n,
+                                                       uint32_t* __restrict__ tsum) {
+  uint32_t base = blockIdx.x * kScanTile;
+  uint32_t v = 0;
+  for (uint32_t k = 0; k < kScanTile / kBlock; k++) {
+    uint32_t i = base + threadIdx.x * (kScanTile / kBlock) + k;
+    if (i < n) v += in[i];
+  }
+  uint32_t ex;
+  uint32_t tot = block_excl_scan(v, &ex);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the tile sums in place.
+__global__ __launch_bounds__(kBlock) void k_scan_tilesums(uint32_t* __restrict__ tsum, uint32_t nt) {
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < nt; b += kBlock) {
+    uint32_t i = b + threadIdx.x;
+    uint32_t v = i < nt ? tsum[i] : 0;
+    uint32_t ex;
+    uint32_t tot = block_excl_scan(v, &ex);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+// out[i] = tile base + exclusive scan within the tile; out[n] = total.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                       const uint32_t* __restrict__ tsum,
+                                                       uint32_t* __restrict__ out, uint32_t total) {
+  constexpr uint32_t per = kScanTile / kBlock;
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * per;
+  uint32_t vals[per];
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < per; k++) {
+    uint32_t i = base + k;
+    vals[k] = i < n ? in[i] : 0;
+    v += vals[k];
+  }
+  uint32_t ex;
+  block_excl_scan(v, &ex);
+  uint32_t run = tsum[blockIdx.x] + ex;
+#pragma unroll
+  for (uint32_t k = 0; k < per; k++) {
+    uint32_t i = base + k;
+    if (i < n) out[i] = run;
+    run += vals[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ rank,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    uint32_t* __restrict__ perm) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[cell_start[key[i]] + rank[i]] = i;
+}
+
+// One thread per cell: insertion sort of the cell's entries by global id,
+// so that within a cell particles appear in idx order.
+__global__ __launch_bounds__(kBlock) void k_cell_sort(uint32_t ncell, const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ gid,
+                                                      uint32_t* __restrict__ perm) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  uint32_t a = cell_start[c], b = cell_start[c + 1];
+  if (b - a < 2) return;
+  for (uint32_t i = a + 1; i < b; i++) {
+    uint32_t p = perm[i];
+    uint32_t g = gid[p];
+    uint32_t j = i;
+    while (j > a) {
+      uint32_t q = perm[j - 1];
+      if (gid[q] <= g) break;
+      perm[j] = q;
+      j--;
+    }
+    perm[j] = p;
+  }
+}
+
+// ---- host launchers -------------------------------------------------------
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) {
+  if (n) hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n, kBlock)), dim3(kBlock), 0, st, p, v, n);
+}
+
+void launch_binning(hipStream_t st, const BinArgs& a) {
+  hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(nblk(a.nscene, 64)), dim3(64), 0, st, a.nscene, a.chunk_first, a.sc,
+                     a.part, 2.0, a.geom, a.err);
+  launch_fill_u32(st, a.count, 0u, a.ncell_cap);
+  hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
+                     a.geom, a.key, a.rank, a.count, a.err);
+  uint32_t nt = nblk(a.ncell_cap, kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum);
+  hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(kBlock), 0, st, a.tsum, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum, a.cell_start,
+                     a.n);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
+                     a.cell_start, a.perm);
+  if (a.deterministic)
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(a.ncell_cap, kBlock)), dim3(kBlock), 0, st, a.ncell_cap,
+                       a.cell_start, a.gid, a.perm);
+}
+
+}  // namespace mfree

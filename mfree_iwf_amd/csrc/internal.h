@@ -1,0 +1,62 @@
+// Internal types shared by the host orchestration (capi.hip) and the
+// gfx950 kernels (grid.hip, ul.hip, tl.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfree {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr uint32_t kChunk = 8192;    // particles per bbox-reduction chunk
+constexpr uint32_t kScanTile = 1024; // elements per scan workgroup
+
+// error bits raised by kernels (mapped to mfree_status on the host)
+enum : uint32_t {
+  EB_CELLS = 1u,      // num_cell >= n, or a hash outside the grid
+  EB_NBH = 2u,        // >= max_nbh neighbours under the reference predicate
+  EB_NONFINITE = 4u,  // NaN / Inf in a derivative
+  EB_NONEIGH = 8u,    // particle without neighbour
+};
+
+// Per-body constants, device resident.  Derived constants are computed on
+// the host with the reference's formulas (src/simulation_data.cpp:32-47).
+struct SceneConst {
+  double rho0, K, G2, c0sq;        // G2 = 2*G, c0sq = c0*c0 with c0 = sqrt(K/rho0)
+  double wdeltap, corr_exp, mghn_eps;
+  double alpha, beta, eta, xsph_eps;
+  double dt, half_dt, dt6;         // dt6 = dt*1./6.
+  double time;                     // time at the start of the current step
+  double v_bc;                     // TL strip velocity
+  uint32_t offset, n, cell_offset, pow4;
+};
+
+// Per-body cell geometry of the current stage (src/grid.h:83-115).
+struct SceneGeom {
+  double bbmin_x, bbmin_y, cell;
+  uint32_t nx, ny, num_cell, err;
+};
+
+struct Chunk {
+  uint32_t scene, begin, end, pad;
+};
+
+// Sorted per-stage pair inputs, FP64 mode: one 128-byte record per particle.
+struct alignas(16) PairRec64 {
+  double vx, vy, rho, rho21;  // rho21 = 1/(rho*rho)
+  double qw, m, h, c;         // qw = m/rho, c = sqrt(K/rho)
+  double sxx, sxy, syy;       // sigma = S - p I
+  double Rxx, Rxy, Ryy;       // artificial stress
+  double pad0, pad1;
+};
+
+// Mixed mode: 64-byte record of FP32 values; positions are stored relative
+// to the particle's own cell corner (exact FP64 subtraction, then rounded).
+struct alignas(16) PairRec32 {
+  float vx, vy, rho, rho21;
+  float qw, m, h, c;
+  float sxx, sxy, syy, pad0;
+  float Rxx, Rxy, Ryy, pad1;
+};
+
+}  // namespace mfree

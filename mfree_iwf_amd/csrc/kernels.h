@@ -1,0 +1,92 @@
+// Launch interfaces of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include "internal.h"
+
+namespace mfree {
+
+// The 8 integrated fields of the reference's RK4 (src/rk4.h:52-69):
+// x, y, vx, vy, rho, Sxx, Sxy, Syy.
+struct Fields8 {
+  double* f[8];
+};
+
+struct BinArgs {
+  uint32_t n, nscene, nchunks, ncell_cap;
+  int deterministic;
+  const Chunk* chunks;
+  const uint32_t* chunk_first;  // nscene+1
+  const SceneConst* sc;
+  SceneGeom* geom;
+  const double *x, *y, *h;      // storage order
+  const uint32_t* scene;        // storage order
+  const uint32_t* gid;          // storage order
+  double* part;                 // 5 per chunk
+  uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
+  uint32_t* err;
+};
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
+void launch_binning(hipStream_t st, const BinArgs& a);
+
+// UL (Gray-Monaghan) stage.
+struct UlArgs {
+  uint32_t n;
+  int mixed;                    // 0: FP64 pair records, 1: FP32 records
+  const SceneConst* sc;
+  const SceneGeom* geom;
+  const uint32_t* perm;         // sorted position -> storage index
+  const uint32_t* key;          // storage order (cell key)
+  const uint32_t* cell_start;
+  // stage state in storage order (the y_n at stage 1, K otherwise)
+  Fields8 src;
+  const double *m, *h;
+  const uint32_t *scene, *gid;
+  // reorder (stage 1): copy y_n and constants into sorted storage
+  int reorder;
+  Fields8 dst;
+  double *m_dst, *h_dst;
+  uint32_t *scene_dst, *gid_dst;
+  // sorted per-stage records
+  uint2* meta;                  // {cell key, body}
+  double2* pos64;
+  PairRec64* rec64;
+  float2* pos32;
+  PairRec32* rec32;
+  // outputs
+  int stage;                    // 1..4: RK stage; 0: derivative evaluation only
+  int home_identity;            // storage index == sorted position
+  Fields8 y;                    // y_n (storage order), updated at stage 4
+  Fields8 K;                    // next stage state (storage order)
+  Fields8 acc;                  // RK accumulator (storage order)
+  double* d8;                   // stage 0: derivatives by global id
+  double* aux;                  // stage 0: diagnostics by global id (may be null)
+  uint32_t max_nbh;
+  uint32_t* err;
+};
+
+void launch_ul_prepass(hipStream_t st, const UlArgs& a);
+void launch_ul_pairs(hipStream_t st, const UlArgs& a);
+
+// Verlet-list emission in the reference's order (parity helper).
+struct PairListArgs {
+  uint32_t n;                   // particles of the body
+  uint32_t first;               // first sorted position of the body
+  const SceneConst* sc;
+  const SceneGeom* geom;
+  uint32_t scene;
+  const uint32_t* cell_start;
+  const uint32_t* perm;
+  const uint32_t* key;          // storage order
+  const uint32_t* gid;          // storage order
+  const double *x, *y, *h;      // storage order
+  uint32_t* row_count;          // by local idx
+  const uint32_t* row_ptr;      // by local idx (fill pass)
+  uint32_t* col;                // local idx of neighbours
+  int fill;
+};
+void launch_pair_list(hipStream_t st, const PairListArgs& a);
+
+__global__ void k_advance_time(uint32_t nscene, SceneConst* sc);
+
+}  // namespace mfree

@@ -1,0 +1,413 @@
+// Gray-Monaghan updated-Lagrangian SPH stage on gfx950.
+//
+// One derivative evaluation of the reference (src/body.h:125-148 with the
+// action list of src/benchmarks/rings.cpp:86-96) is three steps here:
+//
+//  k_ul_prepass   per particle, in cell-sorted order: the per-particle
+//                 actions that precede the pair sums -- quadrature weight
+//                 (src/riemann_integration.h:28-33), EOS (src/material.h:37-64),
+//                 Monaghan artificial stress (src/correctors.h:37-81) -- and a
+//                 gather of everything a neighbour contributes into one
+//                 128-byte record per particle (coalesced reads in the pair
+//                 pass).  At the first stage of a step it also moves the
+//                 particles' storage into the new sorted order, so storage
+//                 stays spatially coherent.
+//  k_ul_pairs     per particle: ONE pass over the 3x3 cell block evaluating
+//                 all four pair sums of the reference at once --
+//                 derive_stress_monaghan (src/derivatives.h:378-443),
+//                 derive_velocity (src/derivatives.h:32-66), artificial
+//                 viscosity (src/correctors.h:83-137) and XSPH
+//                 (src/correctors.h:139-169) -- then the per-particle
+//                 epilogue (Jaumann rate src/material.h:66-100, continuity
+//                 src/cont_mech.h:108-126, momentum :32-49, advection
+//                 :134-142) and the RK4 stage update (src/rk4.h:52-108).
+//
+// Each accumulator is summed over neighbours in the reference's order
+// (cells ni-major, nj-minor, particles within a cell in idx order), and the
+// FP64 path is compiled with -ffp-contract=off, so its arithmetic follows
+// the reference's expression order.  Pairs with q >= 2 (or j == i) add an
+// exact +-0 to every sum in the reference (W and grad W vanish, and the self
+// pair has v_ij = 0, grad W = 0), so they are skipped without changing a bit.
+#include "kernels.h"
+
+namespace mfree {
+
+namespace {
+
+// glm-ordered Jaumann stress rate (src/material.h:71-78): returns
+// S_t[0][0], S_t[0][1], S_t[1][1] of 2G(epsdot - 1/3 tr I) - S*omega + omega*S
+__device__ inline void jaumann(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
+                               double Sxy, double Syy, double& t00, double& t01, double& t11) {
+  const double e01 = 0.5 * (vx_y + vy_x);
+  const double w10 = 0.5 * (vy_x - vx_y);  // omega[0][1]
+  const double w01 = 0.5 * (vx_y - vy_x);  // omega[1][0]
+  const double tr = vx_x + vy_y;
+  const double s3 = 1. / 3. * tr;
+  const double D00 = vx_x - s3 * 1.;
+  const double D01 = e01 - s3 * 0.;
+  const double D11 = vy_y - s3 * 1.;
+  const double A00 = G2 * D00, A01 = G2 * D01, A11 = G2 * D11;
+  const double SW00 = Sxx * 0. + Sxy * w10;
+  const double SW01 = Sxy * 0. + Syy * w10;
+  const double SW11 = Sxy * w01 + Syy * 0.;
+  const double WS00 = 0. * Sxx + w01 * Sxy;
+  const double WS01 = w10 * Sxx + 0. * Sxy;
+  const double WS11 = w10 * Sxy + 0. * Syy;
+  t00 = (A00 - SW00) + WS00;
+  t01 = (A01 - SW01) + WS01;
+  t11 = (A11 - SW11) + WS11;
+}
+
+// Monaghan artificial stress of one particle (src/correctors.h:37-81).
+__device__ inline void art_stress(double sxx, double sxy, double syy, double rhoi, double eps, double& Rxx,
+                                  double& Rxy, double& Ryy) {
+  const double numer = 2. * sxy;
+  const double denom = sxx - syy + 0.;
+  const double theta = 0.5 * atan2(numer, denom);
+  const double ct = cos(theta);
+  const double sn = sin(theta);
+  const double ct2 = ct * ct;
+  const double sn2 = sn * sn;
+  const double rot_sxx = ct2 * sxx + 2.0 * ct * sn * sxy + sn2 * syy;
+  const double rot_syy = sn2 * sxx - 2.0 * ct * sn * sxy + ct2 * syy;
+  double rot_rxx = 0., rot_ryy = 0.;
+  if (rot_sxx > 0) rot_rxx = -eps * rot_sxx / (rhoi * rhoi);
+  if (rot_syy > 0) rot_ryy = -eps * rot_syy / (rhoi * rhoi);
+  Rxx = ct2 * rot_rxx + sn2 * rot_ryy;
+  Rxy = ct * sn * (rot_rxx - rot_ryy);
+  Ryy = sn2 * rot_rxx + ct2 * rot_ryy;
+}
+
+__device__ inline bool finite8(const double* d) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 8; k++) ok = ok && isfinite(d[k]);
+  return ok;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t p = a.perm[i];
+  const uint32_t s = a.scene[p];
+  const SceneConst* __restrict__ c = a.sc + s;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = a.src.f[k][p];
+  const double m = a.m[p], h = a.h[p];
+  const double x = v[0], y = v[1], vx = v[2], vy = v[3], rho = v[4], Sxx = v[5], Sxy = v[6], Syy = v[7];
+  const double qw = m / rho;
+  const double pr = c->c0sq * (rho - c->rho0);
+  const double sxx = Sxx - pr, syy = Syy - pr, sxy = Sxy;
+  double Rxx, Rxy, Ryy;
+  art_stress(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
+  const double rho21 = 1. / (rho * rho);
+  const double cs = sqrt(c->K / rho);
+  const uint32_t key = a.key[p];
+  a.meta[i] = make_uint2(key, s);
+  if (!a.mixed) {
+    a.pos64[i] = make_double2(x, y);
+    PairRec64 r;
+    r.vx = vx;
+    r.vy = vy;
+    r.rho = rho;
+    r.rho21 = rho21;
+    r.qw = qw;
+    r.m = m;
+    r.h = h;
+    r.c = cs;
+    r.sxx = sxx;
+    r.sxy = sxy;
+    r.syy = syy;
+    r.Rxx = Rxx;
+    r.Rxy = Rxy;
+    r.Ryy = Ryy;
+    r.pad0 = 0.;
+    r.pad1 = 0.;
+    a.rec64[i] = r;
+  } else {
+    const SceneGeom g = a.geom[s];
+    const uint32_t local = key - c->cell_offset;
+    const uint32_t ix = local / g.ny, iy = local - ix * g.ny;
+    const double cx = g.bbmin_x + (double)ix * g.cell;
+    const double cy = g.bbmin_y + (double)iy * g.cell;
+    a.pos32[i] = make_float2((float)(x - cx), (float)(y - cy));
+    PairRec32 r;
+    r.vx = (float)vx;
+    r.vy = (float)vy;
+    r.rho = (float)rho;
+    r.rho21 = (float)rho21;
+    r.qw = (float)qw;
+    r.m = (float)m;
+    r.h = (float)h;
+    r.c = (float)cs;
+    r.sxx = (float)sxx;
+    r.sxy = (float)sxy;
+    r.syy = (float)syy;
+    r.pad0 = 0.f;
+    r.Rxx = (float)Rxx;
+    r.Rxy = (float)Rxy;
+    r.Ryy = (float)Ryy;
+    r.pad1 = 0.f;
+    a.rec32[i] = r;
+  }
+  if (a.reorder) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a.dst.f[k][i] = v[k];
+    a.m_dst[i] = m;
+    a.h_dst[i] = h;
+    a.scene_dst[i] = s;
+    a.gid_dst[i] = a.gid[p];
+  }
+}
+
+// Epilogue + RK4 stage update shared by both precisions.  d = derivatives.
+__device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t home, double* d,
+                                 const double* auxv) {
+  if (!finite8(d)) atomicOr(a.err, EB_NONFINITE);
+  if (a.stage == 0) {
+    const uint32_t g = a.gid[home];
+    double* o = a.d8 + 8 * (size_t)g;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = d[k];
+    if (a.aux) {
+      double* x = a.aux + 16 * (size_t)g;
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = auxv[k];
+    }
+    return;
+  }
+  if (a.stage == 1) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home];
+      a.acc.f[k][home] = d[k];
+      a.K.f[k][home] = yk + d[k] * c->half_dt;
+    }
+  } else if (a.stage == 2 || a.stage == 3) {
+    const double dts = a.stage == 2 ? c->half_dt : c->dt;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home];
+      a.acc.f[k][home] = a.acc.f[k][home] + 2 * d[k];
+      a.K.f[k][home] = yk + d[k] * dts;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home];
+      a.y.f[k][home] = yk + c->dt6 * (a.acc.f[k][home] + d[k]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint2 mt = a.meta[i];
+  const uint32_t s = mt.y;
+  const SceneConst* __restrict__ c = a.sc + s;
+  const SceneGeom g = a.geom[s];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+
+  const double2 pi = a.pos64[i];
+  const PairRec64 ri = a.rec64[i];
+  const double hi = ri.h;
+  const double h1 = 1. / hi;
+  const double fac = 10 * (M_1_PI) / 7.0 * h1 * h1;
+  const double radius2 = hi * hi * 2 * 2;
+  // r2 > cut2 implies q = sqrt(r2)/h >= 2 under any rounding of the
+  // reference's evaluation, so the exact test below decides the rest.
+  const double cut2 = 4.0000004 * (hi * hi);
+  const double wdeltap = c->wdeltap, corr_exp = c->corr_exp;
+  const bool use_as = wdeltap > 0;
+  const bool pow4 = c->pow4 != 0;
+  const double alpha = c->alpha, beta = c->beta, eta = c->eta, xeps = c->xsph_eps;
+  const double sxxi_r = ri.sxx * ri.rho21, sxyi_r = ri.sxy * ri.rho21, syyi_r = ri.syy * ri.rho21;
+
+  double Sxx_x = 0., Sxy_y = 0., Sxy_x = 0., Syy_y = 0.;
+  double vx_x = 0., vx_y = 0., vy_x = 0., vy_y = 0.;
+  double ax = 0., ay = 0., xs = 0., ys = 0.;
+  uint32_t cnt = 0;
+
+  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
+    const uint32_t rb = c->cell_offset + ni * g.ny;
+    const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
+    for (uint32_t j = ja; j < jb; j++) {
+      const double2 pj = a.pos64[j];
+      const double xij = pi.x - pj.x;
+      const double yij = pi.y - pj.y;
+      const double r2 = xij * xij + yij * yij;
+      cnt += (r2 <= radius2 + 1e-6) ? 1u : 0u;
+      if (j == i || r2 > cut2) continue;
+      const double rij = sqrt(r2);
+      const double q = rij * h1;
+      if (q >= 2.) continue;
+      // cubic spline, src/kernel.cpp:20-55
+      double w, wx = 0., wy = 0.;
+      if (q >= 1.) {
+        w = fac * (0.25 * (2 - q) * (2 - q) * (2 - q));
+        if (rij > 1e-12) {
+          const double der = -0.75 * (2 - q) * (2 - q) * h1 / rij;
+          wx = xij * der * fac;
+          wy = yij * der * fac;
+        }
+      } else {
+        w = fac * (1 - 1.5 * q * q * (1 - 0.5 * q));
+        if (rij > 1e-12) {
+          const double der = -3.0 * q * (1 - 0.75 * q) * h1 / rij;
+          wx = xij * der * fac;
+          wy = yij * der * fac;
+        }
+      }
+      const PairRec64 rj = a.rec64[j];
+      // derive_stress_monaghan
+      double Rxx = 0., Rxy = 0., Ryy = 0.;
+      if (use_as) {
+        double fab = w / wdeltap;
+        fab = pow4 ? (fab * fab) * (fab * fab) : pow(fab, corr_exp);
+        Rxx = fab * (ri.Rxx + rj.Rxx);
+        Rxy = fab * (ri.Rxy + rj.Rxy);
+        Ryy = fab * (ri.Ryy + rj.Ryy);
+      }
+      const double mj = rj.m;
+      Sxx_x += mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx;
+      Sxy_y += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy;
+      Sxy_x += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx;
+      Syy_y += mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy;
+      // derive_velocity
+      vx_x += (rj.vx - ri.vx) * wx * rj.qw;
+      vx_y += (rj.vx - ri.vx) * wy * rj.qw;
+      vy_x += (rj.vy - ri.vy) * wx * rj.qw;
+      vy_y += (rj.vy - ri.vy) * wy * rj.qw;
+      // correctors_mghn_artificial_viscosity
+      const double vijx = ri.vx - rj.vx;
+      const double vijy = ri.vy - rj.vy;
+      const double vp = vijx * xij + vijy * yij;
+      if (!(vp >= 0.)) {
+        const double cij = 0.5 * (ri.c + rj.c);
+        const double hij = 0.5 * (hi + rj.h);
+        const double rhoij = 0.5 * (ri.rho + rj.rho);
+        const double muij = (hij * vp) / (r2 + eta * eta * hij * hij);
+        const double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+        ax += -mj * piij * wx;
+        ay += -mj * piij * wy;
+      }
+      // correctors_xsph (mass of particle i, src/correctors.h:163)
+      const double rhoij = 0.5 * (ri.rho + rj.rho);
+      const double fx = -xeps * w * ri.m / rhoij;
+      xs += fx * vijx;
+      ys += fx * vijy;
+    }
+  }
+  if (a.max_nbh && cnt >= a.max_nbh) atomicOr(a.err, EB_NBH);
+  if (cnt == 0) atomicOr(a.err, EB_NONEIGH);
+
+  const uint32_t home = a.home_identity ? i : a.perm[i];
+  const double rhoi = ri.rho;
+  Sxx_x = Sxx_x * rhoi;
+  Sxy_y = Sxy_y * rhoi;
+  Sxy_x = Sxy_x * rhoi;
+  Syy_y = Syy_y * rhoi;
+  const double Sxx = a.src.f[5][home], Sxy = a.src.f[6][home], Syy = a.src.f[7][home];
+  double t00, t01, t11;
+  jaumann(c->G2, vx_x, vx_y, vy_x, vy_y, Sxx, Sxy, Syy, t00, t01, t11);
+  double d[8];
+  d[0] = xs + ri.vx;
+  d[1] = ys + ri.vy;
+  d[2] = ax + (1. / rhoi * (Sxx_x + Sxy_y) + 0. / ri.m);
+  d[3] = ay + (1. / rhoi * (Sxy_x + Syy_y) + 0. / ri.m);
+  d[4] = 0. - rhoi * (vx_x + vy_y);
+  d[5] = 0. + t00;
+  d[6] = 0. + t01;
+  d[7] = 0. + t11;
+  double auxv[16];
+  if (a.stage == 0 && a.aux) {
+    auxv[0] = ri.qw;
+    auxv[1] = c->c0sq * (rhoi - c->rho0);
+    auxv[2] = ri.Rxx;
+    auxv[3] = ri.Rxy;
+    auxv[4] = ri.Ryy;
+    auxv[5] = Sxx_x;
+    auxv[6] = Sxy_y;
+    auxv[7] = Sxy_x;
+    auxv[8] = Syy_y;
+    auxv[9] = vx_x;
+    auxv[10] = vx_y;
+    auxv[11] = vy_x;
+    auxv[12] = vy_y;
+    auxv[13] = (double)cnt;
+    auxv[14] = 0.;
+    auxv[15] = 0.;
+  }
+  ul_finish(a, c, home, d, auxv);
+}
+
+// Time advance of every body (src/rk4.h:107, simulation_time::increment_time).
+__global__ void k_advance_time(uint32_t nscene, SceneConst* sc) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nscene) sc[s].time = sc[s].time + sc[s].dt;
+}
+
+// The reference's Verlet list of one body (src/grid.h:117-177): count pass
+// (fill = 0) and fill pass (fill = 1), in idx space.
+__global__ __launch_bounds__(kBlock) void k_pair_list(PairListArgs a) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  const uint32_t i = a.first + t;  // sorted position
+  const uint32_t p = a.perm[i];
+  const SceneConst* __restrict__ c = a.sc + a.scene;
+  const SceneGeom g = a.geom[a.scene];
+  const uint32_t local = a.key[p] - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const int low_i = (int)gi - 1 < 0 ? 0 : (int)gi - 1;
+  const int low_j = (int)gj - 1 < 0 ? 0 : (int)gj - 1;
+  const int high_i = gi + 2 > g.nx ? g.nx : gi + 2;
+  const int high_j = gj + 2 > g.ny ? g.ny : gj + 2;
+  const double hi = a.h[p], xi = a.x[p], yi = a.y[p];
+  const double radius2 = hi * hi * 2 * 2;
+  const uint32_t row = a.gid[p] - c->offset;
+  uint32_t o = a.fill ? a.row_ptr[row] : 0, cnt = 0;
+  for (int ni = low_i; ni < high_i; ni++) {
+    for (int nj = low_j; nj < high_j; nj++) {
+      const uint32_t cell = c->cell_offset + (uint32_t)ni * g.ny + (uint32_t)nj;
+      for (uint32_t j = a.cell_start[cell]; j < a.cell_start[cell + 1]; j++) {
+        const uint32_t q = a.perm[j];
+        const double xij = xi - a.x[q];
+        const double yij = yi - a.y[q];
+        const double r2 = xij * xij + yij * yij;
+        if (r2 <= radius2 + 1e-6) {
+          if (a.fill) a.col[o + cnt] = a.gid[q] - c->offset;
+          cnt++;
+        }
+      }
+    }
+  }
+  if (!a.fill) a.row_count[row] = cnt;
+}
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+void launch_ul_prepass(hipStream_t st, const UlArgs& a) {
+  hipLaunchKernelGGL(k_ul_prepass, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a);
+}
+
+void launch_ul_pairs32(hipStream_t st, const UlArgs& a);
+
+void launch_ul_pairs(hipStream_t st, const UlArgs& a) {
+  if (a.mixed)
+    launch_ul_pairs32(st, a);
+  else
+    hipLaunchKernelGGL(k_ul_pairs64, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a);
+}
+
+void launch_pair_list(hipStream_t st, const PairListArgs& a) {
+  hipLaunchKernelGGL(k_pair_list, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a);
+}
+
+}  // namespace mfree

@@ -1,0 +1,131 @@
+"""Pythonic handle on one mfree context (a set of bodies stepped together).
+
+Thin wrapper over the C ABI (include/mfree_iwf_amd.h); all compute runs in
+the gfx950 library.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from .scenes import ParticleSet
+
+
+class Solver:
+    def __init__(self, method=_capi.METHOD_UL_MONAGHAN, precision=_capi.PRECISION_FP64, use_graph=True,
+                 deterministic=True, max_nbh=200, device=0, tl_h_search=0.0):
+        L = _capi.lib()
+        opt = _capi.Options()
+        L.mfree_default_options(C.byref(opt))
+        opt.method = method
+        opt.precision = precision
+        opt.use_graph = 1 if use_graph else 0
+        opt.deterministic = 1 if deterministic else 0
+        opt.max_nbh = max_nbh
+        opt.device = device
+        opt.tl_h_search = tl_h_search
+        h = C.c_void_p()
+        _capi.check(L.mfree_create(C.byref(opt), C.byref(h)), what="mfree_create")
+        self._h = h
+        self._L = L
+        self.sizes = []
+
+    # -- lifecycle
+    def close(self):
+        if self._h:
+            self._L.mfree_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st, what):
+        _capi.check(st, self._h, what)
+
+    # -- bodies
+    def add_body(self, particles: ParticleSet, params: _capi.Params) -> int:
+        cp = particles.to_c()
+        bid = C.c_uint32(0)
+        self._chk(self._L.mfree_add_body(self._h, C.byref(params), C.byref(cp), particles.n, C.byref(bid)),
+                  "mfree_add_body")
+        self.sizes.append(particles.n)
+        return bid.value
+
+    def set_strip_bc(self, body, left, right, v_bc):
+        left = np.ascontiguousarray(left, dtype=np.uint32)
+        right = np.ascontiguousarray(right, dtype=np.uint32)
+        self._chk(self._L.mfree_set_strip_bc(self._h, body, left.ctypes.data_as(_capi.u32ptr), len(left),
+                                             right.ctypes.data_as(_capi.u32ptr), len(right), v_bc),
+                  "mfree_set_strip_bc")
+
+    def tl_precompute(self):
+        self._chk(self._L.mfree_tl_precompute(self._h), "mfree_tl_precompute")
+
+    @property
+    def n_total(self):
+        return int(sum(self.sizes))
+
+    # -- stepping
+    def step(self, n=1):
+        self._chk(self._L.mfree_step(self._h, n), "mfree_step")
+
+    def state(self, body=0) -> ParticleSet:
+        ps = ParticleSet.zeros(self.sizes[body])
+        cp = ps.to_c()
+        self._chk(self._L.mfree_get_state(self._h, body, C.byref(cp)), "mfree_get_state")
+        return ps
+
+    def set_state(self, body, ps: ParticleSet):
+        cp = ps.to_c()
+        self._chk(self._L.mfree_set_state(self._h, body, C.byref(cp)), "mfree_set_state")
+
+    def time(self, body=0):
+        t = C.c_double(0)
+        self._chk(self._L.mfree_get_time(self._h, body, C.byref(t)), "mfree_get_time")
+        return t.value
+
+    # -- parity helpers
+    def derivatives(self, with_aux=True):
+        n = self.n_total
+        d8 = np.zeros((n, 8))
+        aux = np.zeros((n, 16)) if with_aux else None
+        self._chk(self._L.mfree_eval_derivatives(self._h, d8.ctypes.data_as(_capi.dptr),
+                                                 aux.ctypes.data_as(_capi.dptr) if with_aux else None),
+                  "mfree_eval_derivatives")
+        return d8, aux
+
+    def pairs(self, body=0):
+        n = self.sizes[body]
+        row = np.zeros(n + 1, dtype=np.uint32)
+        tot = C.c_uint64(0)
+        st = self._L.mfree_get_pairs(self._h, body, row.ctypes.data_as(_capi.u32ptr), None, 0, C.byref(tot))
+        if st not in (_capi.OK, _capi.ERR_CAPACITY):
+            self._chk(st, "mfree_get_pairs")
+        col = np.zeros(max(tot.value, 1), dtype=np.uint32)
+        self._chk(self._L.mfree_get_pairs(self._h, body, row.ctypes.data_as(_capi.u32ptr),
+                                          col.ctypes.data_as(_capi.u32ptr), col.size, C.byref(tot)),
+                  "mfree_get_pairs")
+        return row, col[:tot.value]
+
+    # -- profiling
+    def profile(self, on=True):
+        self._chk(self._L.mfree_profile_enable(self._h, 1 if on else 0), "mfree_profile_enable")
+
+    def profile_read(self):
+        pm, pn, sm, sn = C.c_double(), C.c_uint64(), C.c_double(), C.c_uint64()
+        self._chk(self._L.mfree_profile_read(self._h, C.byref(pm), C.byref(pn), C.byref(sm), C.byref(sn)),
+                  "mfree_profile_read")
+        return dict(pair_ms=pm.value, pair_launches=pn.value, step_ms=sm.value, steps=sn.value)
+
+
+def device_count():
+    return int(_capi.lib().mfree_device_count())
