@@ -132,10 +132,12 @@ mfree_status check_err_bits(mfree_ctx* c) {
   if (!bits) return MFREE_OK;
   uint32_t zero = 0;
   HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
-  if (bits & EB_CELLS) return fail(c, MFREE_ERR_CELLS, "cell grid: num_cell >= n or a particle outside the grid");
+  // a non-finite derivative is reported first: it makes later positions
+  // leave the grid as well
+  if (bits & EB_NONFINITE) return fail(c, MFREE_ERR_NONFINITE, "non-finite derivative");
   if (bits & EB_NBH) return fail(c, MFREE_ERR_NBH_OVERFLOW, "a particle has >= max_nbh neighbours");
   if (bits & EB_NONEIGH) return fail(c, MFREE_ERR_NO_NEIGHBOR, "a particle has no neighbour");
-  if (bits & EB_NONFINITE) return fail(c, MFREE_ERR_NONFINITE, "non-finite derivative");
+  if (bits & EB_CELLS) return fail(c, MFREE_ERR_CELLS, "cell grid: num_cell >= n or a particle outside the grid");
   return fail(c, MFREE_ERR_HIP, "unknown device error bits");
 }
 

@@ -159,7 +159,9 @@ def test_mixed_derivatives_close():
 
 
 def test_error_neighbour_overflow():
-    s = scenes.make_rings(41, hdx=5.0)  # ~300 neighbours per particle
+    s = scenes.make_rings(41, ri=0.0, hdx=5.0)  # discs: ~300 neighbours per particle
+    rc, _, _ = O.verlet_pairs(s.particles.x, s.particles.y, s.particles.h, cap=s.particles.n * 400)
+    assert rc == O.ERR_NBH_OVERFLOW  # the reference asserts here (src/grid.h:167)
     with solver_for(s) as sv:
         with pytest.raises(MfreeError) as ei:
             sv.step(1)
