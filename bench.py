@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run the plain copy kernel over MIB mebibytes (HBM counter calibration)")
+    ap.add_argument("--modes", default="fp64,mixed")
     args = ap.parse_args()
 
     def log(msg):
@@ -147,8 +150,22 @@ def main():
     n = scene.particles.n
     log(f"workload: {n} particles, dt = {scene.params.dt:.4e} s (setup {time.perf_counter() - t0:.1f} s)")
 
+    calib = None
+    if args.calibrate_copy > 0:
+        import ctypes as C
+        from mfree_iwf_amd import _capi
+        g = C.c_double(0)
+        _capi.check(_capi.lib().mfree_calibrate_copy(0, args.calibrate_copy << 20, 5, C.byref(g)),
+                    what="mfree_calibrate_copy")
+        calib = dict(bytes_per_launch_read=args.calibrate_copy << 20, bytes_per_launch_write=args.calibrate_copy << 20,
+                     gbps=g.value)
+        log(f"copy calibration: {args.calibrate_copy} MiB, {g.value:.1f} GB/s")
+
     modes = {}
+    wanted = args.modes.split(",")
     for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        if name not in wanted:
+            continue
         log(f"mode {name}")
         r = run_mode(scene, prec, args.steps, args.warmup, log)
         r.pop("state")
@@ -161,7 +178,7 @@ def main():
         log(f"  mixed vs fp64 after {acc['steps']} steps: pos {acc['rel_l2_pos']:.3e} "
             f"stress {acc['rel_l2_stress']:.3e}")
 
-    head = modes["mixed"]
+    head = modes["mixed"] if "mixed" in modes else modes["fp64"]
     achieved = ALG_BYTES_PER_PARTICLE_STAGE * n / (head["pair_kernel_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                 traffic=None, kernel="k_ul_pairs32",
@@ -194,6 +211,7 @@ def main():
         "accuracy": acc,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "copy_calibration": calib,
     }
     if rank == 0:
         print(json.dumps(line), flush=True)

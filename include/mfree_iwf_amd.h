@@ -174,6 +174,12 @@ mfree_status mfree_profile_enable(mfree_ctx* ctx, int32_t on);
 mfree_status mfree_profile_read(mfree_ctx* ctx, double* pair_ms_total, uint64_t* pair_launches,
                                 double* step_ms_total, uint64_t* steps);
 
+/* HBM calibration (no counterpart in the reference): streams `bytes` of
+ * doubles through a plain grid-stride device copy kernel (k_calib_copy)
+ * `reps` times on `device`; *gbps = 2*bytes*reps / elapsed.  Used to scale
+ * the FETCH_SIZE / WRITE_SIZE counters of the pair kernels. */
+mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps, double* gbps);
+
 /* Total particle count and per-body counts. */
 uint64_t mfree_num_particles(const mfree_ctx* ctx);
 uint32_t mfree_num_bodies(const mfree_ctx* ctx);

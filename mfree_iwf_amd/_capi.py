@@ -85,6 +85,7 @@ SIGNATURES = {
     "mfree_profile_enable": (C.c_int32, [C.c_void_p, C.c_int32]),
     "mfree_profile_read": (C.c_int32, [C.c_void_p, dptr, C.POINTER(C.c_uint64), dptr,
                                        C.POINTER(C.c_uint64)]),
+    "mfree_calibrate_copy": (C.c_int32, [C.c_int32, C.c_uint64, C.c_uint32, dptr]),
     "mfree_num_particles": (C.c_uint64, [C.c_void_p]),
     "mfree_num_bodies": (C.c_uint32, [C.c_void_p]),
     "mfree_scene_rings_count": (C.c_uint32, [C.POINTER(RingsSpec)]),

@@ -547,7 +547,8 @@ mfree_status mfree_add_body(mfree_ctx* c, const mfree_params* p, const mfree_par
   const bool tl = c->opt.method == MFREE_METHOD_TL_RL;
   if (!valid_particles(host, tl)) return fail(c, MFREE_ERR_ARG, "missing particle arrays");
   if (n < 2) return fail(c, MFREE_ERR_ARG, "a body needs at least 2 particles");
-  if ((uint64_t)c->off.back() + n >= (1ull << 31)) return fail(c, MFREE_ERR_ARG, "too many particles");
+  // the mixed pair kernel packs sorted positions into 28 bits
+  if ((uint64_t)c->off.back() + n >= (1ull << 28)) return fail(c, MFREE_ERR_ARG, "too many particles (max 2^28)");
   if (!(p->dt > 0) || !(p->rho0 > 0)) return fail(c, MFREE_ERR_ARG, "dt and rho0 must be positive");
   for (uint32_t i = 0; i < n; i++) {
     if ((host->fcx && host->fcx[i] != 0.) || (host->fcy && host->fcy[i] != 0.))
@@ -802,6 +803,37 @@ mfree_status mfree_profile_read(mfree_ctx* c, double* pair_ms_total, uint64_t* p
   if (step_ms_total) *step_ms_total = c->step_ms;
   if (steps) *steps = c->steps;
   return MFREE_OK;
+}
+
+mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps, double* gbps) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MFREE_ERR_HIP;
+  if (bytes < 8 || reps == 0) return MFREE_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return MFREE_ERR_HIP;
+  const uint64_t n = bytes / 8;
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
+  mfree_status rc = MFREE_ERR_HIP;
+  if (hipMalloc(&a, n * 8) == hipSuccess && hipMalloc(&b, n * 8) == hipSuccess &&
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess && hipMemsetAsync(a, 0, n * 8, st) == hipSuccess) {
+    launch_calib_copy(st, a, b, n);  // warm
+    hipEventRecord(e0, st);
+    for (uint32_t r = 0; r < reps; r++) launch_calib_copy(st, a, b, n);
+    hipEventRecord(e1, st);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0) {
+      if (gbps) *gbps = 2.0 * (double)(n * 8) * reps / (ms * 1e-3) / 1e9;
+      rc = MFREE_OK;
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 uint64_t mfree_num_particles(const mfree_ctx* c) { return c ? c->off.back() : 0; }

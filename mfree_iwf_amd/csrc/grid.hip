@@ -80,15 +80,15 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
   }
 }
 
-// One thread per body: combine the chunk partials and derive the cell
+// One workgroup per body: combine the chunk partials and derive the cell
 // geometry with the reference's arithmetic (src/grid.h:100-114).
-__global__ void k_bbox_final(uint32_t nscene, const uint32_t* __restrict__ chunk_first,
-                             const SceneConst* __restrict__ sc, const double* __restrict__ part,
-                             double kernel_width, SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nscene) return;
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restrict__ chunk_first,
+                                                       const SceneConst* __restrict__ sc,
+                                                       const double* __restrict__ part, double kernel_width,
+                                                       SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
+  const uint32_t s = blockIdx.x;
   double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
-  for (uint32_t c = chunk_first[s]; c < chunk_first[s + 1]; c++) {
+  for (uint32_t c = chunk_first[s] + threadIdx.x; c < chunk_first[s + 1]; c += blockDim.x) {
     const double* p = part + 5 * (size_t)c;
     mnx = fmin(p[0], mnx);
     mny = fmin(p[1], mny);
@@ -96,14 +96,42 @@ __global__ void k_bbox_final(uint32_t nscene, const uint32_t* __restrict__ chunk
     mxy = fmax(p[3], mxy);
     mxh = fmax(p[4], mxh);
   }
+  mnx = wave_reduce_min(mnx);
+  mny = wave_reduce_min(mny);
+  mxx = wave_reduce_max(mxx);
+  mxy = wave_reduce_max(mxy);
+  mxh = wave_reduce_max(mxh);
+  __shared__ double sh[5][kBlock / 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    sh[0][w] = mnx;
+    sh[1][w] = mny;
+    sh[2][w] = mxx;
+    sh[3][w] = mxy;
+    sh[4][w] = mxh;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < kBlock / 64; k++) {
+    mnx = fmin(sh[0][k], mnx);
+    mny = fmin(sh[1][k], mny);
+    mxx = fmax(sh[2][k], mxx);
+    mxy = fmax(sh[3][k], mxy);
+    mxh = fmax(sh[4][k], mxh);
+  }
+  mnx = fmin(sh[0][0], mnx);
+  mny = fmin(sh[1][0], mny);
+  mxx = fmax(sh[2][0], mxx);
+  mxy = fmax(sh[3][0], mxy);
+  mxh = fmax(sh[4][0], mxh);
   mnx -= 1e-6;
   mny -= 1e-6;
   mxx += 1e-6;
   mxy += 1e-6;
-  double dx = mxh * kernel_width;
-  double lx = mxx - mnx;
-  double ly = mxy - mny;
-  double fnx = ceil(lx / dx), fny = ceil(ly / dx);
+  const double dx = mxh * kernel_width;
+  const double lx = mxx - mnx;
+  const double ly = mxy - mny;
+  const double fnx = ceil(lx / dx), fny = ceil(ly / dx);
   SceneGeom g;
   g.bbmin_x = mnx;
   g.bbmin_y = mny;
@@ -111,7 +139,7 @@ __global__ void k_bbox_final(uint32_t nscene, const uint32_t* __restrict__ chunk
   g.err = 0;
   // num_cell < n is asserted by the reference (src/grid.h:114); the same
   // bound keeps every key inside the body's cell range here.
-  double ncell = fnx * fny;
+  const double ncell = fnx * fny;
   if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n)) {
     g.nx = 1;
     g.ny = 1;
@@ -127,32 +155,46 @@ __global__ void k_bbox_final(uint32_t nscene, const uint32_t* __restrict__ chunk
 }
 
 // Per particle (storage order): key = cell_offset + ix*ny + iy, and its rank
-// within the cell from an atomic histogram.
+// within the cell.  Storage is kept in cell order, so neighbouring lanes
+// mostly share a key: each run of equal keys inside a wave takes one atomic
+// for the whole run and hands out consecutive ranks.
 __global__ __launch_bounds__(kBlock) void k_hash_count(uint32_t n, const uint32_t* __restrict__ scene,
                                                        const double* __restrict__ x, const double* __restrict__ y,
                                                        const SceneConst* __restrict__ sc,
                                                        const SceneGeom* __restrict__ geom,
                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                        uint32_t* __restrict__ count, uint32_t* __restrict__ err) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t s = scene[i];
-  const SceneGeom g = geom[s];
-  double fx = (x[i] - g.bbmin_x) / g.cell;
-  double fy = (y[i] - g.bbmin_y) / g.cell;
-  uint32_t ix, iy;
-  bool ok = (fx >= 0.0) && (fy >= 0.0) && (fx < (double)g.nx) && (fy < (double)g.ny);
-  if (ok) {
-    ix = (unsigned int)fx;
-    iy = (unsigned int)fy;
-  } else {
-    ix = 0;
-    iy = 0;
-    if (!g.err) atomicOr(err, EB_CELLS);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t k = 0xFFFFFFFFu;
+  if (i < n) {
+    const uint32_t s = scene[i];
+    const SceneGeom g = geom[s];
+    const double fx = (x[i] - g.bbmin_x) / g.cell;
+    const double fy = (y[i] - g.bbmin_y) / g.cell;
+    uint32_t ix = 0, iy = 0;
+    if ((fx >= 0.0) && (fy >= 0.0) && (fx < (double)g.nx) && (fy < (double)g.ny)) {
+      ix = (unsigned int)fx;
+      iy = (unsigned int)fy;
+    } else if (!g.err) {
+      atomicOr(err, EB_CELLS);
+    }
+    k = sc[s].cell_offset + ix * g.ny + iy;
   }
-  uint32_t k = sc[s].cell_offset + ix * g.ny + iy;
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1u);
+  const uint32_t prev = __shfl_up(k, 1, 64);
+  const bool head = (lane == 0) || (prev != k);
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  const int hl = 63 - __clzll(heads & upto);          // head lane of my run
+  const uint64_t after = (lane == 63) ? 0ull : (heads & ~upto);
+  const int next = after ? __ffsll((long long)after) - 1 : 64;  // next run's head
+  uint32_t base = 0;
+  if (head && k != 0xFFFFFFFFu) base = atomicAdd(&count[k], (uint32_t)(next - lane));
+  base = __shfl(base, hl, 64);
+  if (i < n) {
+    key[i] = k;
+    rank[i] = base + (uint32_t)(lane - hl);
+  }
 }
 
 // ---- exclusive scan over u32 (cell counts -> cell starts) ---------------
@@ -270,6 +312,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(uint32_t ncell, const uint
   }
 }
 
+// Plain streaming copy: the HBM calibration kernel (reads and writes n*8 bytes).
+__global__ __launch_bounds__(kBlock) void k_calib_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                       uint64_t n2) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 // ---- host launchers -------------------------------------------------------
 
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -278,10 +327,14 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) {
   if (n) hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n, kBlock)), dim3(kBlock), 0, st, p, v, n);
 }
 
+void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n) {
+  hipLaunchKernelGGL(k_calib_copy, dim3(256 * 16), dim3(kBlock), 0, st, (const double2*)a, (double2*)b, n / 2);
+}
+
 void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(nblk(a.nscene, 64)), dim3(64), 0, st, a.nscene, a.chunk_first, a.sc,
-                     a.part, 2.0, a.geom, a.err);
+  hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part, 2.0,
+                     a.geom, a.err);
   launch_fill_u32(st, a.count, 0u, a.ncell_cap);
   hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
                      a.geom, a.key, a.rank, a.count, a.err);

@@ -8,7 +8,7 @@
 namespace mfree {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
-constexpr uint32_t kChunk = 8192;    // particles per bbox-reduction chunk
+constexpr uint32_t kChunk = 2048;    // particles per bbox-reduction chunk
 constexpr uint32_t kScanTile = 1024; // elements per scan workgroup
 
 // error bits raised by kernels (mapped to mfree_status on the host)

@@ -89,4 +89,6 @@ void launch_pair_list(hipStream_t st, const PairListArgs& a);
 
 __global__ void k_advance_time(uint32_t nscene, SceneConst* sc);
 
+void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n);
+
 }  // namespace mfree

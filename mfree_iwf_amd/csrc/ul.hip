@@ -203,7 +203,13 @@ __device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__
   }
 }
 
+// Neighbour list entries buffered per thread in LDS (strided by the block
+// size so that lanes hit distinct banks).
+constexpr int kNbCap = 32;
+
 __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
+  __shared__ uint32_t nb_lds[kNbCap * kBlock];
+  uint32_t* const my = nb_lds + threadIdx.x;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
@@ -234,7 +240,79 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   double vx_x = 0., vx_y = 0., vy_x = 0., vy_y = 0.;
   double ax = 0., ay = 0., xs = 0., ys = 0.;
   uint32_t cnt = 0;
+  int nb = 0;
 
+  // pair terms of one buffered neighbour, in the reference's operation order
+  auto pair = [&](uint32_t j) {
+    const double2 pj = a.pos64[j];
+    const double xij = pi.x - pj.x;
+    const double yij = pi.y - pj.y;
+    const double r2 = xij * xij + yij * yij;
+    const double rij = sqrt(r2);
+    const double q = rij * h1;
+    if (q >= 2.) return;
+    // cubic spline, src/kernel.cpp:20-55
+    double w, wx = 0., wy = 0.;
+    if (q >= 1.) {
+      w = fac * (0.25 * (2 - q) * (2 - q) * (2 - q));
+      if (rij > 1e-12) {
+        const double der = -0.75 * (2 - q) * (2 - q) * h1 / rij;
+        wx = xij * der * fac;
+        wy = yij * der * fac;
+      }
+    } else {
+      w = fac * (1 - 1.5 * q * q * (1 - 0.5 * q));
+      if (rij > 1e-12) {
+        const double der = -3.0 * q * (1 - 0.75 * q) * h1 / rij;
+        wx = xij * der * fac;
+        wy = yij * der * fac;
+      }
+    }
+    const PairRec64 rj = a.rec64[j];
+    // derive_stress_monaghan
+    double Rxx = 0., Rxy = 0., Ryy = 0.;
+    if (use_as) {
+      double fab = w / wdeltap;
+      fab = pow4 ? (fab * fab) * (fab * fab) : pow(fab, corr_exp);
+      Rxx = fab * (ri.Rxx + rj.Rxx);
+      Rxy = fab * (ri.Rxy + rj.Rxy);
+      Ryy = fab * (ri.Ryy + rj.Ryy);
+    }
+    const double mj = rj.m;
+    Sxx_x += mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx;
+    Sxy_y += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy;
+    Sxy_x += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx;
+    Syy_y += mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy;
+    // derive_velocity
+    vx_x += (rj.vx - ri.vx) * wx * rj.qw;
+    vx_y += (rj.vx - ri.vx) * wy * rj.qw;
+    vy_x += (rj.vy - ri.vy) * wx * rj.qw;
+    vy_y += (rj.vy - ri.vy) * wy * rj.qw;
+    // correctors_mghn_artificial_viscosity
+    const double vijx = ri.vx - rj.vx;
+    const double vijy = ri.vy - rj.vy;
+    const double vp = vijx * xij + vijy * yij;
+    if (!(vp >= 0.)) {
+      const double cij = 0.5 * (ri.c + rj.c);
+      const double hij = 0.5 * (hi + rj.h);
+      const double rhoij = 0.5 * (ri.rho + rj.rho);
+      const double muij = (hij * vp) / (r2 + eta * eta * hij * hij);
+      const double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+      ax += -mj * piij * wx;
+      ay += -mj * piij * wy;
+    }
+    // correctors_xsph (mass of particle i, src/correctors.h:163)
+    const double rhoij = 0.5 * (ri.rho + rj.rho);
+    const double fx = -xeps * w * ri.m / rhoij;
+    xs += fx * vijx;
+    ys += fx * vijy;
+  };
+  auto flush = [&]() {
+    for (int k = 0; k < nb; k++) pair(my[k * kBlock]);
+    nb = 0;
+  };
+
+  // scan: the 3x3 block ni-major, nj-minor, particles of a cell in idx order
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
     const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
@@ -245,66 +323,11 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
       const double r2 = xij * xij + yij * yij;
       cnt += (r2 <= radius2 + 1e-6) ? 1u : 0u;
       if (j == i || r2 > cut2) continue;
-      const double rij = sqrt(r2);
-      const double q = rij * h1;
-      if (q >= 2.) continue;
-      // cubic spline, src/kernel.cpp:20-55
-      double w, wx = 0., wy = 0.;
-      if (q >= 1.) {
-        w = fac * (0.25 * (2 - q) * (2 - q) * (2 - q));
-        if (rij > 1e-12) {
-          const double der = -0.75 * (2 - q) * (2 - q) * h1 / rij;
-          wx = xij * der * fac;
-          wy = yij * der * fac;
-        }
-      } else {
-        w = fac * (1 - 1.5 * q * q * (1 - 0.5 * q));
-        if (rij > 1e-12) {
-          const double der = -3.0 * q * (1 - 0.75 * q) * h1 / rij;
-          wx = xij * der * fac;
-          wy = yij * der * fac;
-        }
-      }
-      const PairRec64 rj = a.rec64[j];
-      // derive_stress_monaghan
-      double Rxx = 0., Rxy = 0., Ryy = 0.;
-      if (use_as) {
-        double fab = w / wdeltap;
-        fab = pow4 ? (fab * fab) * (fab * fab) : pow(fab, corr_exp);
-        Rxx = fab * (ri.Rxx + rj.Rxx);
-        Rxy = fab * (ri.Rxy + rj.Rxy);
-        Ryy = fab * (ri.Ryy + rj.Ryy);
-      }
-      const double mj = rj.m;
-      Sxx_x += mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx;
-      Sxy_y += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy;
-      Sxy_x += mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx;
-      Syy_y += mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy;
-      // derive_velocity
-      vx_x += (rj.vx - ri.vx) * wx * rj.qw;
-      vx_y += (rj.vx - ri.vx) * wy * rj.qw;
-      vy_x += (rj.vy - ri.vy) * wx * rj.qw;
-      vy_y += (rj.vy - ri.vy) * wy * rj.qw;
-      // correctors_mghn_artificial_viscosity
-      const double vijx = ri.vx - rj.vx;
-      const double vijy = ri.vy - rj.vy;
-      const double vp = vijx * xij + vijy * yij;
-      if (!(vp >= 0.)) {
-        const double cij = 0.5 * (ri.c + rj.c);
-        const double hij = 0.5 * (hi + rj.h);
-        const double rhoij = 0.5 * (ri.rho + rj.rho);
-        const double muij = (hij * vp) / (r2 + eta * eta * hij * hij);
-        const double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
-        ax += -mj * piij * wx;
-        ay += -mj * piij * wy;
-      }
-      // correctors_xsph (mass of particle i, src/correctors.h:163)
-      const double rhoij = 0.5 * (ri.rho + rj.rho);
-      const double fx = -xeps * w * ri.m / rhoij;
-      xs += fx * vijx;
-      ys += fx * vijy;
+      my[nb * kBlock] = j;
+      if (++nb == kNbCap) flush();
     }
   }
+  flush();
   if (a.max_nbh && cnt >= a.max_nbh) atomicOr(a.err, EB_NBH);
   if (cnt == 0) atomicOr(a.err, EB_NONEIGH);
 

@@ -44,7 +44,12 @@ __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_
 
 }  // namespace
 
+constexpr int kNbCap = 32;
+
+
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
+  __shared__ uint32_t nb_lds[kNbCap * kBlock];
+  uint32_t* const my = nb_lds + threadIdx.x;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
@@ -63,8 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   const float h1 = 1.f / hi;
   const float fac = (float)(10 * (M_1_PI) / 7.0) * h1 * h1;
   const float cut2 = 4.0001f * (hi * hi);
-  const float wdeltap = (float)c->wdeltap;
-  const float inv_wdp = 1.f / wdeltap;
+  const float inv_wdp = (float)(1.0 / c->wdeltap);
   const float corr_exp = (float)c->corr_exp;
   const bool use_as = c->wdeltap > 0;
   const bool pow4 = c->pow4 != 0;
@@ -75,69 +79,90 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   Acc Sxx_x{0.f, 0.f}, Sxy_y{0.f, 0.f}, Sxy_x{0.f, 0.f}, Syy_y{0.f, 0.f};
   Acc vx_x{0.f, 0.f}, vx_y{0.f, 0.f}, vy_x{0.f, 0.f}, vy_y{0.f, 0.f};
   Acc ax{0.f, 0.f}, ay{0.f, 0.f}, xs{0.f, 0.f}, ys{0.f, 0.f};
+  int nb = 0;
+
+  auto pair = [&](uint32_t e) {
+    const uint32_t j = e & 0x0FFFFFFFu;
+    const float offx = (float)((int)((e >> 28) & 3u) - 1) * cellf;
+    const float offy = (float)((int)(e >> 30) - 1) * cellf;
+    const float2 pj = a.pos32[j];
+    const float xij = (pi.x - pj.x) + offx;
+    const float yij = (pi.y - pj.y) + offy;
+    const float r2 = xij * xij + yij * yij;
+    const float rij = sqrtf(r2);
+    const float q = rij * h1;
+    if (q >= 2.f) return;
+    float w, der;
+    if (q >= 1.f) {
+      const float t = 2.f - q;
+      w = fac * (0.25f * t * t * t);
+      der = -0.75f * t * t * h1;
+    } else {
+      w = fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+      der = -3.0f * q * (1.f - 0.75f * q) * h1;
+    }
+    const float gsc = rij > 1e-12f ? der * fac / rij : 0.f;
+    const float wx = xij * gsc, wy = yij * gsc;
+    const PairRec32 rj = a.rec32[j];
+    float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
+    if (use_as) {
+      float fab = w * inv_wdp;
+      fab = pow4 ? (fab * fab) * (fab * fab) : powf(fab, corr_exp);
+      Rxx = fab * (ri.Rxx + rj.Rxx);
+      Rxy = fab * (ri.Rxy + rj.Rxy);
+      Ryy = fab * (ri.Ryy + rj.Ryy);
+    }
+    const float mj = rj.m;
+    acc_add(Sxx_x, mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx);
+    acc_add(Sxy_y, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy);
+    acc_add(Sxy_x, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx);
+    acc_add(Syy_y, mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy);
+    const float dvx = rj.vx - ri.vx, dvy = rj.vy - ri.vy;
+    acc_add(vx_x, dvx * wx * rj.qw);
+    acc_add(vx_y, dvx * wy * rj.qw);
+    acc_add(vy_x, dvy * wx * rj.qw);
+    acc_add(vy_y, dvy * wy * rj.qw);
+    const float vp = -(dvx * xij + dvy * yij);
+    if (!(vp >= 0.f)) {
+      const float cij = 0.5f * (ri.c + rj.c);
+      const float hij = 0.5f * (hi + rj.h);
+      const float rhoij = 0.5f * (ri.rho + rj.rho);
+      const float muij = (hij * vp) / (r2 + eta2 * hij * hij);
+      const float piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+      acc_add(ax, -mj * piij * wx);
+      acc_add(ay, -mj * piij * wy);
+    }
+    const float rhoij = 0.5f * (ri.rho + rj.rho);
+    const float fx = -xeps_mi * w / rhoij;
+    acc_add(xs, -fx * dvx);
+    acc_add(ys, -fx * dvy);
+  };
+  auto flush = [&]() {
+    for (int k = 0; k < nb; k++) pair(my[k * kBlock]);
+    nb = 0;
+  };
 
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
-    const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
     const float offx = (float)((int)gi - (int)ni) * cellf;
-    for (uint32_t j = ja; j < jb; j++) {
-      if (j == i) continue;
-      const float2 pj = a.pos32[j];
-      const uint32_t iyj = (a.meta[j].x - c->cell_offset) - ni * g.ny;
-      const float offy = (float)((int)gj - (int)iyj) * cellf;
-      const float xij = (pi.x - pj.x) + offx;
-      const float yij = (pi.y - pj.y) + offy;
-      const float r2 = xij * xij + yij * yij;
-      if (r2 > cut2) continue;
-      const float rij = sqrtf(r2);
-      const float q = rij * h1;
-      if (q >= 2.f) continue;
-      float w, der = 0.f;
-      if (q >= 1.f) {
-        const float t = 2.f - q;
-        w = fac * (0.25f * t * t * t);
-        der = -0.75f * t * t * h1;
-      } else {
-        w = fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
-        der = -3.0f * q * (1.f - 0.75f * q) * h1;
+    uint32_t jb = a.cell_start[rb + lo_j];
+    for (uint32_t nj = lo_j; nj < hi_j; nj++) {
+      const uint32_t ja = jb;
+      jb = a.cell_start[rb + nj + 1];
+      const float offy = (float)((int)gj - (int)nj) * cellf;
+      const uint32_t tag = ((gi - ni + 1) << 28) | ((gj - nj + 1) << 30);
+      for (uint32_t j = ja; j < jb; j++) {
+        const float2 pj = a.pos32[j];
+        const float xij = (pi.x - pj.x) + offx;
+        const float yij = (pi.y - pj.y) + offy;
+        const float r2 = xij * xij + yij * yij;
+        if (j == i || r2 > cut2) continue;
+        my[nb * kBlock] = j | tag;
+        if (++nb == kNbCap) flush();
       }
-      const float gsc = rij > 1e-12f ? der * fac / rij : 0.f;
-      const float wx = xij * gsc, wy = yij * gsc;
-      const PairRec32 rj = a.rec32[j];
-      float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
-      if (use_as) {
-        float fab = w * inv_wdp;
-        fab = pow4 ? (fab * fab) * (fab * fab) : powf(fab, corr_exp);
-        Rxx = fab * (ri.Rxx + rj.Rxx);
-        Rxy = fab * (ri.Rxy + rj.Rxy);
-        Ryy = fab * (ri.Ryy + rj.Ryy);
-      }
-      const float mj = rj.m;
-      acc_add(Sxx_x, mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx);
-      acc_add(Sxy_y, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy);
-      acc_add(Sxy_x, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx);
-      acc_add(Syy_y, mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy);
-      const float dvx = rj.vx - ri.vx, dvy = rj.vy - ri.vy;
-      acc_add(vx_x, dvx * wx * rj.qw);
-      acc_add(vx_y, dvx * wy * rj.qw);
-      acc_add(vy_x, dvy * wx * rj.qw);
-      acc_add(vy_y, dvy * wy * rj.qw);
-      const float vp = -(dvx * xij + dvy * yij);
-      if (!(vp >= 0.f)) {
-        const float cij = 0.5f * (ri.c + rj.c);
-        const float hij = 0.5f * (hi + rj.h);
-        const float rhoij = 0.5f * (ri.rho + rj.rho);
-        const float muij = (hij * vp) / (r2 + eta2 * hij * hij);
-        const float piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
-        acc_add(ax, -mj * piij * wx);
-        acc_add(ay, -mj * piij * wy);
-      }
-      const float rhoij = 0.5f * (ri.rho + rj.rho);
-      const float fx = -xeps_mi * w / rhoij;
-      acc_add(xs, -fx * dvx);
-      acc_add(ys, -fx * dvy);
     }
   }
+  flush();
 
   const uint32_t home = a.home_identity ? i : a.perm[i];
   const double vxi = a.src.f[2][home], vyi = a.src.f[3][home], rhoi = a.src.f[4][home];
