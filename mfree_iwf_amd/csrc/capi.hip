@@ -238,6 +238,8 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.nchunks = c->nchunks;
   a.ncell_cap = c->N;
   a.deterministic = c->opt.deterministic;
+  a.kernel_width = 2.0;  // src/body.h:135,138
+  a.cell_min = 0.0;
   a.chunks = c->chunks;
   a.chunk_first = c->chunk_first;
   a.sc = c->sc;

@@ -85,7 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
 __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restrict__ chunk_first,
                                                        const SceneConst* __restrict__ sc,
                                                        const double* __restrict__ part, double kernel_width,
-                                                       SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
+                                                       double cell_min, SceneGeom* __restrict__ geom,
+                                                       uint32_t* __restrict__ err) {
   const uint32_t s = blockIdx.x;
   double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
   for (uint32_t c = chunk_first[s] + threadIdx.x; c < chunk_first[s + 1]; c += blockDim.x) {
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
   mny -= 1e-6;
   mxx += 1e-6;
   mxy += 1e-6;
-  const double dx = mxh * kernel_width;
+  const double dx = fmax(mxh * kernel_width, cell_min);
   const double lx = mxx - mnx;
   const double ly = mxy - mny;
   const double fnx = ceil(lx / dx), fny = ceil(ly / dx);
@@ -333,8 +334,8 @@ void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n) {
 
 void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part, 2.0,
-                     a.geom, a.err);
+  hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part,
+                     a.kernel_width, a.cell_min, a.geom, a.err);
   launch_fill_u32(st, a.count, 0u, a.ncell_cap);
   hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
                      a.geom, a.key, a.rank, a.count, a.err);

@@ -14,6 +14,8 @@ struct Fields8 {
 struct BinArgs {
   uint32_t n, nscene, nchunks, ncell_cap;
   int deterministic;
+  double kernel_width;          // cell side = max(h_max * kernel_width, cell_min)
+  double cell_min;
   const Chunk* chunks;
   const uint32_t* chunk_first;  // nscene+1
   const SceneConst* sc;
@@ -90,5 +92,37 @@ void launch_pair_list(hipStream_t st, const PairListArgs& a);
 __global__ void k_advance_time(uint32_t nscene, SceneConst* sc);
 
 void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n);
+
+// glm-compatible 2x2 algebra (column-major m[col][row]) shared by the TL
+// kernels: dmat2x2(a, b, c, d) has columns (a, b) and (c, d).
+struct M2 {
+  double m[2][2];
+};
+__device__ inline M2 m2(double a, double b, double c, double d) {
+  M2 r;
+  r.m[0][0] = a;
+  r.m[0][1] = b;
+  r.m[1][0] = c;
+  r.m[1][1] = d;
+  return r;
+}
+// (A*B)[c][r] = A[0][r]*B[c][0] + A[1][r]*B[c][1]
+__device__ inline M2 m2_mul(const M2& a, const M2& b) {
+  M2 r;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) r.m[c][w] = a.m[0][w] * b.m[c][0] + a.m[1][w] * b.m[c][1];
+  return r;
+}
+__device__ inline M2 m2_scale(double s, const M2& a) {
+  return m2(s * a.m[0][0], s * a.m[0][1], s * a.m[1][0], s * a.m[1][1]);
+}
+__device__ inline double m2_det(const M2& a) { return a.m[0][0] * a.m[1][1] - a.m[1][0] * a.m[0][1]; }
+// adjugate times the reciprocal of the determinant, taken once
+__device__ inline M2 m2_inv(const M2& a) {
+  const double ood = 1.0 / (a.m[0][0] * a.m[1][1] - a.m[1][0] * a.m[0][1]);
+  return m2(a.m[1][1] * ood, -a.m[0][1] * ood, -a.m[1][0] * ood, a.m[0][0] * ood);
+}
 
 }  // namespace mfree

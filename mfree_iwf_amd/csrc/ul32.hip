@@ -6,7 +6,7 @@
 //  - positions are cell-local (x minus the particle's own cell corner,
 //    subtracted in FP64), so x_ij keeps ~1e-11 m resolution instead of the
 //    ~1e-8 m of absolute FP32 coordinates;
-//  - every pair sum is accumulated with an error-free TwoSum into a
+//  - every pair sum is accumulated with Kahan compensation into a
 //    (sum, compensation) pair;
 //  - the per-particle epilogue (Jaumann rate, continuity, momentum,
 //    advection) and the RK4 state update stay FP64.
@@ -21,15 +21,16 @@ struct Acc {
   float s, c;
 };
 
+// Kahan compensated summation: c carries the negated low-order part lost
+// by each addition (Kahan 1965).
 __device__ inline void acc_add(Acc& a, float x) {
-  const float t = a.s + x;
-  const float bp = t - a.s;
-  const float e = (a.s - (t - bp)) + (x - bp);
+  const float y = x - a.c;
+  const float t = a.s + y;
+  a.c = (t - a.s) - y;
   a.s = t;
-  a.c += e;
 }
 
-__device__ inline double acc_val(const Acc& a) { return (double)a.s + (double)a.c; }
+__device__ inline double acc_val(const Acc& a) { return (double)a.s - (double)a.c; }
 
 __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
                                  double Sxy, double Syy, double& t00, double& t01, double& t11) {
