@@ -1,0 +1,335 @@
+// Reference-compatible body<T> (src/body.h:35-235) on top of the C ABI.
+//
+// Drivers build bodies exactly as with the reference: new'ed particle
+// arrays, simulation_data, dt, boundary conditions, add_action(&...).  At
+// the first step() the registered action list is matched, by function
+// identity, against the two pipelines of the hot path:
+//   Gray-Monaghan UL  (src/benchmarks/rings.cpp:86-96)   -> particle_monaghan
+//   Total-Lagrangian  (src/benchmarks/tensile.cpp:276-286) -> particle_tl
+// and the body is moved to the GPU (SoA, idx order).  Any other action
+// list, particle type or boundary function is rejected with a message
+// (MFREE_ERR_PIPELINE): there is no CPU execution of the actions here.
+//
+// After step(), get_particles() reads the integrated state back into the
+// particle objects (x, y, vx, vy, rho, Sxx, Sxy, Syy).  Derived per-stage
+// fields (p, quad_weight, gradients, neighbour lists) are not filled.
+#ifndef MFREE_BODY_H_
+#define MFREE_BODY_H_
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../mfree_iwf_amd.h"
+#include "boundary_conditions.h"
+#include "particle.h"
+#include "simulation_data.h"
+#include "simulation_time.h"
+
+template <typename T>
+class body;
+
+namespace mfree_adapter {
+
+[[noreturn]] inline void die(const std::string& msg) {
+  fprintf(stderr, "mfree: %s\n", msg.c_str());
+  abort();
+}
+
+[[noreturn]] inline void direct_call(const char* name) {
+  die(std::string(name) + ": actions run inside body<T>::step() on the GPU and cannot be called directly");
+}
+
+// Set-up requests of TL drivers (find_neighbors + precomp_randles_libersky
+// before the body exists, src/benchmarks/tensile.cpp:271-272), keyed by the
+// particle array.
+struct TlRequest {
+  double h = 0.;
+  bool rl = false;
+};
+inline std::map<const void*, TlRequest>& tl_requests() {
+  static std::map<const void*, TlRequest> r;
+  return r;
+}
+
+// Arithmetic of the pair sums for bodies created afterwards.
+inline int32_t& precision() {
+  static int32_t p = MFREE_PRECISION_FP64;
+  return p;
+}
+
+struct Gpu {
+  mfree_ctx* ctx = nullptr;
+  ~Gpu() {
+    if (ctx) mfree_destroy(ctx);
+  }
+};
+
+}  // namespace mfree_adapter
+
+// ---- actions (src/derivatives.h, src/correctors.h, src/cont_mech.h,
+//      src/material.h, src/riemann_integration.h) ------------------------
+#define MFREE_ACTION(name) \
+  template <typename T>    \
+  void name(body<T>&) {    \
+    mfree_adapter::direct_call(#name); \
+  }
+MFREE_ACTION(riemann_update_integration_weights)
+MFREE_ACTION(material_eos)
+MFREE_ACTION(correctors_mghn_artificial_stress)
+MFREE_ACTION(derive_stress_monaghan)
+MFREE_ACTION(derive_velocity)
+MFREE_ACTION(correctors_mghn_artificial_viscosity)
+MFREE_ACTION(correctors_xsph)
+MFREE_ACTION(material_stress_rate_jaumann)
+MFREE_ACTION(contmech_continuity)
+MFREE_ACTION(contmech_momentum_ul)
+MFREE_ACTION(contmech_advection)
+MFREE_ACTION(derive_displacement)
+MFREE_ACTION(derive_velocity_tl)
+MFREE_ACTION(contmech_velocity_gradient_tl)
+MFREE_ACTION(contmech_cauchy_to_nominal)
+MFREE_ACTION(derive_stress_tl)
+MFREE_ACTION(contmech_momentum_tl)
+#undef MFREE_ACTION
+
+// src/benchmarks/neighbor_search.h:26-28,116-147: recorded, executed on the
+// GPU by the body's first step.
+enum distance_type { distance_manhattan, distance_euclidian };
+
+template <typename T>
+void find_neighbors(T** particles, unsigned int num_part, double h, distance_type type) {
+  (void)num_part;
+  if (type != distance_euclidian) mfree_adapter::die("find_neighbors: only distance_euclidian is supported");
+  mfree_adapter::tl_requests()[(const void*)particles].h = h;
+}
+
+// src/precomp_shape_functions.h:231-290: recorded, executed on the GPU.
+template <typename T>
+void precomp_randles_libersky(T** particles, unsigned int np) {
+  (void)np;
+  mfree_adapter::tl_requests()[(const void*)particles].rl = true;
+}
+
+// src/rk4.h:224-273: the stepper sets the global dt; the integration itself
+// runs on the GPU.
+template <typename T>
+class rk4 {
+ public:
+  rk4() {}
+  rk4(unsigned int num_part, double dt) {
+    (void)num_part;
+    simulation_time::getInstance().m_dt = dt;
+  }
+  rk4(unsigned int num_part, unsigned int num_quad, double dt) {
+    (void)num_part;
+    (void)num_quad;
+    simulation_time::getInstance().m_dt = dt;
+  }
+};
+
+template <typename T>
+class body {
+  static_assert(std::is_base_of<particle, T>::value, "T is not derived from particle");
+
+ public:
+  body() {}
+  body(T** particles, unsigned int n, simulation_data data, double dt,
+       boundary_conditions<T> bc = boundary_conditions<T>(), T** quad_points = 0, unsigned int nq = 0,
+       void* gauss_integration = 0)
+      : m_data(data), m_bc(bc), m_num_part(n), m_particles(particles), m_stepper(n, nq, dt) {
+    if (quad_points || nq || gauss_integration) mfree_adapter::die("weak-form bodies are not part of this library");
+  }
+
+  void add_action(void (*action)(body<T>& b)) { m_actions.push_back(action); }
+
+  // src/body.h:223-225 -> src/rk4.h:71-108
+  void step() {
+    if (!m_gpu) setup();
+    mfree_status st = mfree_step(m_gpu->ctx, 1);
+    if (st != MFREE_OK) mfree_adapter::die(std::string("step: ") + mfree_last_error(m_gpu->ctx));
+    simulation_time::getInstance().increment_time();
+    *m_stale = true;
+  }
+
+  T** get_particles() const {
+    sync();
+    return m_particles;
+  }
+  T** get_cur_particles() const { return get_particles(); }
+  unsigned int get_num_part() const { return m_num_part; }
+  simulation_data get_sim_data() const { return m_data; }
+  bool has_boundary() const { return false; }
+  void update_boundary() {}
+  mfree_ctx* context() const { return m_gpu ? m_gpu->ctx : nullptr; }
+
+ private:
+  simulation_data m_data;
+  std::vector<void (*)(body<T>&)> m_actions;
+  boundary_conditions<T> m_bc;
+  unsigned int m_num_part = 0;
+  T** m_particles = nullptr;
+  rk4<T> m_stepper;
+  std::shared_ptr<mfree_adapter::Gpu> m_gpu;
+  std::shared_ptr<bool> m_stale = std::make_shared<bool>(false);
+
+  int32_t pipeline() const {
+    typedef void (*A)(body<T>&);
+    const std::vector<A> ul = {&riemann_update_integration_weights<T>,
+                               &material_eos<T>,
+                               &correctors_mghn_artificial_stress<T>,
+                               &derive_stress_monaghan<T>,
+                               &derive_velocity<T>,
+                               &correctors_mghn_artificial_viscosity<T>,
+                               &correctors_xsph<T>,
+                               &material_stress_rate_jaumann<T>,
+                               &contmech_continuity<T>,
+                               &contmech_momentum_ul<T>,
+                               &contmech_advection<T>};
+    const std::vector<A> tl = {&derive_displacement<T>,
+                               &derive_velocity_tl<T>,
+                               &contmech_velocity_gradient_tl<T>,
+                               &contmech_continuity<T>,
+                               &material_eos<T>,
+                               &material_stress_rate_jaumann<T>,
+                               &contmech_cauchy_to_nominal<T>,
+                               &derive_stress_tl<T>,
+                               &correctors_mghn_artificial_viscosity<T>,
+                               &contmech_momentum_tl<T>,
+                               &contmech_advection<T>};
+    if (m_actions == ul && std::is_base_of<particle_monaghan, T>::value) return MFREE_METHOD_UL_MONAGHAN;
+    if (m_actions == tl && std::is_base_of<particle_tl, T>::value) return MFREE_METHOD_TL_RL;
+    return -1;
+  }
+
+  // Strip velocity a of a boundary function of the form x = X + a t, y = Y,
+  // vx = a, vy = 0, vx_t = vy_t = 0, found by evaluating it at t = 0 and 1.
+  static bool probe_strip(void (*fn)(T*), double* a) {
+    simulation_time& st = simulation_time::getInstance();
+    const double t_saved = st.m_time;
+    T p0, p1;
+    for (T* p : {&p0, &p1}) {
+      p->X = 0.25;
+      p->Y = 0.5;
+      p->x = p->y = p->vx = p->vy = p->vx_t = p->vy_t = 7.0;
+    }
+    st.m_time = 0.;
+    fn(&p0);
+    st.m_time = 1.;
+    fn(&p1);
+    st.m_time = t_saved;
+    const double v = p0.vx;
+    *a = v;
+    return p0.x == 0.25 && p1.x == 0.25 + v && p0.y == 0.5 && p1.y == 0.5 && p1.vx == v && p0.vy == 0. &&
+           p1.vy == 0. && p0.vx_t == 0. && p0.vy_t == 0.;
+  }
+
+  void setup() {
+    const int32_t method = pipeline();
+    if (method < 0) mfree_adapter::die("body::step: action list is not one of the supported pipelines (MFREE_ERR_PIPELINE)");
+    for (unsigned int i = 0; i < m_num_part; i++)
+      if (m_particles[i]->idx != i) mfree_adapter::die("body: particle i must have idx == i");
+    mfree_options opt;
+    mfree_default_options(&opt);
+    opt.method = method;
+    opt.precision = mfree_adapter::precision();
+    if (method == MFREE_METHOD_TL_RL) {
+      auto it = mfree_adapter::tl_requests().find((const void*)m_particles);
+      if (it == mfree_adapter::tl_requests().end() || !it->second.rl)
+        mfree_adapter::die("TL body: find_neighbors and precomp_randles_libersky must be called on its particles");
+      opt.tl_h_search = it->second.h;
+    }
+    auto gpu = std::make_shared<mfree_adapter::Gpu>();
+    if (mfree_create(&opt, &gpu->ctx) != MFREE_OK) mfree_adapter::die("mfree_create failed");
+    const physical_constants pc = m_data.get_physical_constants();
+    const correction_constants cc = m_data.get_correction_constants();
+    mfree_params prm;
+    prm.E = pc.E();
+    prm.nu = pc.nu();
+    prm.rho0 = pc.rho0();
+    prm.wdeltap = cc.get_monaghan_const().mghn_wdeltap();
+    prm.corr_exp = cc.get_monaghan_const().mghn_corr_exp();
+    prm.mghn_eps = cc.get_monaghan_const().mghn_eps();
+    prm.alpha = cc.get_art_visc_const().artvisc_alpha();
+    prm.beta = cc.get_art_visc_const().artvisc_beta();
+    prm.eta = cc.get_art_visc_const().artvisc_eta();
+    prm.xsph_eps = cc.xsph_eps();
+    prm.dt = simulation_time::getInstance().get_dt();
+    prm.time = simulation_time::getInstance().get_time();
+    const unsigned int n = m_num_part;
+    std::vector<double> f[14];
+    for (auto& v : f) v.resize(n);
+    for (unsigned int i = 0; i < n; i++) {
+      const T* p = m_particles[i];
+      f[0][i] = p->x;
+      f[1][i] = p->y;
+      f[2][i] = p->X;
+      f[3][i] = p->Y;
+      f[4][i] = p->m;
+      f[5][i] = p->rho;
+      f[6][i] = p->h;
+      f[7][i] = p->vx;
+      f[8][i] = p->vy;
+      f[9][i] = p->Sxx;
+      f[10][i] = p->Sxy;
+      f[11][i] = p->Syy;
+      f[12][i] = p->fcx;
+      f[13][i] = p->fcy;
+    }
+    mfree_particles hp = {f[0].data(), f[1].data(), f[2].data(),  f[3].data(),  f[4].data(),
+                          f[5].data(), f[6].data(), f[7].data(),  f[8].data(),  f[9].data(),
+                          f[10].data(), f[11].data(), f[12].data(), f[13].data()};
+    uint32_t bid = 0;
+    if (mfree_add_body(gpu->ctx, &prm, &hp, n, &bid) != MFREE_OK)
+      mfree_adapter::die(std::string("add_body: ") + mfree_last_error(gpu->ctx));
+    if (!m_bc.list().empty()) {
+      if (method != MFREE_METHOD_TL_RL) mfree_adapter::die("boundary conditions on a UL body are not supported");
+      std::vector<uint32_t> left, right;
+      double speed = -1.;
+      for (const auto& e : m_bc.list()) {
+        double a = 0.;
+        if (!probe_strip(std::get<1>(e), &a)) mfree_adapter::die("boundary function is not a velocity strip");
+        if (speed >= 0. && fabs(a) != speed) mfree_adapter::die("strips must move with equal and opposite speed");
+        speed = fabs(a);
+        for (unsigned int k : std::get<0>(e)) (a < 0. ? left : right).push_back(k);
+      }
+      if (mfree_set_strip_bc(gpu->ctx, bid, left.data(), (uint32_t)left.size(), right.data(),
+                             (uint32_t)right.size(), speed) != MFREE_OK)
+        mfree_adapter::die(std::string("set_strip_bc: ") + mfree_last_error(gpu->ctx));
+    }
+    if (method == MFREE_METHOD_TL_RL && mfree_tl_precompute(gpu->ctx) != MFREE_OK)
+      mfree_adapter::die(std::string("TL precompute: ") + mfree_last_error(gpu->ctx));
+    m_gpu = gpu;
+  }
+
+  void sync() const {
+    if (!m_gpu || !*m_stale) return;
+    const unsigned int n = m_num_part;
+    std::vector<double> f[8];
+    for (auto& v : f) v.resize(n);
+    mfree_particles hp = {f[0].data(), f[1].data(), nullptr, nullptr, nullptr, f[4].data(), nullptr,
+                          f[2].data(), f[3].data(), f[5].data(), f[6].data(), f[7].data(), nullptr, nullptr};
+    if (mfree_get_state(m_gpu->ctx, 0, &hp) != MFREE_OK)
+      mfree_adapter::die(std::string("get_state: ") + mfree_last_error(m_gpu->ctx));
+    for (unsigned int i = 0; i < n; i++) {
+      T* p = m_particles[i];
+      p->x = f[0][i];
+      p->y = f[1][i];
+      p->vx = f[2][i];
+      p->vy = f[3][i];
+      p->rho = f[4][i];
+      p->Sxx = f[5][i];
+      p->Sxy = f[6][i];
+      p->Syy = f[7][i];
+    }
+    *m_stale = false;
+  }
+};
+
+#endif

@@ -1,0 +1,41 @@
+// A driver written against the reference's surface (the shape of
+// src/mfree_iwf.cpp:39-108): build bodies with a reference driver, step
+// them, read the particles back.  Writes the final state as text:
+//   <n> <time>
+//   idx x y vx vy rho Sxx Sxy Syy        (one line per particle)
+//
+//   reference_main rings <nbox> <steps> [mixed]
+//   reference_main tensile <nbox> <steps> [mixed]
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mfree/benchmarks/rings.h"
+#include "mfree/benchmarks/tensile.h"
+
+template <typename T>
+static int run(std::vector<body<T>> bodies, unsigned int steps) {
+  for (unsigned int s = 0; s < steps; s++)
+    for (auto& b : bodies) b.step();
+  for (auto& b : bodies) {
+    T** p = b.get_particles();
+    printf("%u %.17g\n", b.get_num_part(), simulation_time::getInstance().get_time());
+    for (unsigned int i = 0; i < b.get_num_part(); i++)
+      printf("%u %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p[i]->idx, p[i]->x, p[i]->y, p[i]->vx, p[i]->vy,
+             p[i]->rho, p[i]->Sxx, p[i]->Sxy, p[i]->Syy);
+  }
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: %s rings|tensile nbox steps [mixed]\n", argv[0]);
+    return 2;
+  }
+  const unsigned int nbox = (unsigned int)atoi(argv[2]), steps = (unsigned int)atoi(argv[3]);
+  if (argc > 4 && !strcmp(argv[4], "mixed")) mfree_adapter::precision() = MFREE_PRECISION_MIXED;
+  if (!strcmp(argv[1], "rings")) return run(rings_monaghan(nbox), steps);
+  if (!strcmp(argv[1], "tensile")) return run(tensile_tl(nbox), steps);
+  fprintf(stderr, "unknown driver %s\n", argv[1]);
+  return 2;
+}
