@@ -21,8 +21,9 @@ def tl_solver(pl, precision=PRECISION_FP64, bc=True, ps=None):
     return sv
 
 
-def oracle_tl(pl, bc=True):
-    t = O.TL(pl.particles, float(pl.particles.h[0]), search=1, contrib_only=1)
+def oracle_tl(pl, bc=True, ps=None):
+    ps = pl.particles if ps is None else ps
+    t = O.TL(ps, float(ps.h[0]), search=1, contrib_only=1)
     assert t.rc == O.OK
     if bc:
         t.set_bc(pl.left, pl.right, pl.v_bc)
@@ -61,7 +62,7 @@ def test_tl_derivatives_fp64_vs_oracle(seed):
     pl = plate()
     ps = perturbed(pl, seed)
     print(f"[tl deriv] n={ps.n} seed={seed}", flush=True)
-    t = oracle_tl(pl)
+    t = oracle_tl(pl, ps=ps)
     with tl_solver(pl, ps=ps) as sv:
         sv.tl_precompute()
         d8, aux = sv.derivatives()
