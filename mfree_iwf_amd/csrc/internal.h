@@ -11,6 +11,18 @@ constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr uint32_t kChunk = 2048;    // particles per bbox-reduction chunk
 constexpr uint32_t kScanTile = 1024; // elements per scan workgroup
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Kernels whose neighbouring workgroups share data remap the block
+// index so that XCD x processes the contiguous range of tiles
+// [x*per, (x+1)*per); the grid is padded to a multiple of 8 and blocks
+// mapped past `ntiles` exit.
+constexpr uint32_t kXcds = 8;
+__host__ __device__ inline uint32_t xcd_grid(uint32_t ntiles) { return (ntiles + kXcds - 1) / kXcds * kXcds; }
+__device__ inline uint32_t xcd_tile() {
+  const uint32_t per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 // error bits raised by kernels (mapped to mfree_status on the host)
 enum : uint32_t {
   EB_CELLS = 1u,      // num_cell >= n, or a hash outside the grid

@@ -117,7 +117,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   using T = typename TlTypes<MIXED>::T;
   using A = typename TlTypes<MIXED>::A;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const SceneConst* __restrict__ c = a.sc + a.scene[i];
   const double t = c->time;
@@ -202,7 +202,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   using T = typename TlTypes<MIXED>::T;
   using A = typename TlTypes<MIXED>::A;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const SceneConst* __restrict__ c = a.sc + a.scene[i];
   const double t = c->time;
@@ -500,11 +500,11 @@ TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stag
 
 void launch_stage(const TlShared& s, const TlArgs& a) {
   if (s.precision == MFREE_PRECISION_MIXED) {
-    hipLaunchKernelGGL(k_tl_pass1<true>, dim3(nblk(a.n)), dim3(kBlock), 0, s.st, a);
-    hipLaunchKernelGGL(k_tl_pass2<true>, dim3(nblk(a.n)), dim3(kBlock), 0, s.st, a);
+    hipLaunchKernelGGL(k_tl_pass1<true>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
+    hipLaunchKernelGGL(k_tl_pass2<true>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
   } else {
-    hipLaunchKernelGGL(k_tl_pass1<false>, dim3(nblk(a.n)), dim3(kBlock), 0, s.st, a);
-    hipLaunchKernelGGL(k_tl_pass2<false>, dim3(nblk(a.n)), dim3(kBlock), 0, s.st, a);
+    hipLaunchKernelGGL(k_tl_pass1<false>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
+    hipLaunchKernelGGL(k_tl_pass2<false>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
   }
 }
 

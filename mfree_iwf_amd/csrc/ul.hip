@@ -210,7 +210,7 @@ constexpr int kNbCap = 32;
 __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
   const uint32_t s = mt.y;
@@ -426,7 +426,7 @@ void launch_ul_pairs(hipStream_t st, const UlArgs& a) {
   if (a.mixed)
     launch_ul_pairs32(st, a);
   else
-    hipLaunchKernelGGL(k_ul_pairs64, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_ul_pairs64, dim3(xcd_grid(nblk(a.n, kBlock))), dim3(kBlock), 0, st, a);
 }
 
 void launch_pair_list(hipStream_t st, const PairListArgs& a) {

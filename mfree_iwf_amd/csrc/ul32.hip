@@ -51,7 +51,7 @@ constexpr int kNbCap = 32;
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
   const uint32_t s = mt.y;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
 }
 
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {
-  hipLaunchKernelGGL(k_ul_pairs32, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL(k_ul_pairs32, dim3(xcd_grid((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
 }
 
 }  // namespace mfree
