@@ -112,6 +112,33 @@ def cpu_baseline(scene, steps, log):
                 seconds=dt)
 
 
+def validate_fp64(scene, nsteps, log):
+    """FP64 GPU mode vs the oracle (stable order, all host threads) after
+    nsteps RK4 steps of the full workload: the 1e-10 rel-L2 gate."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from mfree_iwf_amd import PRECISION_FP64, Solver
+    t0 = time.perf_counter()
+    with Solver(precision=PRECISION_FP64) as sv:
+        sv.add_body(scene.particles, scene.params)
+        sv.step(nsteps)
+        g = sv.state(0)
+    tg = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    rc, o = O.ul_step(scene.particles, scene.params, nsteps, order=O.ORDER_STABLE, threads=threads)
+    to = time.perf_counter() - t0
+    if rc != O.OK:
+        raise RuntimeError(f"oracle failed with status {rc}")
+    res = dict(steps=nsteps, particles=scene.particles.n, oracle_threads=threads, gpu_s=tg, oracle_s=to,
+               rel_l2_pos=rel_l2(np.concatenate([g.x, g.y]), np.concatenate([o.x, o.y])),
+               rel_l2_stress=rel_l2(np.concatenate([g.Sxx, g.Sxy, g.Syy]), np.concatenate([o.Sxx, o.Sxy, o.Syy])),
+               rel_l2_vel=rel_l2(np.concatenate([g.vx, g.vy]), np.concatenate([o.vx, o.vy])),
+               max_abs_stress_pa=float(np.abs(np.concatenate([o.Sxx, o.Sxy, o.Syy])).max()))
+    log(f"fp64 validation: {res}")
+    return res
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     if os.path.exists(p):
@@ -132,6 +159,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB mebibytes (HBM counter calibration)")
     ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--validate-fp64-steps", type=int, default=0,
+                    help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
     args = ap.parse_args()
 
     def log(msg):
@@ -149,6 +178,11 @@ def main():
     scene = scenes.rings_config2()
     n = scene.particles.n
     log(f"workload: {n} particles, dt = {scene.params.dt:.4e} s (setup {time.perf_counter() - t0:.1f} s)")
+
+    if args.validate_fp64_steps > 0:
+        print(json.dumps({"validation": "fp64 GPU vs oracle", **validate_fp64(scene, args.validate_fp64_steps, log)}),
+              flush=True)
+        return
 
     calib = None
     if args.calibrate_copy > 0:
