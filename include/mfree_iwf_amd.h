@@ -85,6 +85,7 @@ typedef struct mfree_options {
   int32_t use_graph;        /* capture RK4 steps in a HIP graph */
   int32_t deterministic;    /* 1: within-cell order by idx (bit-reproducible) */
   double tl_h_search;       /* TL: h passed to find_neighbors; <= 0: per-particle h */
+  int32_t tile_cells;       /* mixed UL: cells per LDS pair tile; 0 = autotune, < 0 = no tiles */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
@@ -179,6 +180,10 @@ mfree_status mfree_profile_read(mfree_ctx* ctx, double* pair_ms_total, uint64_t*
  * `reps` times on `device`; *gbps = 2*bytes*reps / elapsed.  Used to scale
  * the FETCH_SIZE / WRITE_SIZE counters of the pair kernels. */
 mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps, double* gbps);
+
+/* Tile size (cells per LDS tile) the mixed UL pair pass uses, after the
+ * autotuner ran (first compute call); 0 when tiles are not used. */
+int32_t mfree_tile_cells(const mfree_ctx* ctx);
 
 /* Total particle count and per-body counts. */
 uint64_t mfree_num_particles(const mfree_ctx* ctx);

@@ -39,7 +39,7 @@ class Params(C.Structure):
 class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
-                ("tl_h_search", C.c_double)]
+                ("tl_h_search", C.c_double), ("tile_cells", C.c_int32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")
@@ -86,6 +86,7 @@ SIGNATURES = {
     "mfree_profile_read": (C.c_int32, [C.c_void_p, dptr, C.POINTER(C.c_uint64), dptr,
                                        C.POINTER(C.c_uint64)]),
     "mfree_calibrate_copy": (C.c_int32, [C.c_int32, C.c_uint64, C.c_uint32, dptr]),
+    "mfree_tile_cells": (C.c_int32, [C.c_void_p]),
     "mfree_num_particles": (C.c_uint64, [C.c_void_p]),
     "mfree_num_bodies": (C.c_uint32, [C.c_void_p]),
     "mfree_scene_rings_count": (C.c_uint32, [C.POINTER(RingsSpec)]),

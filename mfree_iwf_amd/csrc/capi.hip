@@ -51,6 +51,9 @@ struct mfree_ctx {
   SceneConst* sc = nullptr;
   SceneGeom* geom = nullptr;
   std::vector<SceneConst> h_sc;
+  uint32_t* tile_off = nullptr;  // nscene+1
+  uint32_t tile_cells = 0;       // 0: per-particle pair kernel
+  uint32_t tile_grid = 0;
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
 
@@ -141,6 +144,8 @@ mfree_status check_err_bits(mfree_ctx* c) {
   return fail(c, MFREE_ERR_HIP, "unknown device error bits");
 }
 
+mfree_status autotune_tiles(mfree_ctx* c);
+
 mfree_status commit(mfree_ctx* c) {
   if (c->committed) return MFREE_OK;
   c->S = (uint32_t)c->prm.size();
@@ -198,6 +203,7 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, dalloc(c, &c->perm, N));
   HIPCHK(c, dalloc(c, &c->tsum, (N + kScanTile - 1) / kScanTile + 1));
   HIPCHK(c, dalloc(c, &c->meta, N));
+  HIPCHK(c, dalloc(c, &c->tile_off, c->S + 1));
   if (!tl) {
     if (c->opt.precision == MFREE_PRECISION_MIXED) {
       HIPCHK(c, dalloc(c, &c->pos32, N));
@@ -223,11 +229,21 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, hipMemcpy(c->scene[0], sc.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->gid[0], gid.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
   c->cur = 0;
+  c->tile_grid = (uint32_t)std::max(8, prop.multiProcessorCount) * 2;
+  c->tile_grid = (c->tile_grid + kXcds - 1) / kXcds * kXcds;
   if (tl) {
     mfree_status st = tl_alloc(c->tl, (uint32_t)N, c->hX.data(), c->hY.data(), c->bc_left, c->bc_right, c->off);
     if (st != MFREE_OK) return fail(c, st, "TL allocation failed");
   }
   c->committed = true;
+  if (!tl && c->opt.precision == MFREE_PRECISION_MIXED) {
+    if (c->opt.tile_cells > 0) {
+      c->tile_cells = (uint32_t)c->opt.tile_cells;
+    } else if (c->opt.tile_cells == 0) {
+      mfree_status st = autotune_tiles(c);
+      if (st != MFREE_OK) return st;
+    }
+  }
   return MFREE_OK;
 }
 
@@ -257,6 +273,8 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.cell_start = c->cell_start;
   a.perm = c->perm;
   a.err = c->errb;
+  a.tile_cells = c->tile_cells;
+  a.tile_off = c->tile_cells ? c->tile_off : nullptr;
   return a;
 }
 
@@ -284,6 +302,10 @@ UlArgs ul_args(mfree_ctx* c, int b) {
   u.acc = c->acc;
   u.max_nbh = u.mixed ? 0 : c->opt.max_nbh;
   u.err = c->errb;
+  u.nscene = c->S;
+  u.tile_cells = c->tile_cells;
+  u.tile_off = c->tile_cells ? c->tile_off : nullptr;
+  u.tile_grid = c->tile_grid;
   return u;
 }
 
@@ -339,6 +361,60 @@ void pair_launch(mfree_ctx* c, const UlArgs& u, std::vector<std::pair<hipEvent_t
     hipEventRecord(b, c->stream);
     evs->push_back({a, b});
   }
+}
+
+// Project autotuner for the mixed UL pair pass: times one complete stage
+// (binning, prepass, pair pass in derivative-evaluation mode) of the actual
+// bodies for each candidate tile size and keeps the fastest.  Results are
+// cached per (particle-count bucket, body count) for the process.
+mfree_status autotune_tiles(mfree_ctx* c) {
+  static std::vector<std::pair<uint64_t, uint32_t>> cache;
+  const uint64_t key = ((uint64_t)(64 - __builtin_clzll((uint64_t)c->N | 1)) << 32) | c->S;
+  for (auto& e : cache)
+    if (e.first == key) {
+      c->tile_cells = e.second;
+      return MFREE_OK;
+    }
+  double* d8 = nullptr;
+  HIPCHK(c, hipMalloc(&d8, 8 * (size_t)c->N * sizeof(double)));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  const uint32_t cand[] = {0, 8, 12, 16, 24, 32};
+  float best = 1e30f;
+  uint32_t best_tc = 0;
+  const int b = c->cur;
+  for (uint32_t tc : cand) {
+    c->tile_cells = tc;
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 4; rep++) {
+      hipEventRecord(e0, c->stream);
+      launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+      UlArgs u = ul_args(c, b);
+      u.src = c->y[b];
+      launch_ul_prepass(c->stream, u);
+      u.stage = 0;
+      u.d8 = d8;
+      launch_ul_pairs(c->stream, u);
+      hipEventRecord(e1, c->stream);
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) tmin = std::min(tmin, ms);  // first run warms up
+    }
+    if (tmin < best) {
+      best = tmin;
+      best_tc = tc;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(d8);
+  c->tile_cells = best_tc;
+  cache.push_back({key, best_tc});
+  uint32_t zero = 0;
+  HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+  return MFREE_OK;
 }
 
 // Enqueues one RK4 step of every UL body (src/rk4.h:71-108 with the UL
@@ -505,6 +581,7 @@ void mfree_default_options(mfree_options* o) {
   o->use_graph = 1;
   o->deterministic = 1;
   o->tl_h_search = 0.;
+  o->tile_cells = 0;
 }
 
 mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
@@ -837,6 +914,8 @@ mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps,
   if (b) hipFree(b);
   return rc;
 }
+
+int32_t mfree_tile_cells(const mfree_ctx* c) { return c ? (int32_t)c->tile_cells : 0; }
 
 uint64_t mfree_num_particles(const mfree_ctx* c) { return c ? c->off.back() : 0; }
 uint32_t mfree_num_bodies(const mfree_ctx* c) { return c ? (uint32_t)c->prm.size() : 0; }

@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
 __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restrict__ chunk_first,
                                                        const SceneConst* __restrict__ sc,
                                                        const double* __restrict__ part, double kernel_width,
-                                                       double cell_min, SceneGeom* __restrict__ geom,
-                                                       uint32_t* __restrict__ err) {
+                                                       double cell_min, uint32_t tile_cells,
+                                                       SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
   const uint32_t s = blockIdx.x;
   double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
   for (uint32_t c = chunk_first[s] + threadIdx.x; c < chunk_first[s + 1]; c += blockDim.x) {
@@ -152,6 +152,9 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
     g.ny = (unsigned int)fny;
     g.num_cell = g.nx * g.ny;
   }
+  g.tpc = tile_cells ? (g.ny + tile_cells - 1) / tile_cells : 0;
+  g.ntiles = g.nx * g.tpc;
+  g.pad0 = g.pad1 = 0;
   geom[s] = g;
 }
 
@@ -281,6 +284,23 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = total;
 }
 
+// One workgroup: first pair tile of every body (exclusive scan of the
+// per-body tile counts); tile_off[nscene] = total.
+__global__ __launch_bounds__(kBlock) void k_tile_offsets(uint32_t nscene, const SceneGeom* __restrict__ geom,
+                                                         uint32_t* __restrict__ tile_off) {
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < nscene; b += kBlock) {
+    const uint32_t s = b + threadIdx.x;
+    const uint32_t v = s < nscene ? geom[s].ntiles : 0;
+    uint32_t ex;
+    const uint32_t tot = block_excl_scan(v, &ex);
+    if (s < nscene) tile_off[s] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tile_off[nscene] = carry;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ rank,
                                                     const uint32_t* __restrict__ cell_start,
@@ -335,7 +355,9 @@ void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n) {
 void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part);
   hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part,
-                     a.kernel_width, a.cell_min, a.geom, a.err);
+                     a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err);
+  if (a.tile_cells && a.tile_off)
+    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kBlock), 0, st, a.nscene, a.geom, a.tile_off);
   launch_fill_u32(st, a.count, 0u, a.ncell_cap);
   hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
                      a.geom, a.key, a.rank, a.count, a.err);

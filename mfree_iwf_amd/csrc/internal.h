@@ -47,6 +47,7 @@ struct SceneConst {
 struct SceneGeom {
   double bbmin_x, bbmin_y, cell;
   uint32_t nx, ny, num_cell, err;
+  uint32_t tpc, ntiles, pad0, pad1;  // pair tiles: per cell column, in the body
 };
 
 struct Chunk {

@@ -26,6 +26,8 @@ struct BinArgs {
   double* part;                 // 5 per chunk
   uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
   uint32_t* err;
+  uint32_t tile_cells;          // > 0: also enumerate pair tiles of this many cells
+  uint32_t* tile_off;           // nscene+1: first tile of each body, [nscene] = total
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
@@ -65,6 +67,11 @@ struct UlArgs {
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;
+  // LDS-tiled pair pass (mixed precision)
+  uint32_t nscene;
+  uint32_t tile_cells;          // 0: per-particle kernel without tiles
+  const uint32_t* tile_off;
+  uint32_t tile_grid;           // workgroups of the tile kernel (multiple of 8)
 };
 
 void launch_ul_prepass(hipStream_t st, const UlArgs& a);

@@ -10,6 +10,17 @@
 //    (sum, compensation) pair;
 //  - the per-particle epilogue (Jaumann rate, continuity, momentum,
 //    advection) and the RK4 state update stay FP64.
+//
+// Two kernels:
+//  k_ul_tiles32  a workgroup takes a tile of `tile_cells` consecutive cells
+//                of one cell column, stages every candidate of the three
+//                neighbouring columns (positions and records) in LDS once,
+//                and evaluates its particles from LDS (scan -> compacted
+//                list -> pair terms).  Tiles of all bodies are enumerated on
+//                the device; a fixed grid walks them in XCD-contiguous
+//                ranges.  A tile whose candidates exceed the LDS capacity is
+//                evaluated from global memory instead.
+//  k_ul_pairs32  the same per particle straight from global memory.
 // Compiled with FMA contraction enabled (FP32 math only in this file).
 #include "kernels.h"
 
@@ -32,6 +43,120 @@ __device__ inline void acc_add(Acc& a, float x) {
 
 __device__ inline double acc_val(const Acc& a) { return (double)a.s - (double)a.c; }
 
+struct Sums {
+  Acc Sxx_x{0.f, 0.f}, Sxy_y{0.f, 0.f}, Sxy_x{0.f, 0.f}, Syy_y{0.f, 0.f};
+  Acc vx_x{0.f, 0.f}, vx_y{0.f, 0.f}, vy_x{0.f, 0.f}, vy_y{0.f, 0.f};
+  Acc ax{0.f, 0.f}, ay{0.f, 0.f}, xs{0.f, 0.f}, ys{0.f, 0.f};
+};
+
+// What a neighbour contributes (sigma pre-multiplied by 1/rho^2).
+struct Nb {
+  float vx, vy, rho, qw, m, h, c, sxx, sxy, syy, Rxx, Rxy, Ryy;
+};
+
+// Constants of the particle whose sums are accumulated.
+struct Me {
+  float vx, vy, rho, m, h, c, sxx, sxy, syy, Rxx, Rxy, Ryy;
+  float h1, fac, cut2, inv_wdp, corr_exp, alpha, beta, eta2, xeps_mi;
+  bool use_as, pow4;
+};
+
+__device__ inline Me make_me(const SceneConst* __restrict__ c, const Nb& r) {
+  Me me;
+  me.vx = r.vx;
+  me.vy = r.vy;
+  me.rho = r.rho;
+  me.m = r.m;
+  me.h = r.h;
+  me.c = r.c;
+  me.sxx = r.sxx;
+  me.sxy = r.sxy;
+  me.syy = r.syy;
+  me.Rxx = r.Rxx;
+  me.Rxy = r.Rxy;
+  me.Ryy = r.Ryy;
+  me.h1 = 1.f / r.h;
+  me.fac = (float)(10 * (M_1_PI) / 7.0) * me.h1 * me.h1;
+  me.cut2 = 4.0001f * (r.h * r.h);
+  me.inv_wdp = (float)(1.0 / c->wdeltap);
+  me.corr_exp = (float)c->corr_exp;
+  me.use_as = c->wdeltap > 0;
+  me.pow4 = c->pow4 != 0;
+  me.alpha = (float)c->alpha;
+  me.beta = (float)c->beta;
+  me.eta2 = (float)(c->eta * c->eta);
+  me.xeps_mi = (float)c->xsph_eps * r.m;
+  return me;
+}
+
+__device__ inline Nb nb_from_rec(const PairRec32& r) {
+  Nb n;
+  n.vx = r.vx;
+  n.vy = r.vy;
+  n.rho = r.rho;
+  n.qw = r.qw;
+  n.m = r.m;
+  n.h = r.h;
+  n.c = r.c;
+  n.sxx = r.sxx * r.rho21;
+  n.sxy = r.sxy * r.rho21;
+  n.syy = r.syy * r.rho21;
+  n.Rxx = r.Rxx;
+  n.Rxy = r.Rxy;
+  n.Ryy = r.Ryy;
+  return n;
+}
+
+// All four pair sums of one neighbour at separation (xij, yij), r2 = |x_ij|^2.
+__device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
+  const float rij = sqrtf(r2);
+  const float q = rij * me.h1;
+  if (q >= 2.f) return;
+  float w, der;
+  if (q >= 1.f) {
+    const float t = 2.f - q;
+    w = me.fac * (0.25f * t * t * t);
+    der = -0.75f * t * t * me.h1;
+  } else {
+    w = me.fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+    der = -3.0f * q * (1.f - 0.75f * q) * me.h1;
+  }
+  const float gsc = rij > 1e-12f ? der * me.fac / rij : 0.f;
+  const float wx = xij * gsc, wy = yij * gsc;
+  float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
+  if (me.use_as) {
+    float fab = w * me.inv_wdp;
+    fab = me.pow4 ? (fab * fab) * (fab * fab) : powf(fab, me.corr_exp);
+    Rxx = fab * (me.Rxx + nj.Rxx);
+    Rxy = fab * (me.Rxy + nj.Rxy);
+    Ryy = fab * (me.Ryy + nj.Ryy);
+  }
+  const float mj = nj.m;
+  acc_add(S.Sxx_x, mj * (me.sxx + nj.sxx + Rxx) * wx);
+  acc_add(S.Sxy_y, mj * (me.sxy + nj.sxy + Rxy) * wy);
+  acc_add(S.Sxy_x, mj * (me.sxy + nj.sxy + Rxy) * wx);
+  acc_add(S.Syy_y, mj * (me.syy + nj.syy + Ryy) * wy);
+  const float dvx = nj.vx - me.vx, dvy = nj.vy - me.vy;
+  acc_add(S.vx_x, dvx * wx * nj.qw);
+  acc_add(S.vx_y, dvx * wy * nj.qw);
+  acc_add(S.vy_x, dvy * wx * nj.qw);
+  acc_add(S.vy_y, dvy * wy * nj.qw);
+  const float vp = -(dvx * xij + dvy * yij);
+  if (!(vp >= 0.f)) {
+    const float cij = 0.5f * (me.c + nj.c);
+    const float hij = 0.5f * (me.h + nj.h);
+    const float rhoij = 0.5f * (me.rho + nj.rho);
+    const float muij = (hij * vp) / (r2 + me.eta2 * hij * hij);
+    const float piij = (-me.alpha * cij * muij + me.beta * muij * muij) / rhoij;
+    acc_add(S.ax, -mj * piij * wx);
+    acc_add(S.ay, -mj * piij * wy);
+  }
+  const float rhoij = 0.5f * (me.rho + nj.rho);
+  const float fx = -me.xeps_mi * w / rhoij;
+  acc_add(S.xs, -fx * dvx);
+  acc_add(S.ys, -fx * dvy);
+}
+
 __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
                                  double Sxy, double Syy, double& t00, double& t01, double& t11) {
   const double e01 = 0.5 * (vx_y + vy_x);
@@ -43,141 +168,21 @@ __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_
   t11 = (G2 * (vy_y - s3) - Sxy * w01) + w10 * Sxy;
 }
 
-}  // namespace
-
-constexpr int kNbCap = 32;
-
-
-__global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
-  __shared__ uint32_t nb_lds[kNbCap * kBlock];
-  uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const uint2 mt = a.meta[i];
-  const uint32_t s = mt.y;
-  const SceneConst* __restrict__ c = a.sc + s;
-  const SceneGeom g = a.geom[s];
-  const uint32_t local = mt.x - c->cell_offset;
-  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
-  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
-  const float cellf = (float)g.cell;
-
-  const float2 pi = a.pos32[i];
-  const PairRec32 ri = a.rec32[i];
-  const float hi = ri.h;
-  const float h1 = 1.f / hi;
-  const float fac = (float)(10 * (M_1_PI) / 7.0) * h1 * h1;
-  const float cut2 = 4.0001f * (hi * hi);
-  const float inv_wdp = (float)(1.0 / c->wdeltap);
-  const float corr_exp = (float)c->corr_exp;
-  const bool use_as = c->wdeltap > 0;
-  const bool pow4 = c->pow4 != 0;
-  const float alpha = (float)c->alpha, beta = (float)c->beta, eta2 = (float)(c->eta * c->eta);
-  const float xeps_mi = (float)c->xsph_eps * ri.m;
-  const float sxxi_r = ri.sxx * ri.rho21, sxyi_r = ri.sxy * ri.rho21, syyi_r = ri.syy * ri.rho21;
-
-  Acc Sxx_x{0.f, 0.f}, Sxy_y{0.f, 0.f}, Sxy_x{0.f, 0.f}, Syy_y{0.f, 0.f};
-  Acc vx_x{0.f, 0.f}, vx_y{0.f, 0.f}, vy_x{0.f, 0.f}, vy_y{0.f, 0.f};
-  Acc ax{0.f, 0.f}, ay{0.f, 0.f}, xs{0.f, 0.f}, ys{0.f, 0.f};
-  int nb = 0;
-
-  auto pair = [&](uint32_t e) {
-    const uint32_t j = e & 0x0FFFFFFFu;
-    const float offx = (float)((int)((e >> 28) & 3u) - 1) * cellf;
-    const float offy = (float)((int)(e >> 30) - 1) * cellf;
-    const float2 pj = a.pos32[j];
-    const float xij = (pi.x - pj.x) + offx;
-    const float yij = (pi.y - pj.y) + offy;
-    const float r2 = xij * xij + yij * yij;
-    const float rij = sqrtf(r2);
-    const float q = rij * h1;
-    if (q >= 2.f) return;
-    float w, der;
-    if (q >= 1.f) {
-      const float t = 2.f - q;
-      w = fac * (0.25f * t * t * t);
-      der = -0.75f * t * t * h1;
-    } else {
-      w = fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
-      der = -3.0f * q * (1.f - 0.75f * q) * h1;
-    }
-    const float gsc = rij > 1e-12f ? der * fac / rij : 0.f;
-    const float wx = xij * gsc, wy = yij * gsc;
-    const PairRec32 rj = a.rec32[j];
-    float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
-    if (use_as) {
-      float fab = w * inv_wdp;
-      fab = pow4 ? (fab * fab) * (fab * fab) : powf(fab, corr_exp);
-      Rxx = fab * (ri.Rxx + rj.Rxx);
-      Rxy = fab * (ri.Rxy + rj.Rxy);
-      Ryy = fab * (ri.Ryy + rj.Ryy);
-    }
-    const float mj = rj.m;
-    acc_add(Sxx_x, mj * (sxxi_r + rj.sxx * rj.rho21 + Rxx) * wx);
-    acc_add(Sxy_y, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wy);
-    acc_add(Sxy_x, mj * (sxyi_r + rj.sxy * rj.rho21 + Rxy) * wx);
-    acc_add(Syy_y, mj * (syyi_r + rj.syy * rj.rho21 + Ryy) * wy);
-    const float dvx = rj.vx - ri.vx, dvy = rj.vy - ri.vy;
-    acc_add(vx_x, dvx * wx * rj.qw);
-    acc_add(vx_y, dvx * wy * rj.qw);
-    acc_add(vy_x, dvy * wx * rj.qw);
-    acc_add(vy_y, dvy * wy * rj.qw);
-    const float vp = -(dvx * xij + dvy * yij);
-    if (!(vp >= 0.f)) {
-      const float cij = 0.5f * (ri.c + rj.c);
-      const float hij = 0.5f * (hi + rj.h);
-      const float rhoij = 0.5f * (ri.rho + rj.rho);
-      const float muij = (hij * vp) / (r2 + eta2 * hij * hij);
-      const float piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
-      acc_add(ax, -mj * piij * wx);
-      acc_add(ay, -mj * piij * wy);
-    }
-    const float rhoij = 0.5f * (ri.rho + rj.rho);
-    const float fx = -xeps_mi * w / rhoij;
-    acc_add(xs, -fx * dvx);
-    acc_add(ys, -fx * dvy);
-  };
-  auto flush = [&]() {
-    for (int k = 0; k < nb; k++) pair(my[k * kBlock]);
-    nb = 0;
-  };
-
-  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
-    const uint32_t rb = c->cell_offset + ni * g.ny;
-    const float offx = (float)((int)gi - (int)ni) * cellf;
-    uint32_t jb = a.cell_start[rb + lo_j];
-    for (uint32_t nj = lo_j; nj < hi_j; nj++) {
-      const uint32_t ja = jb;
-      jb = a.cell_start[rb + nj + 1];
-      const float offy = (float)((int)gj - (int)nj) * cellf;
-      const uint32_t tag = ((gi - ni + 1) << 28) | ((gj - nj + 1) << 30);
-      for (uint32_t j = ja; j < jb; j++) {
-        const float2 pj = a.pos32[j];
-        const float xij = (pi.x - pj.x) + offx;
-        const float yij = (pi.y - pj.y) + offy;
-        const float r2 = xij * xij + yij * yij;
-        if (j == i || r2 > cut2) continue;
-        my[nb * kBlock] = j | tag;
-        if (++nb == kNbCap) flush();
-      }
-    }
-  }
-  flush();
-
+// FP64 epilogue and RK4 stage update of sorted particle i.
+__device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t i, const Sums& S) {
   const uint32_t home = a.home_identity ? i : a.perm[i];
   const double vxi = a.src.f[2][home], vyi = a.src.f[3][home], rhoi = a.src.f[4][home];
   const double Sxx = a.src.f[5][home], Sxy = a.src.f[6][home], Syy = a.src.f[7][home];
-  const double dSxx_x = acc_val(Sxx_x) * rhoi, dSxy_y = acc_val(Sxy_y) * rhoi;
-  const double dSxy_x = acc_val(Sxy_x) * rhoi, dSyy_y = acc_val(Syy_y) * rhoi;
-  const double gvx_x = acc_val(vx_x), gvx_y = acc_val(vx_y), gvy_x = acc_val(vy_x), gvy_y = acc_val(vy_y);
+  const double dSxx_x = acc_val(S.Sxx_x) * rhoi, dSxy_y = acc_val(S.Sxy_y) * rhoi;
+  const double dSxy_x = acc_val(S.Sxy_x) * rhoi, dSyy_y = acc_val(S.Syy_y) * rhoi;
+  const double gvx_x = acc_val(S.vx_x), gvx_y = acc_val(S.vx_y), gvy_x = acc_val(S.vy_x), gvy_y = acc_val(S.vy_y);
   double t00, t01, t11;
   jaumann64(c->G2, gvx_x, gvx_y, gvy_x, gvy_y, Sxx, Sxy, Syy, t00, t01, t11);
   double d[8];
-  d[0] = acc_val(xs) + vxi;
-  d[1] = acc_val(ys) + vyi;
-  d[2] = acc_val(ax) + 1. / rhoi * (dSxx_x + dSxy_y);
-  d[3] = acc_val(ay) + 1. / rhoi * (dSxy_x + dSyy_y);
+  d[0] = acc_val(S.xs) + vxi;
+  d[1] = acc_val(S.ys) + vyi;
+  d[2] = acc_val(S.ax) + 1. / rhoi * (dSxx_x + dSxy_y);
+  d[3] = acc_val(S.ay) + 1. / rhoi * (dSxy_x + dSyy_y);
   d[4] = -rhoi * (gvx_x + gvy_y);
   d[5] = t00;
   d[6] = t01;
@@ -192,12 +197,13 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {
+      const PairRec32 r = a.rec32[i];
       double* x = a.aux + 16 * (size_t)gid;
-      x[0] = ri.qw;
+      x[0] = r.qw;
       x[1] = c->c0sq * (rhoi - c->rho0);
-      x[2] = ri.Rxx;
-      x[3] = ri.Rxy;
-      x[4] = ri.Ryy;
+      x[2] = r.Rxx;
+      x[3] = r.Rxy;
+      x[4] = r.Ryy;
       x[5] = dSxx_x;
       x[6] = dSxy_y;
       x[7] = dSxy_x;
@@ -236,8 +242,223 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   }
 }
 
+// Sorted particle i straight from global memory, no buffering.
+__device__ inline void particle_global(const UlArgs& a, uint32_t i) {
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+  const float cellf = (float)g.cell;
+  const float2 pi = a.pos32[i];
+  const Me me = make_me(c, nb_from_rec(a.rec32[i]));
+  Sums S;
+  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
+    const uint32_t rb = c->cell_offset + ni * g.ny;
+    const float offx = (float)((int)gi - (int)ni) * cellf;
+    uint32_t jb = a.cell_start[rb + lo_j];
+    for (uint32_t nj = lo_j; nj < hi_j; nj++) {
+      const uint32_t ja = jb;
+      jb = a.cell_start[rb + nj + 1];
+      const float offy = (float)((int)gj - (int)nj) * cellf;
+      for (uint32_t j = ja; j < jb; j++) {
+        if (j == i) continue;
+        const float2 pj = a.pos32[j];
+        const float xij = (pi.x - pj.x) + offx;
+        const float yij = (pi.y - pj.y) + offy;
+        const float r2 = xij * xij + yij * yij;
+        if (r2 > me.cut2) continue;
+        pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
+      }
+    }
+  }
+  finish(a, c, i, S);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
+  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  particle_global(a, i);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged tiles
+
+constexpr int kTileCap = 768;  // staged candidates per tile (10-bit LDS index)
+constexpr int kTileNb = 32;    // compacted neighbours buffered per thread
+
+__global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
+  __shared__ float s_px[kTileCap], s_py[kTileCap];
+  __shared__ float s_vx[kTileCap], s_vy[kTileCap], s_rho[kTileCap], s_qw[kTileCap], s_m[kTileCap];
+  __shared__ float s_h[kTileCap], s_c[kTileCap], s_sxx[kTileCap], s_sxy[kTileCap], s_syy[kTileCap];
+  __shared__ float s_Rxx[kTileCap], s_Rxy[kTileCap], s_Ryy[kTileCap];
+  __shared__ uint16_t s_nb[kTileNb * kBlock];
+  __shared__ uint32_t s_rs[3], s_off[4];
+
+  const uint32_t TC = a.tile_cells;
+  const uint32_t total = a.tile_off[a.nscene];
+  const uint32_t per = gridDim.x / kXcds;
+  const uint32_t xcd = blockIdx.x % kXcds, k0 = blockIdx.x / kXcds;
+  const uint32_t range = (total + kXcds - 1) / kXcds;
+  const uint32_t t_begin = xcd * range, t_end = min(total, t_begin + range);
+  uint16_t* const my = s_nb + threadIdx.x;
+
+  for (uint32_t t = t_begin + k0; t < t_end; t += per) {
+    // body of tile t: last s with tile_off[s] <= t
+    uint32_t lo = 0, hi = a.nscene;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.tile_off[mid] <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t s = lo;
+    const SceneConst* __restrict__ c = a.sc + s;
+    const SceneGeom g = a.geom[s];
+    const uint32_t lt = t - a.tile_off[s];
+    const uint32_t ix = lt / g.tpc;
+    const uint32_t iy0 = (lt - ix * g.tpc) * TC, iy1 = min(iy0 + TC, g.ny);
+    const uint32_t iy_lo = iy0 > 0 ? iy0 - 1 : 0, iy_hi = min(iy1 + 1, g.ny);
+    const uint32_t rb1 = c->cell_offset + ix * g.ny;
+    const uint32_t t0 = a.cell_start[rb1 + iy0], t1 = a.cell_start[rb1 + iy1];
+    if (t0 == t1) continue;  // empty tile (the same for every thread)
+
+    if (threadIdx.x < 3) {
+      const int row = (int)ix - 1 + (int)threadIdx.x;
+      uint32_t st = 0, len = 0;
+      if (row >= 0 && row < (int)g.nx) {
+        const uint32_t rb = c->cell_offset + (uint32_t)row * g.ny;
+        st = a.cell_start[rb + iy_lo];
+        len = a.cell_start[rb + iy_hi] - st;
+      }
+      s_rs[threadIdx.x] = st;
+      s_off[threadIdx.x + 1] = len;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_off[0] = 0;
+      s_off[2] += s_off[1];
+      s_off[3] += s_off[2];
+    }
+    __syncthreads();
+    const uint32_t ncand = s_off[3];
+    const bool staged = ncand <= (uint32_t)kTileCap;
+    if (staged) {
+      for (uint32_t q = threadIdx.x; q < ncand; q += blockDim.x) {
+        const uint32_t r = q < s_off[1] ? 0 : (q < s_off[2] ? 1 : 2);
+        const uint32_t j = s_rs[r] + (q - s_off[r]);
+        const float2 p = a.pos32[j];
+        const Nb n = nb_from_rec(a.rec32[j]);
+        s_px[q] = p.x;
+        s_py[q] = p.y;
+        s_vx[q] = n.vx;
+        s_vy[q] = n.vy;
+        s_rho[q] = n.rho;
+        s_qw[q] = n.qw;
+        s_m[q] = n.m;
+        s_h[q] = n.h;
+        s_c[q] = n.c;
+        s_sxx[q] = n.sxx;
+        s_sxy[q] = n.sxy;
+        s_syy[q] = n.syy;
+        s_Rxx[q] = n.Rxx;
+        s_Rxy[q] = n.Rxy;
+        s_Ryy[q] = n.Ryy;
+      }
+    }
+    __syncthreads();
+    const float cellf = (float)g.cell;
+    for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
+      if (!staged) {
+        particle_global(a, i);
+        continue;
+      }
+      const uint32_t li = s_off[1] + (i - s_rs[1]);
+      Nb ni;
+      ni.vx = s_vx[li];
+      ni.vy = s_vy[li];
+      ni.rho = s_rho[li];
+      ni.qw = s_qw[li];
+      ni.m = s_m[li];
+      ni.h = s_h[li];
+      ni.c = s_c[li];
+      ni.sxx = s_sxx[li];
+      ni.sxy = s_sxy[li];
+      ni.syy = s_syy[li];
+      ni.Rxx = s_Rxx[li];
+      ni.Rxy = s_Rxy[li];
+      ni.Ryy = s_Ryy[li];
+      const Me me = make_me(c, ni);
+      const float pix = s_px[li], piy = s_py[li];
+      const uint32_t gj = (a.meta[i].x - c->cell_offset) - ix * g.ny;
+      const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+      Sums S;
+      int nb = 0;
+      auto flush = [&]() {
+        for (int k = 0; k < nb; k++) {
+          const uint32_t e = my[k * kBlock];
+          const uint32_t lj = e & 1023u;
+          const float xij = (pix - s_px[lj]) + (float)((int)((e >> 10) & 3u) - 1) * cellf;
+          const float yij = (piy - s_py[lj]) + (float)((int)((e >> 12) & 3u) - 1) * cellf;
+          Nb nj;
+          nj.vx = s_vx[lj];
+          nj.vy = s_vy[lj];
+          nj.rho = s_rho[lj];
+          nj.qw = s_qw[lj];
+          nj.m = s_m[lj];
+          nj.h = s_h[lj];
+          nj.c = s_c[lj];
+          nj.sxx = s_sxx[lj];
+          nj.sxy = s_sxy[lj];
+          nj.syy = s_syy[lj];
+          nj.Rxx = s_Rxx[lj];
+          nj.Rxy = s_Rxy[lj];
+          nj.Ryy = s_Ryy[lj];
+          pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
+        }
+        nb = 0;
+      };
+      for (uint32_t r = 0; r < 3; r++) {
+        const int row = (int)ix - 1 + (int)r;
+        if (row < 0 || row >= (int)g.nx) continue;
+        const uint32_t rb = c->cell_offset + (uint32_t)row * g.ny;
+        const float offx = (float)(1 - (int)r) * cellf;
+        const uint32_t base = s_off[r] - s_rs[r];
+        uint32_t jb = a.cell_start[rb + lo_j];
+        for (uint32_t nj = lo_j; nj < hi_j; nj++) {
+          const uint32_t ja = jb;
+          jb = a.cell_start[rb + nj + 1];
+          const uint32_t dj = gj + 1 - nj;  // gj - nj + 1 in {0, 1, 2}
+          const float offy = (float)((int)dj - 1) * cellf;
+          const uint32_t tag = ((2u - r) << 10) | (dj << 12);
+          for (uint32_t j = ja; j < jb; j++) {
+            if (j == i) continue;
+            const uint32_t lj = base + j;
+            const float xij = (pix - s_px[lj]) + offx;
+            const float yij = (piy - s_py[lj]) + offy;
+            if (xij * xij + yij * yij > me.cut2) continue;
+            my[nb * kBlock] = (uint16_t)(lj | tag);
+            if (++nb == kTileNb) flush();
+          }
+        }
+      }
+      flush();
+      finish(a, c, i, S);
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {
-  hipLaunchKernelGGL(k_ul_pairs32, dim3(xcd_grid((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
+  if (a.tile_cells && a.tile_off)
+    hipLaunchKernelGGL(k_ul_tiles32, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_ul_pairs32, dim3(xcd_grid((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
 }
 
 }  // namespace mfree

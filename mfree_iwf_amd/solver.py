@@ -13,7 +13,7 @@ from .scenes import ParticleSet
 
 class Solver:
     def __init__(self, method=_capi.METHOD_UL_MONAGHAN, precision=_capi.PRECISION_FP64, use_graph=True,
-                 deterministic=True, max_nbh=200, device=0, tl_h_search=0.0):
+                 deterministic=True, max_nbh=200, device=0, tl_h_search=0.0, tile_cells=0):
         L = _capi.lib()
         opt = _capi.Options()
         L.mfree_default_options(C.byref(opt))
@@ -24,6 +24,7 @@ class Solver:
         opt.max_nbh = max_nbh
         opt.device = device
         opt.tl_h_search = tl_h_search
+        opt.tile_cells = tile_cells
         h = C.c_void_p()
         _capi.check(L.mfree_create(C.byref(opt), C.byref(h)), what="mfree_create")
         self._h = h
@@ -115,6 +116,11 @@ class Solver:
                                           col.ctypes.data_as(_capi.u32ptr), col.size, C.byref(tot)),
                   "mfree_get_pairs")
         return row, col[:tot.value]
+
+    @property
+    def tile_cells(self):
+        """Cells per LDS tile of the mixed pair pass (0: no tiles)."""
+        return int(self._L.mfree_tile_cells(self._h))
 
     # -- profiling
     def profile(self, on=True):

@@ -186,3 +186,42 @@ def test_nonfinite_detected():
         with pytest.raises(MfreeError) as ei:
             sv.step(1)
     assert ei.value.status == _capi.ERR_NONFINITE
+
+
+@pytest.mark.parametrize("tc", [1, 5, 16, 32])
+def test_mixed_tiles_match_untiled(tc):
+    # the LDS-staged tile kernel and the per-particle kernel evaluate the same
+    # FP32 sums (only the order of loads differs)
+    s = perturbed_rings(nbox=81, seed=14, dv=5.0, dS=5e5, shuffle=True)
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=tc) as sv:
+        assert sv.tile_cells == tc
+        d8, _ = sv.derivatives(with_aux=False)
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=-1) as sv:
+        assert sv.tile_cells == 0
+        e8, _ = sv.derivatives(with_aux=False)
+    rc, od8, _ = O.ul_derivatives(s.particles, s.params)
+    for k in range(8):
+        assert O.rel_l2(d8[:, k], e8[:, k]) < 1e-6, k
+        assert O.rel_l2(d8[:, k], od8[:, k]) < 2e-4, k
+
+
+def test_mixed_tiles_lds_overflow_fallback():
+    # ~19 particles per cell: the candidates of a 32-cell tile exceed the LDS
+    # capacity and the tile is evaluated from global memory
+    s = perturbed_rings(nbox=41, seed=12, dpos=0.3, hdx=2.2)
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=32) as sv:
+        d8, _ = sv.derivatives(with_aux=False)
+    rc, od8, _ = O.ul_derivatives(s.particles, s.params)
+    for k in range(8):
+        assert O.rel_l2(d8[:, k], od8[:, k]) < 2e-4, k
+
+
+def test_mixed_autotuned_steps_vs_oracle():
+    s = perturbed_rings(nbox=81, seed=15, dv=2.0, dS=1e5, shuffle=True)
+    with solver_for(s, precision=PRECISION_MIXED) as sv:
+        print(f"[autotune] tile_cells={sv.tile_cells}", flush=True)
+        sv.step(20)
+        g = sv.state(0)
+    rc, o = O.ul_step(s.particles, s.params, 20)
+    ep, es = pos_stress_err(g, o)
+    assert ep < 1e-6 and es < 1e-3
