@@ -48,7 +48,8 @@ def run_mode(scene, precision, steps, warmup, log):
         sv.add_body(scene.particles, scene.params)
         t0 = time.perf_counter()
         sv.step(max(warmup, 1))
-        log(f"  warmup {warmup} steps: {time.perf_counter() - t0:.2f} s")
+        log(f"  warmup {warmup} steps: {time.perf_counter() - t0:.2f} s (pair tile cells: {sv.tile_cells})")
+        tile_cells = sv.tile_cells
         # timed region: HIP events around every pair-kernel launch on the
         # context's stream (no graph while they are recorded)
         sv.profile(True)
@@ -71,6 +72,7 @@ def run_mode(scene, precision, steps, warmup, log):
         wall_s=wall,
         pair_kernel_ms=pair_ms,
         pair_launches=prof["pair_launches"],
+        tile_cells=tile_cells,
         state=state,
     )
 

@@ -20,7 +20,9 @@
 //                the device; a fixed grid walks them in XCD-contiguous
 //                ranges.  A tile whose candidates exceed the LDS capacity is
 //                evaluated from global memory instead.
-//  k_ul_pairs32  the same per particle straight from global memory.
+//  k_ul_pairs32  the same per particle straight from global memory, with the
+//                candidates inside the cutoff compacted in LDS first.
+// The autotuner times both (and several tile sizes) on the actual bodies.
 // Compiled with FMA contraction enabled (FP32 math only in this file).
 #include "kernels.h"
 
@@ -242,7 +244,8 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
   }
 }
 
-// Sorted particle i straight from global memory, no buffering.
+// Sorted particle i straight from global memory, no buffering (the
+// tile kernel's path for tiles whose candidates do not fit in LDS).
 __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -279,10 +282,62 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
 
 }  // namespace
 
+constexpr int kNbCap = 32;  // compacted neighbours buffered per thread
+
+// Per particle from global memory: scan the 3x3 block, buffer the candidates
+// inside the cutoff in LDS (sorted position + cell offsets packed in 32
+// bits), then evaluate the pair terms on the compact list with full SIMD
+// occupancy.
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
+  __shared__ uint32_t nb_lds[kNbCap * kBlock];
+  uint32_t* const my = nb_lds + threadIdx.x;
   const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  particle_global(a, i);
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+  const float cellf = (float)g.cell;
+  const float2 pi = a.pos32[i];
+  const Me me = make_me(c, nb_from_rec(a.rec32[i]));
+  Sums S;
+  int nb = 0;
+  auto flush = [&]() {
+    for (int k = 0; k < nb; k++) {
+      const uint32_t e = my[k * kBlock];
+      const uint32_t j = e & 0x0FFFFFFFu;
+      const float2 pj = a.pos32[j];
+      const float xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
+      const float yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
+      pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, xij * xij + yij * yij, S);
+    }
+    nb = 0;
+  };
+  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
+    const uint32_t rb = c->cell_offset + ni * g.ny;
+    const float offx = (float)((int)gi - (int)ni) * cellf;
+    uint32_t jb = a.cell_start[rb + lo_j];
+    for (uint32_t nj = lo_j; nj < hi_j; nj++) {
+      const uint32_t ja = jb;
+      jb = a.cell_start[rb + nj + 1];
+      const float offy = (float)((int)gj - (int)nj) * cellf;
+      const uint32_t tag = ((gi - ni + 1) << 28) | ((gj - nj + 1) << 30);
+      for (uint32_t j = ja; j < jb; j++) {
+        if (j == i) continue;
+        const float2 pj = a.pos32[j];
+        const float xij = (pi.x - pj.x) + offx;
+        const float yij = (pi.y - pj.y) + offy;
+        if (xij * xij + yij * yij > me.cut2) continue;
+        my[nb * kBlock] = j | tag;
+        if (++nb == kNbCap) flush();
+      }
+    }
+  }
+  flush();
+  finish(a, c, i, S);
 }
 
 // ---------------------------------------------------------------------------

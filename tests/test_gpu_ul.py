@@ -194,11 +194,11 @@ def test_mixed_tiles_match_untiled(tc):
     # FP32 sums (only the order of loads differs)
     s = perturbed_rings(nbox=81, seed=14, dv=5.0, dS=5e5, shuffle=True)
     with solver_for(s, precision=PRECISION_MIXED, tile_cells=tc) as sv:
-        assert sv.tile_cells == tc
         d8, _ = sv.derivatives(with_aux=False)
+        assert sv.tile_cells == tc
     with solver_for(s, precision=PRECISION_MIXED, tile_cells=-1) as sv:
-        assert sv.tile_cells == 0
         e8, _ = sv.derivatives(with_aux=False)
+        assert sv.tile_cells == 0
     rc, od8, _ = O.ul_derivatives(s.particles, s.params)
     for k in range(8):
         assert O.rel_l2(d8[:, k], e8[:, k]) < 1e-6, k
