@@ -141,6 +141,37 @@ def validate_fp64(scene, nsteps, log):
     return res
 
 
+def bench_batch(steps, warmup, nscenes, log):
+    """Batched uneven bodies: nscenes ring collisions drawn from seed 0, all
+    advanced by one launch per kernel per RK stage (HIP graph)."""
+    from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver, scenes
+    t0 = time.perf_counter()
+    sc = scenes.batch_config4(nscenes, 0)
+    ntot = sum(s.particles.n for s in sc)
+    log(f"batch: {nscenes} scenes, {ntot} particles (setup {time.perf_counter() - t0:.1f} s)")
+    out = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        with Solver(precision=prec) as sv:
+            for s in sc:
+                sv.add_body(s.particles, s.params)
+            sv.step(max(warmup, 1))
+            sv.profile(True)
+            sv.step(steps)
+            prof = sv.profile_read()
+            sv.profile(False)
+            t0 = time.perf_counter()
+            sv.step(steps)
+            wall_graph = time.perf_counter() - t0
+            tc = sv.tile_cells
+        out[name] = dict(particle_steps_per_s=ntot * steps / (prof["step_ms"] / 1e3),
+                         particle_steps_per_s_graph=ntot * steps / wall_graph,
+                         ms_per_step=prof["step_ms"] / steps,
+                         pair_kernel_ms=prof["pair_ms"] / max(prof["pair_launches"], 1), tile_cells=tc)
+        log(f"  {name}: {out[name]['particle_steps_per_s']:.4e} particle-steps/s "
+            f"({out[name]['ms_per_step']:.3f} ms/step), graph {out[name]['particle_steps_per_s_graph']:.4e}")
+    return ntot, out
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     if os.path.exists(p):
@@ -161,6 +192,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB mebibytes (HBM counter calibration)")
     ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
     args = ap.parse_args()
@@ -176,6 +208,18 @@ def main():
     rank = int(os.environ.get("RANK", "0"))
 
     from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, scenes
+    if args.workload == "batch256":
+        ntot, out = bench_batch(args.steps, args.warmup, 256, log)
+        head = out["mixed"]
+        print(json.dumps({
+            "metric": "particle-steps/s, 256 uneven ring-collision bodies batched on one GPU",
+            "value": head["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"256 ring scenes from seed 0, {ntot} particles, 2k-60k per scene, "
+                                   "mixed precision", "particles": ntot, "mode": "mixed"},
+            "modes": out}), flush=True)
+        return
     t0 = time.perf_counter()
     scene = scenes.rings_config2()
     n = scene.particles.n

@@ -225,3 +225,24 @@ def test_mixed_autotuned_steps_vs_oracle():
     rc, o = O.ul_step(s.particles, s.params, 20)
     ep, es = pos_stress_err(g, o)
     assert ep < 1e-6 and es < 1e-3
+
+
+def test_batched_uneven_scenes_vs_oracle():
+    # bodies of the batched workload (different sizes, speeds, moduli and time
+    # steps) advanced together; every body matches its own oracle run
+    specs = scenes.batch_specs(12, seed=3, nmin=500, nmax=6000)
+    bodies = [scenes.make_rings(sp["nbox"], ri=sp["ri"], ro=sp["ro"], centre=sp["centre"], v0=sp["v0"], E=sp["E"])
+              for sp in specs]
+    print(f"[batch] {len(bodies)} bodies, {sum(b.particles.n for b in bodies)} particles", flush=True)
+    with Solver() as sv:
+        for b in bodies:
+            sv.add_body(b.particles, b.params)
+        sv.step(10)
+        got = [sv.state(k) for k in range(len(bodies))]
+        times = [sv.time(k) for k in range(len(bodies))]
+    for b, g, t in zip(bodies, got, times):
+        rc, o = O.ul_step(b.particles, b.params, 10)
+        assert rc == O.OK
+        ep, es = pos_stress_err(g, o)
+        assert ep <= 1e-10 and es <= 1e-10
+        assert abs(t - 10 * b.params.dt) < 1e-18
