@@ -231,6 +231,98 @@ uint32_t mfree_scene_plate_count(const mfree_plate_spec* spec, uint32_t* n_left,
 mfree_status mfree_scene_plate(const mfree_plate_spec* spec, mfree_particles* out, uint32_t cap,
                                uint32_t* idx_left, uint32_t* idx_right, uint32_t* n_out);
 
+/* ---------------------------------------------------------------------
+ * Learned stepper.  No counterpart in the reference: its only integrator is
+ * the RK4 of src/rk4.h:71-108, which these calls stand in for, 4 steps at a
+ * time, on UL (Gray-Monaghan) contexts.
+ *
+ * One learned step (tau = 4 dt per body):
+ *   D1 = derivatives at the state (the stage-0 pass of the UL pipeline);
+ *   D2 = derivatives after rho and S were advanced by tau*D1;
+ *   base update: S += tau*dS1, rho += tau*drho1, v += tau*a2,
+ *                x += tau*xdot1 + tau^2/2*a2        (FP64)
+ *   plus the network's correction of all 8 fields.  The network is a
+ *   3-layer message-passing net (hidden 128, SiLU) on the radius-2h
+ *   neighbour graph (edge features: kernel weight and gradient, i.e.
+ *   functions of the relative position and distance), with 16
+ *   dimensionless node features (current and previous velocity as their
+ *   difference, stress, density, D1, D2, kernel sum, Courant number).
+ *   Forward and backward run as BF16 MFMA kernels with FP32 accumulation.
+ */
+typedef struct mfree_surrogate mfree_surrogate;
+
+typedef struct mfree_surrogate_options {
+  int32_t device;
+  uint64_t seed;             /* weight initialisation and minibatch order */
+  uint32_t max_batch_nodes;  /* training minibatch cap (whole bodies of one snapshot) */
+  float max_compression;     /* snapshots of bodies with max |rho/rho0 - 1| above this are not trained on */
+} mfree_surrogate_options;
+
+void mfree_surrogate_default_options(mfree_surrogate_options* o);
+mfree_status mfree_surrogate_create(const mfree_surrogate_options* o, mfree_surrogate** out);
+void mfree_surrogate_destroy(mfree_surrogate* s);
+const char* mfree_surrogate_last_error(const mfree_surrogate* s);
+
+/* Training data from rollouts of ctx (any number of bodies): per capture,
+ * `gap` plain RK4 steps, the velocities are recorded, 4 plain steps, the
+ * learned-step inputs at that state, then 4 more plain steps whose result
+ * is the target.  One snapshot = one body at one capture; snapshots whose
+ * body exceeds max_compression are kept out of training (counted by
+ * mfree_surrogate_snapshots are the usable ones).  Recomputes the feature /
+ * target normalisation over the usable snapshots. */
+mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* ctx, uint32_t ncapture, uint32_t gap);
+uint64_t mfree_surrogate_snapshots(const mfree_surrogate* s);
+uint64_t mfree_surrogate_dataset_nodes(const mfree_surrogate* s);
+
+/* `iters` Adam steps (cosine-decayed from lr), each on the whole bodies of
+ * one capture up to max_batch_nodes nodes.  loss: iters floats (normalised
+ * MSE per minibatch) or NULL. */
+mfree_status mfree_surrogate_train(mfree_surrogate* s, uint32_t iters, float lr, float* loss);
+
+/* Records the current velocities as "one learned step ago" (the learned
+ * step's previous-velocity feature). */
+mfree_status mfree_surrogate_record_velocity(mfree_surrogate* s, mfree_ctx* ctx);
+
+/* One learned step of every body of ctx.  use_net = 0: base update only
+ * (the method without the network). */
+mfree_status mfree_surrogate_step(mfree_surrogate* s, mfree_ctx* ctx, int32_t use_net);
+
+typedef struct mfree_judge_stats {
+  uint32_t groups, fallbacks, learned_steps, plain_steps;
+  double max_deviation, last_deviation;
+  double ms_learned, ms_judge, ms_fallback;
+} mfree_judge_stats;
+
+/* nsteps time steps (dt units) of ctx with the learned stepper: 4 plain
+ * warm-up steps, then groups of `group` learned steps; the last one of each
+ * group is judged against 4 plain RK4 steps from the same state (the plain
+ * result is kept).  If the relative L2 deviation of all positions exceeds
+ * `threshold` the group is rolled back and redone with plain steps.  Steps
+ * that do not fill a group run plain. */
+mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* ctx, uint32_t nsteps, uint32_t group,
+                                     double threshold, mfree_judge_stats* stats);
+
+/* Network parameters: flat FP32 (mfree_mpnn_param_count floats, tensors in
+ * mfree_mpnn_layout order, each row-major out x in) and normalisation
+ * (2*16 + 2*8 floats: feature mean, feature std, target mean, target std). */
+uint64_t mfree_mpnn_param_count(void);
+uint32_t mfree_mpnn_layout(int32_t* shapes /* 2 per tensor, or NULL */);
+mfree_status mfree_surrogate_get_params(mfree_surrogate* s, float* w, float* norm);
+mfree_status mfree_surrogate_set_params(mfree_surrogate* s, const float* w, const float* norm);
+
+/* The learned step's graph at the current state of ctx, rows and columns by
+ * global particle index (CSR), edge3 = (w, gx, gy) per edge. */
+mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* ctx, uint32_t* row_ptr, uint32_t* col,
+                                   float* edge3, uint64_t cap, uint64_t* nedges);
+
+/* Gradient check: one forward + backward of the network with weights w on a
+ * host graph (CSR, at most 64*n edges, edge3 = (w, gx, gy) with w
+ * symmetric and (gx, gy) antisymmetric), normalised node features x (n x 16)
+ * and targets t (n x 8).  y: n x 8 outputs; loss: MSE; grad: dloss/dw. */
+mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                                         const float* edge3, const float* x, const float* t, const float* w,
+                                         float* y, float* loss, float* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,3 +8,4 @@ from ._capi import (METHOD_TL_RL, METHOD_UL_MONAGHAN, PRECISION_FP64, PRECISION_
                     Params)
 from .scenes import ParticleSet, cubic_spline, make_plate, make_rings  # noqa: F401
 from .solver import Solver, device_count  # noqa: F401
+from .surrogate import Surrogate  # noqa: F401

@@ -28,6 +28,8 @@ PRECISION_MIXED = 1
 
 dptr = C.POINTER(C.c_double)
 u32ptr = C.POINTER(C.c_uint32)
+fptr = C.POINTER(C.c_float)
+i32ptr = C.POINTER(C.c_int32)
 
 
 class Params(C.Structure):
@@ -61,6 +63,17 @@ class PlateSpec(C.Structure):
 
 
 # name -> (restype, argtypes); every symbol include/mfree_iwf_amd.h declares.
+class SurrogateOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("seed", C.c_uint64), ("max_batch_nodes", C.c_uint32),
+                ("max_compression", C.c_float)]
+
+
+class JudgeStats(C.Structure):
+    _fields_ = [("groups", C.c_uint32), ("fallbacks", C.c_uint32), ("learned_steps", C.c_uint32),
+                ("plain_steps", C.c_uint32), ("max_deviation", C.c_double), ("last_deviation", C.c_double),
+                ("ms_learned", C.c_double), ("ms_judge", C.c_double), ("ms_fallback", C.c_double)]
+
+
 SIGNATURES = {
     "mfree_abi_version": (C.c_int32, []),
     "mfree_status_string": (C.c_char_p, [C.c_int32]),
@@ -95,6 +108,26 @@ SIGNATURES = {
     "mfree_scene_plate_count": (C.c_uint32, [C.POINTER(PlateSpec), u32ptr, u32ptr]),
     "mfree_scene_plate": (C.c_int32, [C.POINTER(PlateSpec), C.POINTER(Particles), C.c_uint32, u32ptr,
                                       u32ptr, C.POINTER(C.c_uint32)]),
+    "mfree_surrogate_default_options": (None, [C.POINTER(SurrogateOptions)]),
+    "mfree_surrogate_create": (C.c_int32, [C.POINTER(SurrogateOptions), C.POINTER(C.c_void_p)]),
+    "mfree_surrogate_destroy": (None, [C.c_void_p]),
+    "mfree_surrogate_last_error": (C.c_char_p, [C.c_void_p]),
+    "mfree_surrogate_collect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "mfree_surrogate_snapshots": (C.c_uint64, [C.c_void_p]),
+    "mfree_surrogate_dataset_nodes": (C.c_uint64, [C.c_void_p]),
+    "mfree_surrogate_train": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_float, fptr]),
+    "mfree_surrogate_record_velocity": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "mfree_surrogate_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "mfree_surrogate_rollout": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
+                                            C.POINTER(JudgeStats)]),
+    "mfree_mpnn_param_count": (C.c_uint64, []),
+    "mfree_mpnn_layout": (C.c_uint32, [i32ptr]),
+    "mfree_surrogate_get_params": (C.c_int32, [C.c_void_p, fptr, fptr]),
+    "mfree_surrogate_set_params": (C.c_int32, [C.c_void_p, fptr, fptr]),
+    "mfree_surrogate_graph": (C.c_int32, [C.c_void_p, C.c_void_p, u32ptr, u32ptr, fptr, C.c_uint64,
+                                          C.POINTER(C.c_uint64)]),
+    "mfree_mpnn_forward_backward": (C.c_int32, [C.c_int32, C.c_uint32, u32ptr, u32ptr, fptr, fptr, fptr, fptr,
+                                                fptr, fptr, fptr]),
 }
 
 _lib = None

@@ -11,74 +11,11 @@
 #include <string>
 #include <vector>
 
-#include "kernels.h"
-#include "tl.h"
+#include "ctx.h"
 
 using namespace mfree;
 
-struct mfree_ctx {
-  mfree_options opt;
-  std::string err;
-  bool committed = false;
-  hipStream_t stream = nullptr;
-
-  // body registry (host)
-  std::vector<mfree_params> prm;
-  std::vector<uint32_t> off{0};
-  std::vector<double> time;
-  std::vector<std::vector<uint32_t>> bc_left, bc_right;
-  std::vector<double> v_bc;
-  // initial host state in idx order, uploaded at commit
-  std::vector<double> hx, hy, hX, hY, hm, hrho, hh, hvx, hvy, hSxx, hSxy, hSyy;
-
-  // device
-  uint32_t N = 0, S = 0, nchunks = 0;
-  std::vector<void*> allocs;
-  Fields8 y[2]{}, K{}, acc{};
-  double *m[2]{}, *h[2]{};
-  uint32_t *scene[2]{}, *gid[2]{};
-  int cur = 0;
-  uint32_t *key = nullptr, *rank = nullptr, *count = nullptr, *cell_start = nullptr, *perm = nullptr;
-  uint32_t *tsum = nullptr, *errb = nullptr;
-  uint2* meta = nullptr;
-  double2* pos64 = nullptr;
-  PairRec64* rec64 = nullptr;
-  float2* pos32 = nullptr;
-  PairRec32* rec32 = nullptr;
-  double* part = nullptr;
-  Chunk* chunks = nullptr;
-  uint32_t* chunk_first = nullptr;
-  SceneConst* sc = nullptr;
-  SceneGeom* geom = nullptr;
-  std::vector<SceneConst> h_sc;
-  uint32_t* tile_off = nullptr;  // nscene+1
-  uint32_t tile_cells = 0;       // 0: per-particle pair kernel
-  uint32_t tile_grid = 0;
-
-  TlState tl;  // Total-Lagrangian frozen lists (method TL)
-
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-
-  bool prof = false;
-  double pair_ms = 0., step_ms = 0.;
-  uint64_t pair_n = 0, steps = 0;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-};
-
-namespace {
-
-mfree_status fail(mfree_ctx* c, mfree_status s, const std::string& msg) {
-  if (c) c->err = msg;
-  return s;
-}
-
-#define HIPCHK(ctx, call)                                                                          \
-  do {                                                                                             \
-    hipError_t e_ = (call);                                                                        \
-    if (e_ != hipSuccess)                                                                          \
-      return fail(ctx, MFREE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
-  } while (0)
+namespace mfree {
 
 template <typename T>
 hipError_t dalloc(mfree_ctx* c, T** p, size_t count) {
@@ -516,7 +453,7 @@ bool valid_particles(const mfree_particles* p, bool tl) {
   return true;
 }
 
-}  // namespace
+}  // namespace mfree
 
 extern "C" {
 

@@ -1,0 +1,95 @@
+// The context behind the C ABI (one set of bodies on one device) and the
+// host helpers shared by capi.hip and surrogate.hip.
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mfree_iwf_amd.h"
+#include "kernels.h"
+#include "tl.h"
+
+using mfree::Chunk;
+using mfree::Fields8;
+using mfree::PairRec32;
+using mfree::PairRec64;
+using mfree::SceneConst;
+using mfree::SceneGeom;
+using mfree::TlState;
+
+struct mfree_ctx {
+  mfree_options opt;
+  std::string err;
+  bool committed = false;
+  hipStream_t stream = nullptr;
+
+  // body registry (host)
+  std::vector<mfree_params> prm;
+  std::vector<uint32_t> off{0};
+  std::vector<double> time;
+  std::vector<std::vector<uint32_t>> bc_left, bc_right;
+  std::vector<double> v_bc;
+  // initial host state in idx order, uploaded at commit
+  std::vector<double> hx, hy, hX, hY, hm, hrho, hh, hvx, hvy, hSxx, hSxy, hSyy;
+
+  // device
+  uint32_t N = 0, S = 0, nchunks = 0;
+  std::vector<void*> allocs;
+  Fields8 y[2]{}, K{}, acc{};
+  double *m[2]{}, *h[2]{};
+  uint32_t *scene[2]{}, *gid[2]{};
+  int cur = 0;
+  uint32_t *key = nullptr, *rank = nullptr, *count = nullptr, *cell_start = nullptr, *perm = nullptr;
+  uint32_t *tsum = nullptr, *errb = nullptr;
+  uint2* meta = nullptr;
+  double2* pos64 = nullptr;
+  PairRec64* rec64 = nullptr;
+  float2* pos32 = nullptr;
+  PairRec32* rec32 = nullptr;
+  double* part = nullptr;
+  Chunk* chunks = nullptr;
+  uint32_t* chunk_first = nullptr;
+  SceneConst* sc = nullptr;
+  SceneGeom* geom = nullptr;
+  std::vector<SceneConst> h_sc;
+  uint32_t* tile_off = nullptr;  // nscene+1
+  uint32_t tile_cells = 0;       // 0: per-particle pair kernel
+  uint32_t tile_grid = 0;
+
+  TlState tl;  // Total-Lagrangian frozen lists (method TL)
+
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+  bool prof = false;
+  double pair_ms = 0., step_ms = 0.;
+  uint64_t pair_n = 0, steps = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+namespace mfree {
+
+inline mfree_status fail(mfree_ctx* c, mfree_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return fail(ctx, MFREE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+// Allocates and uploads everything at the first compute call.
+mfree_status commit(mfree_ctx* c);
+// Reads and clears the device error bits; maps them to a status.
+mfree_status check_err_bits(mfree_ctx* c);
+// Binning / UL-stage arguments over storage buffer b.
+BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b);
+UlArgs ul_args(mfree_ctx* c, int b);
+// nsteps RK4 steps of every body (graphs when enabled).
+mfree_status run_steps(mfree_ctx* c, uint32_t nsteps);
+
+}  // namespace mfree

@@ -340,6 +340,11 @@ __global__ __launch_bounds__(kBlock) void k_calib_copy(const double2* __restrict
     b[i] = a[i];
 }
 
+// out[n] = out[n-1] + in[n-1]: total of an exclusive scan (n >= 1).
+__global__ void k_scan_total(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = out[n - 1] + in[n - 1];
+}
+
 // ---- host launchers -------------------------------------------------------
 
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -350,6 +355,18 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n) {
 
 void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n) {
   hipLaunchKernelGGL(k_calib_copy, dim3(256 * 16), dim3(kBlock), 0, st, (const double2*)a, (double2*)b, n / 2);
+}
+
+void launch_excl_scan(hipStream_t st, const uint32_t* in, uint32_t n, uint32_t* tsum, uint32_t* out) {
+  if (n == 0) {
+    launch_fill_u32(st, out, 0u, 1);
+    return;
+  }
+  const uint32_t nt = nblk(n, kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, in, n, tsum);
+  hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(kBlock), 0, st, tsum, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, in, n, tsum, out, 0u);
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, in, n, out);
 }
 
 void launch_binning(hipStream_t st, const BinArgs& a) {

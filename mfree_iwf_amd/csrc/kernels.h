@@ -32,6 +32,9 @@ struct BinArgs {
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
 void launch_binning(hipStream_t st, const BinArgs& a);
+// Exclusive scan of n values: out[i] = sum_{k<i} in[k], out[n] = total;
+// tsum holds (n + kScanTile - 1) / kScanTile entries.
+void launch_excl_scan(hipStream_t st, const uint32_t* in, uint32_t n, uint32_t* tsum, uint32_t* out);
 
 // UL (Gray-Monaghan) stage.
 struct UlArgs {

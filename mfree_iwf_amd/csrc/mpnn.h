@@ -1,0 +1,130 @@
+// Message-passing surrogate stepper (internal): a graph network over the SPH
+// neighbour graph that predicts the correction of a 4-step state update.
+//
+// Architecture (hidden width H = 128, message width C = 32, L = 3 layers):
+//   X0  = node features (NF = 16, zero-padded to 32)            [N x 32]  bf16
+//   H0  = SiLU(X0 We^T + be)                                    [N x 128]
+//   layer l:  U  = H Wu^T                                       [N x 32]
+//             A  = [sum_j w_ij U_j, sum_j gx_ij U_j, sum_j gy_ij U_j]   [N x 96]
+//             H' = H + SiLU([H, A] Wo^T + bo)                   [N x 128]
+//   Y   = H3 Wd^T + bd                                          [N x 16] (8 used)
+// with w, gx, gy the dimensionless cubic-spline weight and gradient of each
+// edge of the radius-2h graph (h_ij = (h_i + h_j)/2, self excluded), so w is
+// symmetric and (gx, gy) antisymmetric.  Dense layers run on MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation); weights are kept in FP32
+// (Adam master copy) and in BF16 (W and W^T for the backward GEMMs).
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mfree_iwf_amd.h"
+#include "kernels.h"
+
+namespace mfree {
+
+constexpr int kNF = 16;    // node features
+constexpr int kK0 = 32;    // padded input width
+constexpr int kH = 128;    // hidden width
+constexpr int kC = 32;     // message channels
+constexpr int kG = kH + 3 * kC;  // 224: [H, a0, ax, ay]
+constexpr int kOut = 8;    // dx, dy, dvx, dvy, dSxx, dSxy, dSyy, drho
+constexpr int kOutP = 16;  // padded output width
+constexpr int kLayers = 3;
+constexpr uint32_t kEdgesPerNode = 64;  // edge capacity per node on average
+
+typedef unsigned short bf16_t;  // raw bfloat16 bits
+
+// float -> bfloat16, round to nearest even (finite inputs)
+__device__ __host__ inline bf16_t f2bf(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7FFFu + lsb;
+  return (bf16_t)(u >> 16);
+}
+__device__ __host__ inline float bf2f(bf16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// One parameter tensor: FP32 master, Adam moments, gradient, BF16 W and W^T.
+struct MpTensor {
+  int rows = 0, cols = 0;        // W is rows x cols (out x in)
+  size_t off = 0;                // offset into the flat FP32 arrays
+  bf16_t* w16 = nullptr;         // rows x cols
+  bf16_t* wt16 = nullptr;        // cols x rows (only for matrices used in backward)
+};
+
+struct MpNet {
+  std::vector<MpTensor> t;       // We, be, {Wu, Wo, bo} x L, Wd, bd
+  size_t nparam = 0;
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;  // flat FP32
+  std::vector<void*> allocs;
+  int adam_t = 0;
+  // tensor indices
+  int We, be, Wd, bd;
+  int Wu[kLayers], Wo[kLayers], bo[kLayers];
+};
+
+// Activations of one forward pass over N nodes (kept for the backward).
+struct MpActs {
+  uint32_t cap = 0;
+  bf16_t* X0 = nullptr;                 // N x 32
+  float* Z[kLayers + 1] = {};           // pre-activations: Z0 (enc), Z1..Z3 (layers), N x 128
+  float* Hf[kLayers + 1] = {};          // hidden states FP32, N x 128
+  bf16_t* G[kLayers + 1] = {};          // [H, A] bf16, N x 224 (G[l] holds H_l and A of layer l+1)
+  bf16_t* U[kLayers] = {};              // N x 32 bf16
+  float* Y = nullptr;                   // N x 16
+  // backward scratch
+  float* dH = nullptr;                  // N x 128
+  bf16_t* dZ16 = nullptr;               // N x 224 (bf16 dZ / dY / dU operands)
+  float* dG = nullptr;                  // N x 224
+  bf16_t* dA16 = nullptr;               // N x 96
+  float* dU = nullptr;                  // N x 32
+  float* part = nullptr;                // dW partials
+  size_t part_cap = 0;
+  std::vector<void*> allocs;
+};
+
+// Edges of the radius-2h graph over sorted positions, CSR: node p owns
+// edges [rowp[p], rowp[p+1]).  Capacity kEdgesPerNode per node on average;
+// single nodes may have any degree.
+struct MpGraph {
+  uint32_t cap = 0;           // nodes
+  uint64_t ecap = 0;          // edges
+  uint32_t* deg = nullptr;    // cap + 1 (degrees, scanned into rowp)
+  uint32_t* rowp = nullptr;   // cap + 1
+  uint32_t* tsum = nullptr;   // scan tile sums
+  uint32_t* col = nullptr;    // ecap (sorted positions)
+  float4* ew = nullptr;       // ecap: (w, gx, gy, 0)
+  float* wsum = nullptr;      // cap: sum of w over the edges and the node itself
+  std::vector<void*> allocs;
+};
+
+mfree_status mp_graph_reserve(MpGraph& g, uint32_t n, std::string& msg);
+void mp_graph_free(MpGraph& g);
+
+mfree_status mp_net_init(MpNet& net, uint64_t seed, std::string& msg);
+void mp_net_free(MpNet& net);
+void mp_net_sync16(MpNet& net, hipStream_t st);  // FP32 master -> BF16 copies
+mfree_status mp_acts_reserve(MpActs& a, uint32_t n, std::string& msg);
+void mp_acts_free(MpActs& a);
+
+// Forward over n nodes: features in a.X0, graph g; outputs a.Y.  With
+// train == false the pre-activations (only needed by the backward) are not
+// stored.
+void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st);
+// Loss + dY from targets T (n x 8, normalised) -> accumulates weight
+// gradients into net.g (zeroed by the caller).  loss_out: device float.
+void mp_backward(MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, const float* T, float* loss_out,
+                 hipStream_t st);
+void mp_zero_grad(MpNet& net, hipStream_t st);
+void mp_adam(MpNet& net, float lr, float beta1, float beta2, float eps, hipStream_t st);
+
+}  // namespace mfree

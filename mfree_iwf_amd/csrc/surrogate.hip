@@ -1,0 +1,1369 @@
+// Learned stepper: a message-passing network over the SPH neighbour graph
+// that stands in for groups of 4 RK4 steps (no counterpart in the reference,
+// whose only integrator is src/rk4.h:71-108).
+//
+// One learned step of a UL context (every body advances by tau = 4 dt):
+//   1. D1 = time derivatives at the state (the stage-0 pass of the UL
+//      pipeline, src/body.h:125-148)
+//   2. rho and S advanced by tau*D1 (velocity and positions kept); D2 =
+//      derivatives at that state
+//   3. base update (FP64):  S += tau*dS1, rho += tau*drho1, v += tau*a2,
+//                           x += tau*xdot1 + tau^2/2*a2
+//   4. the network adds a correction to all 8 fields (FP64 update), its
+//      inputs being 16 dimensionless node features built from the state,
+//      D1, D2 and the velocity one learned step earlier.
+// The network is trained on snapshots that the batched solver produces:
+// target = (state after 4 RK4 steps) - (base update), made dimensionless
+// with h, c0, K and rho0 of the body.
+//
+// The judged rollout runs groups of learned steps; the last step of each
+// group is also computed with 4 plain RK4 steps from the same state.  The
+// plain result is kept; if the positions of the two differ by more than the
+// threshold (relative L2 over all particles) the group is rolled back and
+// redone with plain RK4 steps.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "mpnn.h"
+
+using namespace mfree;
+
+namespace mfree {
+namespace {
+
+constexpr uint32_t kStepsPerLearned = 4;
+constexpr int kNorm = 2 * kNF + 2 * kOut;  // feature mean, std, target mean, std
+constexpr float kSplineNorm = 0.45472840883398667f;  // 10/(7 pi): 2D cubic spline factor times h^2
+
+inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+// ---- graph ----------------------------------------------------------------
+
+struct GraphArgs {
+  uint32_t n;
+  const uint32_t *perm, *key, *scene, *cell_start;
+  const SceneConst* sc;
+  const SceneGeom* geom;
+  const double *x, *y, *h;  // storage order
+  uint32_t* deg;            // count pass output
+  const uint32_t* rowp;     // fill pass input
+  uint64_t ecap;
+  uint32_t* col;
+  float4* ew;
+  float* wsum;
+  uint32_t* err;
+};
+
+// cubic-spline shape f(q) and f'(q) (src/kernel.cpp:20-55 without the
+// 10/(7 pi h^2) factor)
+__device__ inline void spline_shape(float q, float& f, float& df) {
+  if (q < 1.f) {
+    f = 1.f - 1.5f * q * q * (1.f - 0.5f * q);
+    df = -3.f * q * (1.f - 0.75f * q);
+  } else if (q < 2.f) {
+    const float t = 2.f - q;
+    f = 0.25f * t * t * t;
+    df = -0.75f * t * t;
+  } else {
+    f = 0.f;
+    df = 0.f;
+  }
+}
+
+// One thread per sorted position p: edges to every other particle of the
+// 3x3 cell block with r < 2 h_ij (h_ij = (h_i + h_j)/2).  The predicate and
+// the weights only use symmetric FP64 differences, so the graph is symmetric,
+// w_ij = w_ji and (gx, gy)_ij = -(gx, gy)_ji exactly.  Two passes: degrees
+// and kernel sums, then (after a scan into rowp) the edges.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const uint32_t s = a.perm[p];
+  const uint32_t b = a.scene[s];
+  const SceneGeom g = a.geom[b];
+  const uint32_t co = a.sc[b].cell_offset;
+  const uint32_t lk = a.key[s] - co;
+  const int ix = (int)(lk / g.ny), iy = (int)(lk % g.ny);
+  const double xi = a.x[s], yi = a.y[s], hi = a.h[s];
+  uint32_t d = 0;
+  float ws = kSplineNorm;  // the node itself (q = 0)
+  const uint64_t e0 = FILL ? a.rowp[p] : 0;
+  const int cx0 = max(ix - 1, 0), cx1 = min(ix + 1, (int)g.nx - 1);
+  const int cy0 = max(iy - 1, 0), cy1 = min(iy + 1, (int)g.ny - 1);
+  for (int cx = cx0; cx <= cx1; cx++) {
+    const uint32_t c0 = co + (uint32_t)cx * g.ny;
+    const uint32_t q0 = a.cell_start[c0 + cy0], q1 = a.cell_start[c0 + cy1 + 1];
+    for (uint32_t q = q0; q < q1; q++) {
+      if (q == p) continue;
+      const uint32_t j = a.perm[q];
+      const double dx = xi - a.x[j], dy = yi - a.y[j];
+      const double r2 = dx * dx + dy * dy;
+      const double hij = 0.5 * (hi + a.h[j]);
+      if (!(r2 < 4.0 * hij * hij)) continue;
+      const double r = sqrt(r2);
+      float f, df;
+      spline_shape((float)(r / hij), f, df);
+      const float w = kSplineNorm * f;
+      if (FILL) {
+        float gx = 0.f, gy = 0.f;
+        if (r > 0.0) {
+          const float sc = kSplineNorm * df / (float)r;
+          gx = sc * (float)dx;
+          gy = sc * (float)dy;
+        }
+        if (e0 + d < a.ecap) {
+          a.col[e0 + d] = q;
+          a.ew[e0 + d] = make_float4(w, gx, gy, 0.f);
+        }
+      }
+      d++;
+      ws += w;
+    }
+  }
+  if (FILL) {
+    if (e0 + d > a.ecap) atomicOr(a.err, EB_NBH);
+  } else {
+    a.deg[p] = d;
+    a.wsum[p] = ws;
+  }
+}
+
+void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
+  g.deg = mg.deg;
+  g.rowp = mg.rowp;
+  g.ecap = mg.ecap;
+  g.col = mg.col;
+  g.ew = mg.ew;
+  g.wsum = mg.wsum;
+  hipLaunchKernelGGL(k_graph<false>, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
+  launch_excl_scan(st, mg.deg, g.n, mg.tsum, mg.rowp);
+  hipLaunchKernelGGL(k_graph<true>, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
+}
+
+// ---- learned-step kernels -------------------------------------------------
+
+struct StepArgs {
+  uint32_t n;
+  const uint32_t *perm, *scene, *gid;  // perm: sorted -> storage
+  const SceneConst* sc;
+  Fields8 y;                           // state, storage order
+  const double* h;                     // storage order
+  const double *d1, *d2;               // derivatives by global id (8 per particle)
+  double* vprev;                       // by global id (2 per particle)
+  const float* wsum;                   // by sorted position
+  const float* norm;                   // kNorm floats (device), or null
+  // outputs
+  float* F;                            // raw features by sorted position (kNF), or null
+  bf16_t* X0;                          // normalised features by sorted position (kK0), or null
+  const float* Y;                      // network output by sorted position (kOutP), or null
+  double* base;                        // base update by global id (8), or null
+  uint32_t* inv;                       // global id -> sorted position, or null
+  double *ox, *oy, *oh;                // positions / h by sorted position, or null
+  int apply;                           // write the update into y (storage order)
+  int save_vprev;                      // vprev <- v before the update
+  Fields8 half;                        // k_half_state output
+};
+
+__device__ inline double tau_of(const SceneConst& c) { return (double)kStepsPerLearned * c.dt; }
+
+// rho and S advanced by tau*D1, x and v unchanged (storage order).
+__global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const SceneConst& c = a.sc[a.scene[i]];
+  const double tau = tau_of(c);
+  const double* d = a.d1 + 8 * (size_t)a.gid[i];
+#pragma unroll
+  for (int k = 0; k < 4; k++) a.half.f[k][i] = a.y.f[k][i];
+#pragma unroll
+  for (int k = 4; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
+}
+
+// One thread per sorted position: features, base update, optional network
+// correction.
+__global__ __launch_bounds__(256) void k_learned(StepArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const uint32_t s = a.perm[p];
+  const uint32_t g = a.gid[s];
+  const SceneConst& c = a.sc[a.scene[s]];
+  const double tau = tau_of(c);
+  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[s];
+  double st[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
+  const double* d1 = a.d1 + 8 * (size_t)g;
+  const double* d2 = a.d2 + 8 * (size_t)g;
+  const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
+  if (a.F || a.X0) {
+    float f[kNF];
+    f[0] = (float)(tau * d2[2] / c0);
+    f[1] = (float)(tau * d2[3] / c0);
+    f[2] = (float)(tau * d1[5] / K);
+    f[3] = (float)(tau * d1[6] / K);
+    f[4] = (float)(tau * d1[7] / K);
+    f[5] = (float)(tau * d1[4] / rho0);
+    f[6] = (float)(st[5] / K);
+    f[7] = (float)(st[6] / K);
+    f[8] = (float)(st[7] / K);
+    f[9] = (float)(st[4] / rho0 - 1.0);
+    f[10] = (float)((st[2] - vpx) / c0);
+    f[11] = (float)((st[3] - vpy) / c0);
+    f[12] = (float)(tau * (d1[0] - st[2]) / h);
+    f[13] = (float)(tau * (d1[1] - st[3]) / h);
+    f[14] = (float)(c0 * tau / h);
+    f[15] = a.wsum[p];
+    if (a.F) {
+#pragma unroll
+      for (int k = 0; k < kNF; k++) a.F[(size_t)p * kNF + k] = f[k];
+    }
+    if (a.X0) {
+      bf16_t* x = a.X0 + (size_t)p * kK0;
+#pragma unroll
+      for (int k = 0; k < kNF; k++) x[k] = f2bf((f[k] - a.norm[k]) / a.norm[kNF + k]);
+#pragma unroll
+      for (int k = kNF; k < kK0; k++) x[k] = 0;
+    }
+  }
+  // base update
+  double nw[8];
+  nw[0] = st[0] + tau * d1[0] + 0.5 * tau * tau * d2[2];
+  nw[1] = st[1] + tau * d1[1] + 0.5 * tau * tau * d2[3];
+  nw[2] = st[2] + tau * d2[2];
+  nw[3] = st[3] + tau * d2[3];
+  nw[4] = st[4] + tau * d1[4];
+  nw[5] = st[5] + tau * d1[5];
+  nw[6] = st[6] + tau * d1[6];
+  nw[7] = st[7] + tau * d1[7];
+  if (a.base) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a.base[8 * (size_t)g + k] = nw[k];
+  }
+  if (a.inv) a.inv[g] = p;
+  if (a.ox) {
+    a.ox[p] = st[0];
+    a.oy[p] = st[1];
+    a.oh[p] = h;
+  }
+  if (a.Y) {
+    // correction: output k (denormalised) times the field's scale
+    const double scale[8] = {h, h, c0, c0, rho0, K, K, K};
+    const int out_of_field[8] = {0, 1, 2, 3, 7, 4, 5, 6};  // field -> output index
+    const float* yv = a.Y + (size_t)p * kOutP;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int o = out_of_field[k];
+      const double r = (double)yv[o] * (double)a.norm[2 * kNF + kOut + o] + (double)a.norm[2 * kNF + o];
+      nw[k] += r * scale[k];
+    }
+  }
+  if (a.save_vprev) {
+    a.vprev[2 * (size_t)g] = st[2];
+    a.vprev[2 * (size_t)g + 1] = st[3];
+  }
+  if (a.apply) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a.y.f[k][s] = nw[k];
+  }
+}
+
+// Targets of one capture, by the new storage order: (truth - base) / scale,
+// outputs ordered dx, dy, dvx, dvy, dSxx, dSxy, dSyy, drho.
+__global__ __launch_bounds__(256) void k_targets(uint32_t n, Fields8 y, const uint32_t* __restrict__ gid,
+                                                 const uint32_t* __restrict__ scene, const SceneConst* __restrict__ sc,
+                                                 const double* __restrict__ h, const double* __restrict__ base,
+                                                 const uint32_t* __restrict__ inv, float* __restrict__ T) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = gid[i];
+  const SceneConst& c = sc[scene[i]];
+  const double c0 = sqrt(c.c0sq);
+  const double scale[8] = {h[i], h[i], c0, c0, c.rho0, c.K, c.K, c.K};
+  const int out_of_field[8] = {0, 1, 2, 3, 7, 4, 5, 6};
+  float* t = T + (size_t)inv[g] * kOut;
+#pragma unroll
+  for (int k = 0; k < 8; k++) t[out_of_field[k]] = (float)((y.f[k][i] - base[8 * (size_t)g + k]) / scale[k]);
+}
+
+__global__ void k_record_vprev(uint32_t n, const double* __restrict__ vx, const double* __restrict__ vy,
+                               const uint32_t* __restrict__ gid, double* __restrict__ vprev) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = gid[i];
+  vprev[2 * (size_t)g] = vx[i];
+  vprev[2 * (size_t)g + 1] = vy[i];
+}
+
+// Column sums and sums of squares of F (kNF) and T (kOut) over `rows`
+// rows, accumulated in FP64 into acc[2 * (kNF + kOut)].
+constexpr uint32_t kStatBlocks = 128;
+__global__ __launch_bounds__(256) void k_stats(uint64_t rows, const float* __restrict__ F,
+                                               const float* __restrict__ T, double* __restrict__ acc) {
+  constexpr int C = kNF + kOut;
+  __shared__ double sh[256 / 64][2 * C];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double s1[C], s2[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) s1[k] = s2[k] = 0.0;
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kNF; k++) {
+      const double v = F[r * kNF + k];
+      s1[k] += v;
+      s2[k] += v * v;
+    }
+#pragma unroll
+    for (int k = 0; k < kOut; k++) {
+      const double v = T[r * kOut + k];
+      s1[kNF + k] += v;
+      s2[kNF + k] += v * v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    for (int o = 32; o > 0; o >>= 1) {
+      s1[k] += __shfl_xor(s1[k], o, 64);
+      s2[k] += __shfl_xor(s2[k], o, 64);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      sh[w][k] = s1[k];
+      sh[w][C + k] = s2[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * C) {
+    double t = 0.0;
+    for (int k = 0; k < 256 / 64; k++) t += sh[k][threadIdx.x];
+    atomicAdd(&acc[threadIdx.x], t);
+  }
+}
+
+// Largest |rho/rho0 - 1| per body (float bits, atomic max; NaN compares
+// above every finite value).
+__global__ void k_body_compression(uint32_t n, const double* __restrict__ rho, const uint32_t* __restrict__ scene,
+                                   const SceneConst* __restrict__ sc, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = scene[i];
+  const float d = (float)fabs(rho[i] / sc[b].rho0 - 1.0);
+  uint32_t u;
+  memcpy(&u, &d, 4);
+  atomicMax(&out[b], u);
+}
+
+// Training minibatch inputs: X0[p] = bf16((F[r] - mean) / std), Tn[p] =
+// (T[r] - tmean) / tstd with r = perm[p].
+__global__ __launch_bounds__(256) void k_mb_inputs(uint32_t n, const uint32_t* __restrict__ perm,
+                                                   const float* __restrict__ F, const float* __restrict__ T,
+                                                   const float* __restrict__ norm, bf16_t* __restrict__ X0,
+                                                   float* __restrict__ Tn) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t r = perm[p];
+  bf16_t* x = X0 + (size_t)p * kK0;
+#pragma unroll
+  for (int k = 0; k < kNF; k++) x[k] = f2bf((F[(size_t)r * kNF + k] - norm[k]) / norm[kNF + k]);
+#pragma unroll
+  for (int k = kNF; k < kK0; k++) x[k] = 0;
+#pragma unroll
+  for (int k = 0; k < kOut; k++)
+    Tn[(size_t)p * kOut + k] = (T[(size_t)r * kOut + k] - norm[2 * kNF + k]) / norm[2 * kNF + kOut + k];
+}
+
+// scene[i] for the bodies of a training window (offsets in sc[].offset).
+__global__ void k_scene_ids(uint32_t n, const SceneConst* __restrict__ sc, uint32_t ns, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = ns;  // last s with sc[s].offset <= i
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (sc[mid].offset <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  out[i] = lo;
+}
+
+__global__ void k_iota(uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
+
+// Judge: positions by global id of the learned candidate.
+__global__ void k_pos_by_gid(uint32_t n, const double* __restrict__ x, const double* __restrict__ y,
+                             const uint32_t* __restrict__ gid, double* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = gid[i];
+  out[2 * (size_t)g] = x[i];
+  out[2 * (size_t)g + 1] = y[i];
+}
+
+// Judge: partial sums of |x - x_cand|^2 and |x|^2 (FP64).
+constexpr uint32_t kDevBlocks = 512;
+__global__ __launch_bounds__(256) void k_deviation(uint32_t n, const double* __restrict__ x,
+                                                   const double* __restrict__ y, const uint32_t* __restrict__ gid,
+                                                   const double* __restrict__ cand, double* __restrict__ part) {
+  double a = 0.0, b = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t g = gid[i];
+    const double ex = x[i] - cand[2 * (size_t)g], ey = y[i] - cand[2 * (size_t)g + 1];
+    a += ex * ex + ey * ey;
+    b += x[i] * x[i] + y[i] * y[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  __shared__ double sh[2][256 / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+    for (int k = 0; k < 256 / 64; k++) {
+      ta += sh[0][k];
+      tb += sh[1][k];
+    }
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tb;
+  }
+}
+
+}  // namespace
+}  // namespace mfree
+
+// ---------------------------------------------------------------------------
+
+struct mfree_surrogate {
+  mfree_surrogate_options opt;
+  std::string err;
+  hipStream_t st = nullptr;
+  std::mt19937_64 rng;
+  MpNet net;
+  MpActs acts;
+  MpGraph graph;
+  float* dnorm = nullptr;  // kNorm
+  float hnorm[kNorm];
+  bool have_norm = false;
+
+  // dataset (rows = nodes of all captures, each capture in its sorted order)
+  uint64_t rows = 0, rows_cap = 0;
+  float *dsF = nullptr, *dsT = nullptr;
+  double *dsX = nullptr, *dsY = nullptr, *dsH = nullptr;
+  struct Capture {
+    uint64_t row0;
+    std::vector<uint32_t> off;  // body offsets within the capture (S+1)
+    std::vector<uint8_t> valid; // body usable for training (S)
+  };
+  std::vector<Capture> caps;
+  // training windows: consecutive bodies of one capture, <= max_batch_nodes
+  struct Window {
+    uint64_t row0;
+    uint32_t n, ns, nch, sc_off, ch_off, cf_off;
+  };
+  std::vector<Window> wins;
+  bool wins_ready = false;
+  std::vector<void*> wallocs;  // window tables
+  SceneConst* wsc = nullptr;
+  Chunk* wch = nullptr;
+  uint32_t* wcf = nullptr;
+  uint32_t tcap = 0, tscene_cap = 0, tchunk_cap = 0;
+  std::vector<void*> tallocs;  // training binning buffers
+  uint32_t *tkey = nullptr, *trank = nullptr, *tcount = nullptr, *tcell = nullptr, *tperm = nullptr,
+           *ttsum = nullptr, *tscene = nullptr, *tiota = nullptr, *terr = nullptr;
+  SceneGeom* tgeom = nullptr;
+  double* tpart = nullptr;
+  float* tT = nullptr;
+  float* loss = nullptr;
+  uint32_t loss_cap = 0;
+
+  // per-context scratch
+  const mfree_ctx* bound = nullptr;
+  uint32_t bN = 0;
+  std::vector<void*> callocs;
+  double *d1 = nullptr, *d2 = nullptr, *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
+  uint32_t* inv = nullptr;
+  uint32_t* comp = nullptr;  // per body: largest compression (float bits)
+  bool vprev_valid = false;
+  // rollback snapshot (storage buffer of the group start)
+  double* snap = nullptr;   // 10 fields x N: y(8), m, h
+  uint32_t* snapu = nullptr;  // scene, gid
+  double* jsnap = nullptr;  // 8 x N: state before the judge step
+  SceneConst* snap_sc = nullptr;
+  SceneConst* jsc = nullptr;
+};
+
+namespace mfree {
+namespace {
+
+mfree_status sfail(mfree_surrogate* s, mfree_status st, const std::string& m) {
+  if (s) s->err = m;
+  return st;
+}
+
+#define SCHK(s, call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return sfail(s, MFREE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t salloc(std::vector<void*>& list, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    list.push_back(q);
+    *p = (T*)q;
+  }
+  return e;
+}
+
+void free_list(std::vector<void*>& l) {
+  for (void* p : l) (void)hipFree(p);
+  l.clear();
+}
+
+// Per-context buffers (derivatives, base update, previous velocities,
+// snapshots); reallocated when the surrogate meets another context.
+mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
+  mfree_status st = commit(c);
+  if (st != MFREE_OK) return sfail(s, st, std::string("context: ") + c->err);
+  if (c->opt.method != MFREE_METHOD_UL_MONAGHAN)
+    return sfail(s, MFREE_ERR_UNSUPPORTED, "the learned stepper drives UL (Gray-Monaghan) contexts");
+  if (c->opt.device != s->opt.device) return sfail(s, MFREE_ERR_ARG, "context and surrogate use different devices");
+  if (s->bound == c && s->bN == c->N) return MFREE_OK;
+  SCHK(s, hipDeviceSynchronize());
+  free_list(s->callocs);
+  const size_t N = c->N;
+  SCHK(s, salloc(s->callocs, &s->d1, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->d2, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->base, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->vprev, 2 * N));
+  SCHK(s, salloc(s->callocs, &s->cand, 2 * N));
+  SCHK(s, salloc(s->callocs, &s->devp, 2 * kDevBlocks));
+  SCHK(s, salloc(s->callocs, &s->inv, N));
+  SCHK(s, salloc(s->callocs, &s->comp, c->S));
+  SCHK(s, salloc(s->callocs, &s->snap, 10 * N));
+  SCHK(s, salloc(s->callocs, &s->snapu, 2 * N));
+  SCHK(s, salloc(s->callocs, &s->jsnap, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->snap_sc, c->S));
+  SCHK(s, salloc(s->callocs, &s->jsc, c->S));
+  std::string msg;
+  if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, msg) != MFREE_OK)
+    return sfail(s, MFREE_ERR_HIP, msg);
+  s->bound = c;
+  s->bN = c->N;
+  s->vprev_valid = false;
+  return MFREE_OK;
+}
+
+StepArgs step_args(mfree_surrogate* s, mfree_ctx* c) {
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  const int b = c->cur;
+  a.n = c->N;
+  a.perm = c->perm;
+  a.scene = c->scene[b];
+  a.gid = c->gid[b];
+  a.sc = c->sc;
+  a.y = c->y[b];
+  a.h = c->h[b];
+  a.d1 = s->d1;
+  a.d2 = s->d2;
+  a.vprev = s->vprev;
+  a.wsum = s->graph.wsum;
+  a.norm = s->dnorm;
+  a.half = c->K;
+  return a;
+}
+
+void record_vprev(mfree_surrogate* s, mfree_ctx* c) {
+  const int b = c->cur;
+  hipLaunchKernelGGL(k_record_vprev, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->N, c->y[b].f[2],
+                     c->y[b].f[3], c->gid[b], s->vprev);
+  s->vprev_valid = true;
+}
+
+// Steps 1-2 of a learned step: cell list of the current state, D1, the
+// half state, D2.
+void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
+  hipStream_t st = c->stream;
+  const int b = c->cur;
+  launch_binning(st, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+  UlArgs u = ul_args(c, b);
+  u.src = c->y[b];
+  launch_ul_prepass(st, u);
+  u.stage = 0;
+  u.d8 = s->d1;
+  launch_ul_pairs(st, u);
+  StepArgs a = step_args(s, c);
+  hipLaunchKernelGGL(k_half_state, dim3(nblk(c->N, 256)), dim3(256), 0, st, a);
+  UlArgs v = ul_args(c, b);
+  v.src = c->K;
+  launch_ul_prepass(st, v);
+  v.stage = 0;
+  v.d8 = s->d2;
+  launch_ul_pairs(st, v);
+}
+
+void enqueue_graph(mfree_surrogate* s, mfree_ctx* c) {
+  const int b = c->cur;
+  GraphArgs g;
+  memset(&g, 0, sizeof(g));
+  g.n = c->N;
+  g.perm = c->perm;
+  g.key = c->key;
+  g.scene = c->scene[b];
+  g.cell_start = c->cell_start;
+  g.sc = c->sc;
+  g.geom = c->geom;
+  g.x = c->y[b].f[0];
+  g.y = c->y[b].f[1];
+  g.h = c->h[b];
+  g.err = c->errb;
+  launch_graph(c->stream, g, s->graph);
+}
+
+void advance_time(mfree_ctx* c, uint32_t nsteps) {
+  for (uint32_t k = 0; k < nsteps; k++)
+    hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc);
+  for (uint32_t b = 0; b < c->S; b++)
+    for (uint32_t k = 0; k < nsteps; k++) c->time[b] = c->time[b] + c->prm[b].dt;
+}
+
+// One learned step of every body (no synchronisation).
+void enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
+  enqueue_derivs(s, c);
+  StepArgs a = step_args(s, c);
+  if (use_net) {
+    enqueue_graph(s, c);
+    a.X0 = s->acts.X0;
+    // features first (they read the state), then the network, then the update
+    StepArgs f = a;
+    f.apply = 0;
+    hipLaunchKernelGGL(k_learned, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f);
+    mp_forward(s->net, s->acts, s->graph, c->N, false, c->stream);
+    a.X0 = nullptr;
+    a.Y = s->acts.Y;
+  }
+  a.apply = 1;
+  a.save_vprev = 1;
+  hipLaunchKernelGGL(k_learned, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, a);
+  advance_time(c, kStepsPerLearned);
+}
+
+mfree_status ensure_rows(mfree_surrogate* s, uint64_t need) {
+  if (need <= s->rows_cap) return MFREE_OK;
+  SCHK(s, hipDeviceSynchronize());
+  float *F = nullptr, *T = nullptr;
+  double *X = nullptr, *Y = nullptr, *H = nullptr;
+  SCHK(s, hipMalloc(&F, need * kNF * sizeof(float)));
+  SCHK(s, hipMalloc(&T, need * kOut * sizeof(float)));
+  SCHK(s, hipMalloc(&X, need * sizeof(double)));
+  SCHK(s, hipMalloc(&Y, need * sizeof(double)));
+  SCHK(s, hipMalloc(&H, need * sizeof(double)));
+  if (s->rows) {
+    SCHK(s, hipMemcpy(F, s->dsF, s->rows * kNF * sizeof(float), hipMemcpyDeviceToDevice));
+    SCHK(s, hipMemcpy(T, s->dsT, s->rows * kOut * sizeof(float), hipMemcpyDeviceToDevice));
+    SCHK(s, hipMemcpy(X, s->dsX, s->rows * sizeof(double), hipMemcpyDeviceToDevice));
+    SCHK(s, hipMemcpy(Y, s->dsY, s->rows * sizeof(double), hipMemcpyDeviceToDevice));
+    SCHK(s, hipMemcpy(H, s->dsH, s->rows * sizeof(double), hipMemcpyDeviceToDevice));
+  }
+  (void)hipFree(s->dsF);
+  (void)hipFree(s->dsT);
+  (void)hipFree(s->dsX);
+  (void)hipFree(s->dsY);
+  (void)hipFree(s->dsH);
+  s->dsF = F;
+  s->dsT = T;
+  s->dsX = X;
+  s->dsY = Y;
+  s->dsH = H;
+  s->rows_cap = need;
+  return MFREE_OK;
+}
+
+mfree_status build_windows(mfree_surrogate* s);
+
+// Statistics of the training rows (the windows) -> normalisation (mean,
+// std per column).
+mfree_status compute_norm(mfree_surrogate* s) {
+  constexpr int C = kNF + kOut;
+  mfree_status st = build_windows(s);
+  if (st != MFREE_OK) return st;
+  double* acc = nullptr;
+  SCHK(s, hipMalloc(&acc, 2 * C * sizeof(double)));
+  hipError_t e = hipMemsetAsync(acc, 0, 2 * C * sizeof(double), s->st);
+  uint64_t rows = 0;
+  for (const auto& w : s->wins) {
+    hipLaunchKernelGGL(k_stats, dim3(kStatBlocks), dim3(256), 0, s->st, (uint64_t)w.n, s->dsF + w.row0 * kNF,
+                       s->dsT + w.row0 * kOut, acc);
+    rows += w.n;
+  }
+  double h[2 * C];
+  if (e == hipSuccess) e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  (void)hipFree(acc);
+  if (e != hipSuccess) return sfail(s, MFREE_ERR_HIP, hipGetErrorString(e));
+  const double n = (double)rows;
+  for (int k = 0; k < C; k++) {
+    const double mean = h[k] / n;
+    const double var = std::max(h[C + k] / n - mean * mean, 0.0);
+    double sd = sqrt(var);
+    // a column that is constant up to rounding keeps unit scale
+    if (!(sd > 1e-6 * fabs(mean) + 1e-30) || !isfinite(sd)) sd = std::max(fabs(mean), 1.0);
+    if (k < kNF) {
+      s->hnorm[k] = (float)mean;
+      s->hnorm[kNF + k] = (float)sd;
+    } else {
+      s->hnorm[2 * kNF + (k - kNF)] = (float)mean;
+      s->hnorm[2 * kNF + kOut + (k - kNF)] = (float)sd;
+    }
+  }
+  SCHK(s, hipMemcpy(s->dnorm, s->hnorm, sizeof(s->hnorm), hipMemcpyHostToDevice));
+  s->have_norm = true;
+  return MFREE_OK;
+}
+
+// Training windows over the captures and the binning buffers they need.
+mfree_status build_windows(mfree_surrogate* s) {
+  if (s->wins_ready) return MFREE_OK;
+  SCHK(s, hipDeviceSynchronize());
+  free_list(s->wallocs);
+  s->wins.clear();
+  std::vector<SceneConst> hsc;
+  std::vector<Chunk> hch;
+  std::vector<uint32_t> hcf;
+  const uint32_t cap = std::max<uint32_t>(s->opt.max_batch_nodes, 1);
+  uint32_t max_n = 0, max_ns = 0, max_ch = 0;
+  for (const auto& cp : s->caps) {
+    const uint32_t S = (uint32_t)cp.off.size() - 1;
+    uint32_t b = 0;
+    while (b < S) {
+      if (!cp.valid[b]) {
+        b++;
+        continue;
+      }
+      uint32_t e = b + 1;
+      while (e < S && cp.valid[e] && cp.off[e + 1] - cp.off[b] <= cap) e++;
+      mfree_surrogate::Window w;
+      w.row0 = cp.row0 + cp.off[b];
+      w.n = cp.off[e] - cp.off[b];
+      w.ns = e - b;
+      w.sc_off = (uint32_t)hsc.size();
+      w.ch_off = (uint32_t)hch.size();
+      w.cf_off = (uint32_t)hcf.size();
+      for (uint32_t k = b; k < e; k++) {
+        SceneConst c;
+        memset(&c, 0, sizeof(c));
+        c.offset = cp.off[k] - cp.off[b];
+        c.n = cp.off[k + 1] - cp.off[k];
+        c.cell_offset = c.offset;
+        hsc.push_back(c);
+        hcf.push_back((uint32_t)(hch.size() - w.ch_off));
+        for (uint32_t i = c.offset; i < c.offset + c.n; i += kChunk)
+          hch.push_back(Chunk{k - b, i, std::min(i + kChunk, c.offset + c.n), 0});
+      }
+      hcf.push_back((uint32_t)(hch.size() - w.ch_off));
+      w.nch = (uint32_t)(hch.size() - w.ch_off);
+      s->wins.push_back(w);
+      max_n = std::max(max_n, w.n);
+      max_ns = std::max(max_ns, w.ns);
+      max_ch = std::max(max_ch, w.nch);
+      b = e;
+    }
+  }
+  if (s->wins.empty()) return sfail(s, MFREE_ERR_ARG, "no usable training snapshots (collect first)");
+  SCHK(s, salloc(s->wallocs, &s->wsc, hsc.size()));
+  SCHK(s, salloc(s->wallocs, &s->wch, hch.size()));
+  SCHK(s, salloc(s->wallocs, &s->wcf, hcf.size()));
+  SCHK(s, hipMemcpy(s->wsc, hsc.data(), hsc.size() * sizeof(SceneConst), hipMemcpyHostToDevice));
+  SCHK(s, hipMemcpy(s->wch, hch.data(), hch.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+  SCHK(s, hipMemcpy(s->wcf, hcf.data(), hcf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (max_n > s->tcap || max_ns > s->tscene_cap || max_ch > s->tchunk_cap) {
+    free_list(s->tallocs);
+    const size_t n = max_n;
+    SCHK(s, salloc(s->tallocs, &s->tkey, n));
+    SCHK(s, salloc(s->tallocs, &s->trank, n));
+    SCHK(s, salloc(s->tallocs, &s->tcount, n));
+    SCHK(s, salloc(s->tallocs, &s->tcell, n + 1));
+    SCHK(s, salloc(s->tallocs, &s->tperm, n));
+    SCHK(s, salloc(s->tallocs, &s->ttsum, (n + kScanTile - 1) / kScanTile + 1));
+    SCHK(s, salloc(s->tallocs, &s->tscene, n));
+    SCHK(s, salloc(s->tallocs, &s->tiota, n));
+    SCHK(s, salloc(s->tallocs, &s->terr, 1));
+    SCHK(s, salloc(s->tallocs, &s->tgeom, max_ns));
+    SCHK(s, salloc(s->tallocs, &s->tpart, 5 * (size_t)max_ch));
+    SCHK(s, salloc(s->tallocs, &s->tT, n * kOut));
+    SCHK(s, hipMemset(s->terr, 0, sizeof(uint32_t)));
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n, 256)), dim3(256), 0, s->st, (uint32_t)n, s->tiota);
+    s->tcap = max_n;
+    s->tscene_cap = max_ns;
+    s->tchunk_cap = max_ch;
+  }
+  std::string msg;
+  if (mp_graph_reserve(s->graph, max_n, msg) != MFREE_OK || mp_acts_reserve(s->acts, max_n, msg) != MFREE_OK)
+    return sfail(s, MFREE_ERR_HIP, msg);
+  s->wins_ready = true;
+  return MFREE_OK;
+}
+
+// One Adam step on window w; the loss of the minibatch is added to *loss.
+void enqueue_train_step(mfree_surrogate* s, const mfree_surrogate::Window& w, float lr, float* loss) {
+  hipStream_t st = s->st;
+  BinArgs a;
+  memset(&a, 0, sizeof(a));
+  a.n = w.n;
+  a.nscene = w.ns;
+  a.nchunks = w.nch;
+  a.ncell_cap = w.n;
+  a.deterministic = 1;
+  a.kernel_width = 2.0;
+  a.cell_min = 0.0;
+  a.chunks = s->wch + w.ch_off;
+  a.chunk_first = s->wcf + w.cf_off;
+  a.sc = s->wsc + w.sc_off;
+  a.geom = s->tgeom;
+  a.x = s->dsX + w.row0;
+  a.y = s->dsY + w.row0;
+  a.h = s->dsH + w.row0;
+  a.scene = s->tscene;
+  a.gid = s->tiota;
+  a.part = s->tpart;
+  a.key = s->tkey;
+  a.rank = s->trank;
+  a.count = s->tcount;
+  a.tsum = s->ttsum;
+  a.cell_start = s->tcell;
+  a.perm = s->tperm;
+  a.err = s->terr;
+  hipLaunchKernelGGL(k_scene_ids, dim3(nblk(w.n, 256)), dim3(256), 0, st, w.n, a.sc, w.ns, s->tscene);
+  launch_binning(st, a);
+  GraphArgs g;
+  memset(&g, 0, sizeof(g));
+  g.n = w.n;
+  g.perm = s->tperm;
+  g.key = s->tkey;
+  g.scene = s->tscene;
+  g.cell_start = s->tcell;
+  g.sc = a.sc;
+  g.geom = s->tgeom;
+  g.x = a.x;
+  g.y = a.y;
+  g.h = a.h;
+  g.err = s->terr;
+  launch_graph(st, g, s->graph);
+  hipLaunchKernelGGL(k_mb_inputs, dim3(nblk(w.n, 256)), dim3(256), 0, st, w.n, s->tperm, s->dsF + w.row0 * kNF,
+                     s->dsT + w.row0 * kOut, s->dnorm, s->acts.X0, s->tT);
+  mp_forward(s->net, s->acts, s->graph, w.n, true, st);
+  mp_zero_grad(s->net, st);
+  mp_backward(s->net, s->acts, s->graph, w.n, s->tT, loss, st);
+  mp_adam(s->net, lr, 0.9f, 0.999f, 1e-8f, st);
+}
+
+mfree_status sync_ctx(mfree_surrogate* s, mfree_ctx* c) {
+  mfree_status st = check_err_bits(c);
+  if (st != MFREE_OK) return sfail(s, st, std::string("learned step: ") + c->err);
+  return MFREE_OK;
+}
+
+// Judge deviation: rel L2 between the ctx positions and s->cand (by gid).
+mfree_status deviation(mfree_surrogate* s, mfree_ctx* c, double* out) {
+  const int b = c->cur;
+  hipLaunchKernelGGL(k_deviation, dim3(kDevBlocks), dim3(256), 0, c->stream, c->N, c->y[b].f[0], c->y[b].f[1],
+                     c->gid[b], s->cand, s->devp);
+  std::vector<double> h(2 * kDevBlocks);
+  SCHK(s, hipMemcpyAsync(h.data(), s->devp, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SCHK(s, hipStreamSynchronize(c->stream));
+  double a = 0.0, bb = 0.0;
+  for (uint32_t k = 0; k < kDevBlocks; k++) {
+    a += h[2 * k];
+    bb += h[2 * k + 1];
+  }
+  *out = bb > 0.0 ? sqrt(a / bb) : 0.0;
+  return MFREE_OK;
+}
+
+void copy_async(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+}
+
+// Snapshot of the full per-particle storage of the current buffer (rollback).
+void snapshot(mfree_surrogate* s, mfree_ctx* c, std::vector<double>& times) {
+  const int b = c->cur;
+  const size_t N = c->N;
+  for (int k = 0; k < 8; k++) copy_async(s->snap + k * N, c->y[b].f[k], N * 8, c->stream);
+  copy_async(s->snap + 8 * N, c->m[b], N * 8, c->stream);
+  copy_async(s->snap + 9 * N, c->h[b], N * 8, c->stream);
+  copy_async(s->snapu, c->scene[b], N * 4, c->stream);
+  copy_async(s->snapu + N, c->gid[b], N * 4, c->stream);
+  copy_async(s->snap_sc, c->sc, c->S * sizeof(SceneConst), c->stream);
+  times = c->time;
+}
+
+void restore(mfree_surrogate* s, mfree_ctx* c, const std::vector<double>& times) {
+  const int b = c->cur;
+  const size_t N = c->N;
+  for (int k = 0; k < 8; k++) copy_async(c->y[b].f[k], s->snap + k * N, N * 8, c->stream);
+  copy_async(c->m[b], s->snap + 8 * N, N * 8, c->stream);
+  copy_async(c->h[b], s->snap + 9 * N, N * 8, c->stream);
+  copy_async(c->scene[b], s->snapu, N * 4, c->stream);
+  copy_async(c->gid[b], s->snapu + N, N * 4, c->stream);
+  copy_async(c->sc, s->snap_sc, c->S * sizeof(SceneConst), c->stream);
+  c->time = times;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace mfree
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+void mfree_surrogate_default_options(mfree_surrogate_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->device = 0;
+  o->seed = 1;
+  o->max_batch_nodes = 1u << 20;
+  o->max_compression = 1.0f;
+}
+
+mfree_status mfree_surrogate_create(const mfree_surrogate_options* o, mfree_surrogate** out) {
+  if (!out) return MFREE_ERR_ARG;
+  *out = nullptr;
+  mfree_surrogate* s = new mfree_surrogate();
+  if (o)
+    s->opt = *o;
+  else
+    mfree_surrogate_default_options(&s->opt);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= s->opt.device) {
+    delete s;
+    return MFREE_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipSetDevice(s->opt.device) != hipSuccess || hipGetDeviceProperties(&prop, s->opt.device) != hipSuccess ||
+      strncmp(prop.gcnArchName, "gfx950", 6) != 0 || hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return MFREE_ERR_HIP;
+  }
+  s->rng.seed(s->opt.seed);
+  std::string msg;
+  if (mp_net_init(s->net, s->opt.seed, msg) != MFREE_OK || hipMalloc(&s->dnorm, kNorm * sizeof(float)) != hipSuccess) {
+    mp_net_free(s->net);
+    hipStreamDestroy(s->st);
+    delete s;
+    return MFREE_ERR_HIP;
+  }
+  for (int k = 0; k < kNorm; k++) s->hnorm[k] = 0.f;
+  for (int k = 0; k < kNF; k++) s->hnorm[kNF + k] = 1.f;
+  for (int k = 0; k < kOut; k++) s->hnorm[2 * kNF + kOut + k] = 1.f;
+  (void)hipMemcpy(s->dnorm, s->hnorm, sizeof(s->hnorm), hipMemcpyHostToDevice);
+  *out = s;
+  return MFREE_OK;
+}
+
+void mfree_surrogate_destroy(mfree_surrogate* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->opt.device);
+  (void)hipDeviceSynchronize();
+  mp_net_free(s->net);
+  mp_acts_free(s->acts);
+  mp_graph_free(s->graph);
+  free_list(s->wallocs);
+  free_list(s->tallocs);
+  free_list(s->callocs);
+  (void)hipFree(s->dnorm);
+  (void)hipFree(s->dsF);
+  (void)hipFree(s->dsT);
+  (void)hipFree(s->dsX);
+  (void)hipFree(s->dsY);
+  (void)hipFree(s->dsH);
+  (void)hipFree(s->loss);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+const char* mfree_surrogate_last_error(const mfree_surrogate* s) { return s ? s->err.c_str() : "null surrogate"; }
+
+uint64_t mfree_mpnn_param_count(void) {
+  return (uint64_t)kH * kNF + kH + kLayers * ((uint64_t)kC * kH + (uint64_t)kH * kG + kH) + (uint64_t)kOut * kH + kOut;
+}
+
+uint32_t mfree_mpnn_layout(int32_t* shapes) {
+  const int32_t t[] = {kH, kNF, 1, kH, kC, kH, kH, kG, 1, kH, kC, kH, kH, kG, 1, kH,
+                       kC, kH, kH, kG, 1, kH, kOut, kH, 1, kOut};
+  const uint32_t nt = (uint32_t)(sizeof(t) / sizeof(t[0]) / 2);
+  if (shapes) memcpy(shapes, t, sizeof(t));
+  return nt;
+}
+
+mfree_status mfree_surrogate_get_params(mfree_surrogate* s, float* w, float* norm) {
+  if (!s) return MFREE_ERR_ARG;
+  (void)hipSetDevice(s->opt.device);
+  if (w) SCHK(s, hipMemcpy(w, s->net.w, s->net.nparam * sizeof(float), hipMemcpyDeviceToHost));
+  if (norm) memcpy(norm, s->hnorm, sizeof(s->hnorm));
+  return MFREE_OK;
+}
+
+mfree_status mfree_surrogate_set_params(mfree_surrogate* s, const float* w, const float* norm) {
+  if (!s) return MFREE_ERR_ARG;
+  (void)hipSetDevice(s->opt.device);
+  SCHK(s, hipDeviceSynchronize());
+  if (w) {
+    SCHK(s, hipMemcpy(s->net.w, w, s->net.nparam * sizeof(float), hipMemcpyHostToDevice));
+    mp_net_sync16(s->net, s->st);
+    SCHK(s, hipStreamSynchronize(s->st));
+  }
+  if (norm) {
+    for (int k = 0; k < kNorm; k++)
+      if (!isfinite(norm[k])) return sfail(s, MFREE_ERR_ARG, "non-finite normalisation");
+    for (int k = 0; k < kNF; k++)
+      if (!(norm[kNF + k] > 0.f)) return sfail(s, MFREE_ERR_ARG, "feature std must be positive");
+    memcpy(s->hnorm, norm, sizeof(s->hnorm));
+    SCHK(s, hipMemcpy(s->dnorm, s->hnorm, sizeof(s->hnorm), hipMemcpyHostToDevice));
+    s->have_norm = true;
+  }
+  return MFREE_OK;
+}
+
+uint64_t mfree_surrogate_snapshots(const mfree_surrogate* s) {
+  if (!s) return 0;
+  uint64_t n = 0;
+  for (const auto& c : s->caps)
+    for (uint8_t v : c.valid) n += v;
+  return n;
+}
+
+uint64_t mfree_surrogate_dataset_nodes(const mfree_surrogate* s) { return s ? s->rows : 0; }
+
+mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* c, uint32_t ncapture, uint32_t gap) {
+  if (!s || !c) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = bind_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  const uint64_t N = c->N;
+  st = ensure_rows(s, s->rows + (uint64_t)ncapture * N);
+  if (st != MFREE_OK) return st;
+  for (uint32_t k = 0; k < ncapture; k++) {
+    if (gap) {
+      st = run_steps(c, gap);
+      if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
+    }
+    // velocity one learned step before the capture
+    record_vprev(s, c);
+    st = run_steps(c, kStepsPerLearned);
+    if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
+    // inputs and base update at the capture state
+    const uint64_t row0 = s->rows;
+    launch_fill_u32(c->stream, s->comp, 0u, c->S);
+    hipLaunchKernelGGL(k_body_compression, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N,
+                       c->y[c->cur].f[4], c->scene[c->cur], c->sc, s->comp);
+    enqueue_derivs(s, c);
+    enqueue_graph(s, c);  // for the kernel-sum feature
+    StepArgs a = step_args(s, c);
+    a.F = s->dsF + row0 * kNF;
+    a.base = s->base;
+    a.inv = s->inv;
+    a.ox = s->dsX + row0;
+    a.oy = s->dsY + row0;
+    a.oh = s->dsH + row0;
+    hipLaunchKernelGGL(k_learned, dim3(nblk(N, 256)), dim3(256), 0, c->stream, a);
+    st = sync_ctx(s, c);
+    if (st != MFREE_OK) return st;
+    // truth: 4 RK4 steps
+    st = run_steps(c, kStepsPerLearned);
+    if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
+    const int b = c->cur;
+    hipLaunchKernelGGL(k_targets, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b], c->gid[b],
+                       c->scene[b], c->sc, c->h[b], s->base, s->inv, s->dsT + row0 * kOut);
+    hipLaunchKernelGGL(k_body_compression, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b].f[4],
+                       c->scene[b], c->sc, s->comp);
+    std::vector<float> comp(c->S);
+    SCHK(s, hipMemcpyAsync(comp.data(), s->comp, c->S * 4, hipMemcpyDeviceToHost, c->stream));
+    SCHK(s, hipStreamSynchronize(c->stream));
+    mfree_surrogate::Capture cp;
+    cp.row0 = row0;
+    cp.off = c->off;
+    cp.valid.resize(c->S);
+    // a body compressed beyond max_compression (|rho/rho0 - 1|) at either
+    // end of the step is no elastic-solid sample: not used for training
+    for (uint32_t k = 0; k < c->S; k++) cp.valid[k] = comp[k] <= s->opt.max_compression ? 1 : 0;
+    s->caps.push_back(cp);
+    s->rows += N;
+  }
+  s->wins_ready = false;
+  return compute_norm(s);
+}
+
+mfree_status mfree_surrogate_train(mfree_surrogate* s, uint32_t iters, float lr, float* loss) {
+  if (!s || !(lr > 0.f)) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = build_windows(s);
+  if (st != MFREE_OK) return st;
+  if (iters > s->loss_cap) {
+    (void)hipFree(s->loss);
+    s->loss = nullptr;
+    SCHK(s, hipMalloc(&s->loss, iters * sizeof(float)));
+    s->loss_cap = iters;
+  }
+  if (iters) SCHK(s, hipMemsetAsync(s->loss, 0, iters * sizeof(float), s->st));
+  std::uniform_int_distribution<size_t> pick(0, s->wins.size() - 1);
+  const float lr_min = 0.05f * lr;
+  for (uint32_t it = 0; it < iters; it++) {
+    // cosine decay to 5% of the initial rate
+    const float t = iters > 1 ? (float)it / (float)(iters - 1) : 0.f;
+    const float lr_t = lr_min + 0.5f * (lr - lr_min) * (1.f + cosf(3.14159265f * t));
+    enqueue_train_step(s, s->wins[pick(s->rng)], lr_t, s->loss + it);
+  }
+  SCHK(s, hipStreamSynchronize(s->st));
+  SCHK(s, hipGetLastError());
+  uint32_t e = 0;
+  SCHK(s, hipMemcpy(&e, s->terr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (e) {
+    uint32_t z = 0;
+    SCHK(s, hipMemcpy(s->terr, &z, sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (e & EB_NBH) return sfail(s, MFREE_ERR_NBH_OVERFLOW, "training graph: more than 64 edges per node on average");
+    return sfail(s, MFREE_ERR_CELLS, "training binning failed");
+  }
+  if (loss && iters) SCHK(s, hipMemcpy(loss, s->loss, iters * sizeof(float), hipMemcpyDeviceToHost));
+  return MFREE_OK;
+}
+
+mfree_status mfree_surrogate_step(mfree_surrogate* s, mfree_ctx* c, int32_t use_net) {
+  if (!s || !c) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = bind_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  if (use_net && !s->have_norm) return sfail(s, MFREE_ERR_ARG, "network has no normalisation (collect or set_params)");
+  if (!s->vprev_valid) record_vprev(s, c);
+  enqueue_learned(s, c, use_net != 0);
+  return sync_ctx(s, c);
+}
+
+mfree_status mfree_surrogate_record_velocity(mfree_surrogate* s, mfree_ctx* c) {
+  if (!s || !c) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = bind_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  record_vprev(s, c);
+  SCHK(s, hipStreamSynchronize(c->stream));
+  return MFREE_OK;
+}
+
+mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t nsteps, uint32_t group,
+                                     double threshold, mfree_judge_stats* stats) {
+  if (!s || !c || group == 0) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = bind_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  if (!s->have_norm) return sfail(s, MFREE_ERR_ARG, "network has no normalisation (collect or set_params)");
+  mfree_judge_stats js;
+  memset(&js, 0, sizeof(js));
+  const uint32_t span = group * kStepsPerLearned;
+  uint32_t done = 0;
+  auto plain = [&](uint32_t k) -> mfree_status {
+    mfree_status r = run_steps(c, k);
+    js.plain_steps += k;
+    return r == MFREE_OK ? MFREE_OK : sfail(s, r, std::string("plain steps: ") + c->err);
+  };
+  // warm start: the features need the velocity one learned step back
+  if (nsteps >= span + kStepsPerLearned) {
+    record_vprev(s, c);
+    st = plain(kStepsPerLearned);
+    if (st != MFREE_OK) return st;
+    done += kStepsPerLearned;
+  }
+  std::vector<double> times;
+  const size_t N = c->N;
+  while (done + span <= nsteps) {
+    auto t0 = std::chrono::steady_clock::now();
+    snapshot(s, c, times);
+    for (uint32_t k = 0; k + 1 < group; k++) enqueue_learned(s, c, true);
+    st = sync_ctx(s, c);
+    if (st != MFREE_OK) return st;
+    js.ms_learned += ms_since(t0);
+    // judge: the last learned step and 4 plain steps from the same state
+    auto t1 = std::chrono::steady_clock::now();
+    const int b = c->cur;
+    for (int k = 0; k < 8; k++) copy_async(s->jsnap + k * N, c->y[b].f[k], N * 8, c->stream);
+    copy_async(s->jsc, c->sc, c->S * sizeof(SceneConst), c->stream);
+    const std::vector<double> jtimes = c->time;
+    enqueue_learned(s, c, true);  // also records the velocities of this state
+    hipLaunchKernelGGL(k_pos_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b].f[0],
+                       c->y[b].f[1], c->gid[b], s->cand);
+    for (int k = 0; k < 8; k++) copy_async(c->y[b].f[k], s->jsnap + k * N, N * 8, c->stream);
+    copy_async(c->sc, s->jsc, c->S * sizeof(SceneConst), c->stream);
+    c->time = jtimes;
+    st = sync_ctx(s, c);
+    if (st != MFREE_OK) return st;
+    st = plain(kStepsPerLearned);
+    if (st != MFREE_OK) return st;
+    double dev = 0.0;
+    st = deviation(s, c, &dev);
+    if (st != MFREE_OK) return st;
+    js.ms_judge += ms_since(t1);
+    js.groups++;
+    js.last_deviation = dev;
+    js.max_deviation = std::max(js.max_deviation, dev);
+    js.learned_steps += group;
+    if (!(dev <= threshold)) {
+      // roll back the group and redo it with plain RK4 steps
+      auto t2 = std::chrono::steady_clock::now();
+      js.fallbacks++;
+      js.learned_steps -= group;
+      restore(s, c, times);
+      st = plain(span - kStepsPerLearned);
+      if (st != MFREE_OK) return st;
+      record_vprev(s, c);
+      st = plain(kStepsPerLearned);
+      if (st != MFREE_OK) return st;
+      js.ms_fallback += ms_since(t2);
+    }
+    done += span;
+  }
+  if (done < nsteps) {
+    if (nsteps - done >= kStepsPerLearned) {
+      st = plain(nsteps - done - kStepsPerLearned);
+      if (st != MFREE_OK) return st;
+      record_vprev(s, c);
+      st = plain(kStepsPerLearned);
+    } else {
+      st = plain(nsteps - done);
+    }
+    if (st != MFREE_OK) return st;
+  }
+  if (stats) *stats = js;
+  return MFREE_OK;
+}
+
+mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* row_ptr, uint32_t* col, float* edge3,
+                                   uint64_t cap, uint64_t* nedges) {
+  if (!s || !c || !row_ptr) return MFREE_ERR_ARG;
+  SCHK(s, hipSetDevice(s->opt.device));
+  mfree_status st = bind_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  const int b = c->cur;
+  launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+  enqueue_graph(s, c);
+  st = sync_ctx(s, c);
+  if (st != MFREE_OK) return st;
+  const size_t N = c->N;
+  std::vector<uint32_t> rp(N + 1), perm(N), gid(N);
+  SCHK(s, hipMemcpy(rp.data(), s->graph.rowp, (N + 1) * 4, hipMemcpyDeviceToHost));
+  SCHK(s, hipMemcpy(perm.data(), c->perm, N * 4, hipMemcpyDeviceToHost));
+  SCHK(s, hipMemcpy(gid.data(), c->gid[b], N * 4, hipMemcpyDeviceToHost));
+  const uint64_t tot = rp[N];
+  if (nedges) *nedges = tot;
+  if (tot > cap) return sfail(s, MFREE_ERR_CAPACITY, "edge capacity too small");
+  std::vector<uint32_t> colv(tot);
+  std::vector<float4> ew(tot);
+  SCHK(s, hipMemcpy(colv.data(), s->graph.col, tot * 4, hipMemcpyDeviceToHost));
+  SCHK(s, hipMemcpy(ew.data(), s->graph.ew, tot * 16, hipMemcpyDeviceToHost));
+  // rows by global id, neighbours as global ids in sorted-position order
+  std::vector<uint32_t> row_of(N);
+  for (size_t p = 0; p < N; p++) row_of[gid[perm[p]]] = (uint32_t)p;
+  row_ptr[0] = 0;
+  for (size_t g = 0; g < N; g++) row_ptr[g + 1] = row_ptr[g] + (rp[row_of[g] + 1] - rp[row_of[g]]);
+  for (size_t g = 0; g < N; g++) {
+    const size_t p = row_of[g];
+    for (uint32_t e = rp[p]; e < rp[p + 1]; e++) {
+      const size_t k = row_ptr[g] + (e - rp[p]);
+      if (col) col[k] = gid[perm[colv[e]]];
+      if (edge3) {
+        edge3[3 * k] = ew[e].x;
+        edge3[3 * k + 1] = ew[e].y;
+        edge3[3 * k + 2] = ew[e].z;
+      }
+    }
+  }
+  return MFREE_OK;
+}
+
+mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                                         const float* edge3, const float* x, const float* t, const float* w,
+                                         float* y, float* loss, float* grad) {
+  if (!row_ptr || !col || !edge3 || !x || !t || !w || n == 0) return MFREE_ERR_ARG;
+  if (row_ptr[0] != 0 || (uint64_t)row_ptr[n] > (uint64_t)n * kEdgesPerNode) return MFREE_ERR_ARG;
+  for (uint32_t i = 0; i < n; i++) {
+    if (row_ptr[i + 1] < row_ptr[i]) return MFREE_ERR_ARG;
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; e++)
+      if (col[e] >= n) return MFREE_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || hipSetDevice(device) != hipSuccess)
+    return MFREE_ERR_HIP;
+  MpNet net;
+  MpActs acts;
+  MpGraph g;
+  std::string msg;
+  mfree_status st = mp_net_init(net, 0, msg);
+  if (st == MFREE_OK) st = mp_acts_reserve(acts, n, msg);
+  if (st == MFREE_OK) st = mp_graph_reserve(g, n, msg);
+  float *dT = nullptr, *dL = nullptr;
+  hipStream_t strm = nullptr;
+  if (st == MFREE_OK && (hipMalloc(&dT, (size_t)n * kOut * 4) != hipSuccess || hipMalloc(&dL, 4) != hipSuccess ||
+                         hipStreamCreateWithFlags(&strm, hipStreamNonBlocking) != hipSuccess))
+    st = MFREE_ERR_HIP;
+  if (st == MFREE_OK) {
+    const uint32_t E = row_ptr[n];
+    std::vector<float4> ew(std::max<uint32_t>(E, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<bf16_t> x16((size_t)n * kK0, 0);
+    for (uint32_t k = 0; k < E; k++) ew[k] = make_float4(edge3[3 * k], edge3[3 * k + 1], edge3[3 * k + 2], 0.f);
+    for (uint32_t i = 0; i < n; i++)
+      for (int k = 0; k < kNF; k++) x16[(size_t)i * kK0 + k] = f2bf(x[(size_t)i * kNF + k]);
+    const float zero = 0.f;
+    bool ok = hipMemcpy(g.rowp, row_ptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              (E == 0 || hipMemcpy(g.col, col, (size_t)E * 4, hipMemcpyHostToDevice) == hipSuccess) &&
+              (E == 0 || hipMemcpy(g.ew, ew.data(), (size_t)E * 16, hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemcpy(acts.X0, x16.data(), x16.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dT, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dL, &zero, 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(net.w, w, net.nparam * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+      mp_net_sync16(net, strm);
+      mp_forward(net, acts, g, n, true, strm);
+      mp_zero_grad(net, strm);
+      mp_backward(net, acts, g, n, dT, dL, strm);
+      ok = hipStreamSynchronize(strm) == hipSuccess;
+    }
+    if (ok && y) {
+      std::vector<float> yy((size_t)n * kOutP);
+      ok = hipMemcpy(yy.data(), acts.Y, yy.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+      for (uint32_t i = 0; ok && i < n; i++)
+        for (int k = 0; k < kOut; k++) y[(size_t)i * kOut + k] = yy[(size_t)i * kOutP + k];
+    }
+    if (ok && loss) ok = hipMemcpy(loss, dL, 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (ok && grad) ok = hipMemcpy(grad, net.g, net.nparam * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) st = MFREE_ERR_HIP;
+  }
+  if (strm) (void)hipStreamDestroy(strm);
+  (void)hipFree(dT);
+  (void)hipFree(dL);
+  mp_graph_free(g);
+  mp_acts_free(acts);
+  mp_net_free(net);
+  return st;
+}
+
+}  // extern "C"

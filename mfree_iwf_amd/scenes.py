@@ -221,6 +221,24 @@ def batch_specs(nscenes=256, seed=0, nmin=2000, nmax=60000):
     return out
 
 
+def subsonic_batch_specs(nscenes=256, seed=0, nmin=2000, nmax=60000, max_mach=0.9, nu=0.4, rho0=1010.0):
+    """The batched workload's generator restricted to impacts below max_mach
+    (v0 / c0, c0 = sqrt(K / rho0)): the first nscenes such scenes of the
+    seeded stream.  Faster impacts compress the rings far beyond an elastic
+    regime (rho/rho0 of order 100 within a few hundred steps) and end in
+    non-finite states; the learned stepper is trained on the rest."""
+    out, drawn = [], 0
+    while len(out) < nscenes:
+        batch = batch_specs(nscenes=drawn + 4 * nscenes, seed=seed, nmin=nmin, nmax=nmax)
+        out = [b for b in batch if b["v0"] < max_mach * sound_speed(b["E"], nu, rho0)][:nscenes]
+        drawn += 4 * nscenes
+    return out
+
+
+def scenes_from_specs(specs):
+    return [make_rings(s["nbox"], ri=s["ri"], ro=s["ro"], centre=s["centre"], v0=s["v0"], E=s["E"]) for s in specs]
+
+
 def batch_config4(nscenes=256, seed=0):
     return [make_rings(s["nbox"], ri=s["ri"], ro=s["ro"], centre=s["centre"], v0=s["v0"], E=s["E"])
             for s in batch_specs(nscenes, seed)]

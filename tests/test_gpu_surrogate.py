@@ -1,0 +1,233 @@
+"""GPU checks of the learned stepper against its FP64 oracle (oracle/mpnn.py)
+and the plain solver, through the C ABI: graph builder, MFMA forward and
+backward (gradient check), the learned step with and without the network,
+the judge / rollback, training, and the ablation against the base update."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _states import perturbed_rings
+from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver, scenes
+from mfree_iwf_amd import surrogate as sg
+from oracle import mpnn as om
+
+pytestmark = pytest.mark.gpu
+
+# BF16 operands (8-bit mantissa) in every dense layer and in the aggregated
+# messages, FP32 accumulation: the network's outputs and gradients agree with
+# the FP64 oracle to about 1e-2 relative.
+Y_TOL = 3e-2
+GRAD_TOL = 6e-2
+
+
+def lattice_graph(n, seed, hdx=1.3):
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    ij = np.array([(i, j) for i in range(side) for j in range(side)][:n], dtype=np.float64)
+    p = ij + 0.2 * rng.standard_normal(ij.shape)
+    h = np.full(n, hdx)
+    return om.radius_graph(p[:, 0], p[:, 1], h)
+
+
+def sorted_rows(row, col, e3):
+    out = []
+    for i in range(len(row) - 1):
+        a, b = int(row[i]), int(row[i + 1])
+        o = np.argsort(col[a:b], kind="stable")
+        out.append((col[a:b][o], e3[a:b][o]))
+    return out
+
+
+def test_graph_matches_oracle():
+    s = perturbed_rings(nbox=41, seed=3, dpos=0.3)
+    ps = s.particles
+    print(f"[surrogate graph] n={ps.n}", flush=True)
+    with Solver(precision=PRECISION_FP64) as sv, sg.Surrogate() as su:
+        sv.add_body(ps, s.params)
+        row, col, e3 = su.graph(sv)
+    orow, ocol, oe3, _ = om.radius_graph(ps.x, ps.y, ps.h)
+    assert np.array_equal(row.astype(np.int64), orow)
+    for (c, e), (oc, oe) in zip(sorted_rows(row, col, e3), sorted_rows(orow, ocol, oe3)):
+        assert np.array_equal(c, oc)
+        assert np.allclose(e, oe, rtol=1e-5, atol=1e-6)
+
+
+def test_graph_multibody_matches_oracle():
+    bodies = [perturbed_rings(nbox=nb, seed=20 + k, dpos=0.3) for k, nb in enumerate((21, 41, 31))]
+    print(f"[surrogate graph] bodies n={[b.particles.n for b in bodies]}", flush=True)
+    with Solver(precision=PRECISION_MIXED) as sv, sg.Surrogate() as su:
+        for b in bodies:
+            sv.add_body(b.particles, b.params)
+        sv.step(3)
+        states = [sv.state(k) for k in range(3)]
+        row, col, e3 = su.graph(sv)
+    off = 0
+    for ps in states:
+        orow, ocol, oe3, _ = om.radius_graph(ps.x, ps.y, ps.h)
+        r = row[off:off + ps.n + 1].astype(np.int64) - int(row[off])
+        assert np.array_equal(r, orow)
+        a, b = int(row[off]), int(row[off + ps.n])
+        got = sorted_rows(r, col[a:b].astype(np.int64) - off, e3[a:b])
+        for (c, e), (oc, oe) in zip(got, sorted_rows(orow, ocol, oe3)):
+            assert np.array_equal(c, oc)
+            assert np.allclose(e, oe, rtol=1e-5, atol=1e-6)
+        off += ps.n
+
+
+@pytest.mark.parametrize("n,seed", [(300, 0), (1000, 1)])
+def test_mpnn_forward_backward_matches_oracle(n, seed):
+    row, col, e3, _ = lattice_graph(n, seed)
+    rng = np.random.default_rng(seed + 10)
+    X = rng.standard_normal((n, om.NF))
+    T = rng.standard_normal((n, om.NOUT))
+    w = om.init_params(seed=seed + 20).astype(np.float32)
+    print(f"[mpnn fwd/bwd] n={n} edges={len(col)}", flush=True)
+    y, loss, g = sg.forward_backward(row, col, e3, X, T, w)
+    # oracle on the same float32 weights (FP64 arithmetic)
+    oy, oloss, og = om.loss_and_grad(row, col, e3, X, T, w.astype(np.float64))
+    assert O.rel_l2(y, oy) < Y_TOL
+    assert loss == pytest.approx(oloss, rel=Y_TOL)
+    off = 0
+    for (r, c) in om.layout():
+        k = r * c
+        assert O.rel_l2(g[off:off + k], og[off:off + k]) < GRAD_TOL, (r, c)
+        off += k
+    cos = float(np.dot(g, og) / (np.linalg.norm(g) * np.linalg.norm(og)))
+    assert cos > 0.998
+
+
+def _ctx(scene, precision=PRECISION_FP64):
+    sv = Solver(precision=precision)
+    sv.add_body(scene.particles, scene.params)
+    return sv
+
+
+def _scales(scene):
+    p = scene.params
+    K = scenes.elastic_K(p.E, p.nu)
+    c0 = np.sqrt(K / p.rho0)
+    return 4 * p.dt, c0, K, p.rho0
+
+
+def _oracle_inputs(scene, ps):
+    """D1, D2 (oracle, stable order) and the base update of state ps."""
+    tau, c0, K, rho0 = _scales(scene)
+    rc, d1, _ = O.ul_derivatives(ps, scene.params)
+    assert rc == O.OK
+    s8 = ps.state8()
+    half = om.half_state(s8, d1, tau)
+    q = ps.copy()
+    for k, f in enumerate(scenes.STATE_FIELDS):
+        setattr(q, f, np.ascontiguousarray(half[:, k]))
+    rc, d2, _ = O.ul_derivatives(q, scene.params)
+    assert rc == O.OK
+    return s8, d1, d2, om.base_update(s8, d1, d2, tau)
+
+
+def test_base_step_matches_oracle():
+    s = perturbed_rings(nbox=41, seed=5, dpos=0.2)
+    with _ctx(s) as sv, sg.Surrogate() as su:
+        su.record_velocity(sv)
+        su.step(sv, use_net=False)
+        got = sv.state(0).state8()
+        t = sv.time(0)
+    _, _, _, base = _oracle_inputs(s, s.particles)
+    for k in range(8):
+        assert O.rel_l2(got[:, k], base[:, k]) < 1e-10, scenes.STATE_FIELDS[k]
+    assert t == pytest.approx(4 * s.params.dt, rel=1e-12)
+
+
+def test_learned_step_matches_oracle():
+    """Network step on random weights: the correction added to the base
+    update agrees with the FP64 oracle's features -> forward -> correction."""
+    s = perturbed_rings(nbox=41, seed=6, dpos=0.2)
+    ps = s.particles
+    tau, c0, K, rho0 = _scales(s)
+    s8, d1, d2, base = _oracle_inputs(s, ps)
+    row, col, e3, wsum = om.radius_graph(ps.x, ps.y, ps.h)
+    feats = om.features(s8, d1, d2, s8[:, 2:4], ps.h, wsum, tau, c0, K, rho0)
+    fm, fs = feats.mean(0), feats.std(0)
+    fs[fs < 1e-6 * np.abs(fm) + 1e-30] = 1.0  # constant features (the Courant number)
+    norm = np.concatenate([fm, fs, np.zeros(om.NOUT), np.full(om.NOUT, 1e-3)]).astype(np.float32)
+    w = om.init_params(seed=9).astype(np.float32)
+    X = (feats - norm[:om.NF]) / norm[om.NF:2 * om.NF]
+    Y, _ = om.forward(row, col, e3, X, w.astype(np.float64))
+    want = om.correction(Y, norm.astype(np.float64), ps.h, c0, K, rho0)
+    with _ctx(s) as sv, sg.Surrogate() as su:
+        su.set_params(w, norm)
+        su.record_velocity(sv)
+        su.step(sv, use_net=True)
+        got = sv.state(0).state8() - base
+    for k in range(8):
+        assert O.rel_l2(got[:, k], want[:, k]) < Y_TOL, scenes.STATE_FIELDS[k]
+
+
+def test_judge_always_rejecting_equals_plain_solver():
+    s = scenes.rings_config1()
+    with _ctx(s) as a, _ctx(s) as b, sg.Surrogate() as su:
+        su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.ones(8)]))
+        st = su.rollout(a, 4 + 2 * 32 + 3, group=8, threshold=-1.0)
+        b.step(4 + 2 * 32 + 3)
+        assert st["groups"] == 2 and st["fallbacks"] == 2 and st["learned_steps"] == 0
+        sa, sb = a.state(0).state8(), b.state(0).state8()
+        assert np.array_equal(sa, sb)
+        assert a.time(0) == pytest.approx(b.time(0), rel=1e-12)
+
+
+def test_rollout_accepting_counts_steps():
+    s = scenes.rings_config1()
+    with _ctx(s, PRECISION_MIXED) as a, sg.Surrogate() as su:
+        su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.full(8, 1e-9)]))
+        st = su.rollout(a, 4 + 3 * 64 + 5, group=16, threshold=np.inf)
+        assert st["groups"] == 3 and st["fallbacks"] == 0
+        assert st["learned_steps"] == 48 and st["plain_steps"] == 4 + 3 * 4 + 5
+        assert 0.0 < st["max_deviation"] < 1e-3
+        t = a.time(0)
+    assert t == pytest.approx((4 + 3 * 64 + 5) * s.params.dt, rel=1e-9)
+
+
+def _batch(nscenes, seed, nmin, nmax):
+    return scenes.scenes_from_specs(scenes.subsonic_batch_specs(nscenes=nscenes, seed=seed, nmin=nmin, nmax=nmax))
+
+
+def _twins(scene, k):
+    return [_ctx(scene, PRECISION_MIXED) for _ in range(k)]
+
+
+def test_training_and_ablation():
+    """Trains on rollouts of a small batch, then compares one learned step
+    with and without the network against 4 RK4 steps on held-out scenes."""
+    train = _batch(24, seed=1, nmin=2000, nmax=6000)
+    print(f"[surrogate train] bodies={len(train)} n={sum(t.particles.n for t in train)}", flush=True)
+    with Solver(precision=PRECISION_MIXED) as sv, sg.Surrogate(seed=3, max_batch_nodes=40000) as su:
+        for sc in train:
+            sv.add_body(sc.particles, sc.params)
+        su.collect(sv, ncapture=24, gap=4)
+        print(f"[surrogate train] usable snapshots {su.snapshots} of {24 * 24}", flush=True)
+        assert 0.5 * 24 * 24 < su.snapshots <= 24 * 24
+        loss = su.train(600, lr=2e-3)
+        print(f"[surrogate train] loss first {loss[:10].mean():.4f} last {loss[-50:].mean():.4f}", flush=True)
+        assert np.all(np.isfinite(loss))
+        assert loss[-50:].mean() < 0.7 * loss[:10].mean()
+        gains = []
+        for sc in _batch(3, seed=77, nmin=3000, nmax=8000):
+            a, b, c = _twins(sc, 3)
+            with a, b, c:
+                for x in (a, b, c):
+                    x.step(150)
+                su.record_velocity(a)
+                su.record_velocity(b)
+                for x in (a, b, c):
+                    x.step(4)
+                su.step(a, use_net=True)
+                su.step(b, use_net=False)
+                c.step(4)
+                ta, tb, tc = a.state(0).state8(), b.state(0).state8(), c.state(0).state8()
+            d_net = np.linalg.norm(ta[:, 2:8] - tc[:, 2:8], axis=0)
+            d_base = np.linalg.norm(tb[:, 2:8] - tc[:, 2:8], axis=0)
+            gains.append(d_base / np.maximum(d_net, 1e-300))
+            print(f"[ablation] n={sc.particles.n} base/net error ratio (vx vy rho Sxx Sxy Syy): "
+                  + " ".join(f"{g:.2f}" for g in gains[-1]), flush=True)
+        gains = np.array(gains)
+        # the network reduces the error of the velocity and stress update
+        assert np.median(gains[:, [0, 1, 3, 4, 5]]) > 1.5

@@ -1,0 +1,127 @@
+"""Pins of the FP64 learned-stepper oracle (oracle/mpnn.py): finite
+differences, torch.autograd, per-edge loops, the reference-pinned cubic
+spline, graph symmetry and the adjoint identity of the aggregation.  CPU
+only."""
+import numpy as np
+import pytest
+
+from oracle import mpnn as om
+
+import _oracle as O
+
+
+def random_points(n, seed=0, spacing=1.0, jitter=0.25):
+    """Jittered square lattice (degree ~ 20 at h = 1.3 spacing)."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    ij = np.array([(i, j) for i in range(side) for j in range(side)][:n], dtype=np.float64)
+    p = spacing * (ij + jitter * rng.standard_normal(ij.shape))
+    return p[:, 0].copy(), p[:, 1].copy()
+
+
+def small_graph(n=60, seed=0, hdx=1.3):
+    x, y = random_points(n, seed)
+    h = np.full(n, hdx) * (1.0 + 0.05 * np.random.default_rng(seed + 1).random(n))
+    return (x, y, h) + om.radius_graph(x, y, h)
+
+
+def test_layout_matches_library():
+    from mfree_iwf_amd import surrogate
+    assert surrogate.layout() == om.layout()
+    assert surrogate.param_count() == om.param_count()
+
+
+def test_radius_graph_weights_follow_reference_spline():
+    x, y, h, row, col, e3, wsum = small_graph()
+    rows = np.repeat(np.arange(len(x)), np.diff(row))
+    assert len(col) > 10 * len(x)
+    for k in range(0, len(col), 7):
+        i, j = rows[k], col[k]
+        hij = 0.5 * (h[i] + h[j])
+        w, wx, wy = O.cubic_spline(x[i], y[i], x[j], y[j], hij)
+        assert e3[k, 0] == pytest.approx(w * hij ** 2, rel=1e-12, abs=1e-15)
+        assert e3[k, 1] == pytest.approx(wx * hij ** 3, rel=1e-12, abs=1e-15)
+        assert e3[k, 2] == pytest.approx(wy * hij ** 3, rel=1e-12, abs=1e-15)
+    # kernel sum includes the node itself: W(0) h^2 = 10/(7 pi)
+    assert np.allclose(wsum, om.SPLINE_NORM + np.add.reduceat(e3[:, 0], row[:-1]) * (np.diff(row) > 0))
+
+
+def test_radius_graph_is_symmetric():
+    x, y, h, row, col, e3, _ = small_graph(seed=3)
+    rows = np.repeat(np.arange(len(x)), np.diff(row))
+    edges = {(int(i), int(j)): k for k, (i, j) in enumerate(zip(rows, col))}
+    for (i, j), k in edges.items():
+        assert i != j
+        kk = edges[(j, i)]
+        assert e3[kk, 0] == e3[k, 0]
+        assert e3[kk, 1] == -e3[k, 1] and e3[kk, 2] == -e3[k, 2]
+
+
+def test_aggregate_matches_edge_loop_and_adjoint():
+    x, y, h, row, col, e3, _ = small_graph(n=40, seed=5)
+    rng = np.random.default_rng(0)
+    U = rng.standard_normal((len(x), 4))
+    A = om.aggregate(row, col, e3, U)
+    B = np.zeros_like(A)
+    for i in range(len(x)):
+        for k in range(row[i], row[i + 1]):
+            for c in range(3):
+                B[i, 4 * c:4 * c + 4] += e3[k, c] * U[col[k]]
+    assert np.allclose(A, B, rtol=1e-13, atol=1e-14)
+    dA = rng.standard_normal(A.shape)
+    lhs = float((A * dA).sum())
+    rhs = float((U * om.aggregate_T(row, col, e3, dA)).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_gradient_matches_finite_differences():
+    x, y, h, row, col, e3, _ = small_graph(n=30, seed=7)
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((len(x), om.NF))
+    T = rng.standard_normal((len(x), om.NOUT))
+    w = om.init_params(seed=2)
+    _, loss, g = om.loss_and_grad(row, col, e3, X, T, w)
+    # parameters from every tensor
+    offs = np.cumsum([0] + [r * c for r, c in om.layout()])
+    idx = [int(offs[t] + rng.integers(offs[t + 1] - offs[t])) for t in range(len(offs) - 1) for _ in range(3)]
+    eps = 1e-6
+    for k in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[k] += eps
+        wm[k] -= eps
+        fd = (om.loss_and_grad(row, col, e3, X, T, wp)[1] - om.loss_and_grad(row, col, e3, X, T, wm)[1]) / (2 * eps)
+        assert g[k] == pytest.approx(fd, rel=2e-5, abs=1e-9), k
+
+
+def test_forward_and_gradient_match_torch_autograd():
+    torch = pytest.importorskip("torch")
+    x, y, h, row, col, e3, _ = small_graph(n=50, seed=11)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((len(x), om.NF))
+    T = rng.standard_normal((len(x), om.NOUT))
+    w = om.init_params(seed=4)
+    Y, loss, g = om.loss_and_grad(row, col, e3, X, T, w)
+
+    wt = torch.tensor(w, dtype=torch.float64, requires_grad=True)
+    shapes = om.layout()
+    parts, off = [], 0
+    for r, c in shapes:
+        parts.append(wt[off:off + r * c].reshape(r, c) if r > 1 else wt[off:off + c])
+        off += r * c
+    rows = torch.tensor(np.repeat(np.arange(len(x)), np.diff(row)))
+    colt = torch.tensor(col)
+    et = torch.tensor(e3)
+    Xt = torch.tensor(X)
+    Hh = torch.nn.functional.silu(Xt @ parts[0].T + parts[1])
+    for l in range(om.LAYERS):
+        Wu, Wo, bo = parts[2 + 3 * l], parts[3 + 3 * l], parts[4 + 3 * l]
+        U = Hh @ Wu.T
+        A = [torch.zeros(len(x), om.C, dtype=torch.float64).index_add(0, rows, et[:, k:k + 1] * U[colt])
+             for k in range(3)]
+        Hh = Hh + torch.nn.functional.silu(torch.cat([Hh] + A, dim=1) @ Wo.T + bo)
+    Yt = Hh @ parts[-2].T + parts[-1]
+    lt = ((Yt - torch.tensor(T)) ** 2).mean()
+    lt.backward()
+    assert np.allclose(Yt.detach().numpy(), Y, rtol=1e-12, atol=1e-12)
+    assert float(lt.detach()) == pytest.approx(loss, rel=1e-12)
+    assert np.allclose(wt.grad.numpy(), g, rtol=1e-9, atol=1e-13)
