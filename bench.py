@@ -77,8 +77,19 @@ def run_mode(scene, precision, steps, warmup, log):
     )
 
 
+def errors(a, b):
+    """rel-L2 of state a against the FP64 reference state b."""
+    return dict(
+        rel_l2_pos=rel_l2(np.concatenate([a.x, a.y]), np.concatenate([b.x, b.y])),
+        rel_l2_vel=rel_l2(np.concatenate([a.vx, a.vy]), np.concatenate([b.vx, b.vy])),
+        rel_l2_stress=rel_l2(np.concatenate([a.Sxx, a.Sxy, a.Syy]), np.concatenate([b.Sxx, b.Sxy, b.Syy])),
+        rel_l2_rho=rel_l2(a.rho, b.rho),
+    )
+
+
 def accuracy(scene, nsteps, log):
-    """Mixed vs FP64 GPU after nsteps RK4 steps from the same state."""
+    """Mixed vs FP64 GPU after nsteps RK4 steps from the same state.
+    Returns (summary, FP64 end state)."""
     from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver
     out = {}
     for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
@@ -89,12 +100,55 @@ def accuracy(scene, nsteps, log):
             out[name] = sv.state(0)
             log(f"  accuracy run {name}: {nsteps} steps in {time.perf_counter() - t0:.2f} s")
     a, b = out["mixed"], out["fp64"]
-    return dict(
-        steps=nsteps,
-        rel_l2_pos=rel_l2(np.concatenate([a.x, a.y]), np.concatenate([b.x, b.y])),
-        rel_l2_stress=rel_l2(np.concatenate([a.Sxx, a.Sxy, a.Syy]), np.concatenate([b.Sxx, b.Sxy, b.Syy])),
-        max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()),
-    )
+    res = dict(steps=nsteps, **errors(a, b),
+               max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()))
+    return res, b
+
+
+def bench_surrogate(scene, args, ref_state, log):
+    """Learned stepper: training data from the batched workload's generator
+    (subsonic scenes, seed 0), training on the GPU, then the judged rollout
+    of the full workload over the accuracy horizon, timed end to end."""
+    from mfree_iwf_amd import PRECISION_MIXED, Solver, Surrogate, scenes
+    t0 = time.perf_counter()
+    train = scenes.scenes_from_specs(scenes.subsonic_batch_specs(args.sur_scenes, seed=0))
+    ntrain = sum(s.particles.n for s in train)
+    log(f"surrogate: {len(train)} training bodies, {ntrain} particles (setup {time.perf_counter() - t0:.1f} s)")
+    res = {}
+    with Surrogate(seed=0, max_batch_nodes=args.sur_batch) as su:
+        with Solver(precision=PRECISION_MIXED) as gen:
+            for s in train:
+                gen.add_body(s.particles, s.params)
+            t0 = time.perf_counter()
+            su.collect(gen, ncapture=args.sur_captures, gap=args.sur_gap)
+            res["collect_s"] = time.perf_counter() - t0
+        res["snapshots"] = su.snapshots
+        res["dataset_nodes"] = su.dataset_nodes
+        log(f"  collected {su.snapshots} usable snapshots ({su.dataset_nodes} nodes) in {res['collect_s']:.1f} s")
+        t0 = time.perf_counter()
+        loss = su.train(args.sur_iters, lr=args.sur_lr)
+        res["train_s"] = time.perf_counter() - t0
+        res["train_iters"] = args.sur_iters
+        res["loss_first"] = float(np.mean(loss[:20]))
+        res["loss_last"] = float(np.mean(loss[-100:]))
+        log(f"  trained {args.sur_iters} iterations in {res['train_s']:.1f} s, loss {res['loss_first']:.4f} -> "
+            f"{res['loss_last']:.4f}")
+        n = scene.particles.n
+        nsteps = args.accuracy_steps
+        with Solver(precision=PRECISION_MIXED) as sv:
+            sv.add_body(scene.particles, scene.params)
+            sv.step(0)  # commit (autotuner) outside the timed region
+            t0 = time.perf_counter()
+            st = su.rollout(sv, nsteps, group=args.sur_group, threshold=args.sur_threshold)
+            wall = time.perf_counter() - t0
+            state = sv.state(0)
+        res.update(particle_steps_per_s=n * nsteps / wall, rollout_s=wall, steps=nsteps, judge=st)
+        log(f"  rollout {nsteps} steps in {wall:.2f} s: {res['particle_steps_per_s']:.4e} particle-steps/s, "
+            f"judge {st}")
+        if ref_state is not None:
+            res["error_vs_fp64"] = errors(state, ref_state)
+            log(f"  surrogate vs fp64 after {nsteps} steps: {res['error_vs_fp64']}")
+    return res
 
 
 def cpu_baseline(scene, steps, log):
@@ -191,7 +245,15 @@ def main():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB mebibytes (HBM counter calibration)")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed,surrogate")
+    ap.add_argument("--sur-scenes", type=int, default=256)
+    ap.add_argument("--sur-captures", type=int, default=80)
+    ap.add_argument("--sur-gap", type=int, default=0)
+    ap.add_argument("--sur-iters", type=int, default=2000)
+    ap.add_argument("--sur-lr", type=float, default=1e-3)
+    ap.add_argument("--sur-batch", type=int, default=1 << 20)
+    ap.add_argument("--sur-group", type=int, default=16)
+    ap.add_argument("--sur-threshold", type=float, default=1e-4)
     ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
@@ -253,12 +315,18 @@ def main():
         log(f"  {r['particle_steps_per_s']:.4e} particle-steps/s ({r['ms_per_step']:.3f} ms/step), "
             f"graph {r['particle_steps_per_s_graph']:.4e}, pair kernel {r['pair_kernel_ms']:.3f} ms")
 
-    acc = accuracy(scene, args.accuracy_steps, log) if args.accuracy_steps > 0 else None
+    acc, ref_state = accuracy(scene, args.accuracy_steps, log) if args.accuracy_steps > 0 else (None, None)
     if acc:
         log(f"  mixed vs fp64 after {acc['steps']} steps: pos {acc['rel_l2_pos']:.3e} "
             f"stress {acc['rel_l2_stress']:.3e}")
+    if "surrogate" in wanted:
+        log("mode surrogate")
+        modes["surrogate"] = bench_surrogate(scene, args, ref_state, log)
 
-    head = modes["mixed"] if "mixed" in modes else modes["fp64"]
+    head = modes.get("mixed") or modes.get("fp64")
+    if head is None:  # a surrogate-only run: no SPH pair-kernel timing
+        print(json.dumps({"metric": METRIC, "modes": modes, "accuracy": acc}), flush=True)
+        return
     achieved = ALG_BYTES_PER_PARTICLE_STAGE * n / (head["pair_kernel_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                 traffic=None, kernel="k_ul_pairs32",

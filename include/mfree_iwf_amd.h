@@ -238,8 +238,8 @@ mfree_status mfree_scene_plate(const mfree_plate_spec* spec, mfree_particles* ou
  *
  * One learned step (tau = 4 dt per body):
  *   D1 = derivatives at the state (the stage-0 pass of the UL pipeline);
- *   D2 = derivatives after rho and S were advanced by tau*D1;
- *   base update: S += tau*dS1, rho += tau*drho1, v += tau*a2,
+ *   D2 = derivatives after rho and S were advanced over tau with D1;
+ *   base update: S += tau*dS1, rho *= exp(tau*drho1/rho), v += tau*a2,
  *                x += tau*xdot1 + tau^2/2*a2        (FP64)
  *   plus the network's correction of all 8 fields.  The network is a
  *   3-layer message-passing net (hidden 128, SiLU) on the radius-2h
@@ -263,10 +263,11 @@ mfree_status mfree_surrogate_create(const mfree_surrogate_options* o, mfree_surr
 void mfree_surrogate_destroy(mfree_surrogate* s);
 const char* mfree_surrogate_last_error(const mfree_surrogate* s);
 
-/* Training data from rollouts of ctx (any number of bodies): per capture,
- * `gap` plain RK4 steps, the velocities are recorded, 4 plain steps, the
- * learned-step inputs at that state, then 4 more plain steps whose result
- * is the target.  One snapshot = one body at one capture; snapshots whose
+/* Training data from rollouts of ctx (any number of bodies): the
+ * velocities are recorded, 4 plain RK4 steps, then per capture the
+ * learned-step inputs at the current state and 4 plain steps whose result is
+ * the target (the next capture follows directly, or after `gap` more steps
+ * and a fresh 4-step velocity record).  One snapshot = one body at one capture; snapshots whose
  * body exceeds max_compression are kept out of training (counted by
  * mfree_surrogate_snapshots are the usable ones).  Recomputes the feature /
  * target normalisation over the usable snapshots. */
@@ -316,11 +317,12 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* ctx, uint32_t*
                                    float* edge3, uint64_t cap, uint64_t* nedges);
 
 /* Gradient check: one forward + backward of the network with weights w on a
- * host graph (CSR, at most 64*n edges, edge3 = (w, gx, gy) with w
- * symmetric and (gx, gy) antisymmetric), normalised node features x (n x 16)
- * and targets t (n x 8).  y: n x 8 outputs; loss: MSE; grad: dloss/dw. */
+ * host graph (CSR, at most 64 edges per node) over nodes at pos3 = (x, y, h) per
+ * node (the edge features are evaluated from these as in a learned step),
+ * normalised node features x (n x 16) and targets t (n x 8).  The graph
+ * must be symmetric.  y: n x 8 outputs; loss: MSE; grad: dloss/dw. */
 mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
-                                         const float* edge3, const float* x, const float* t, const float* w,
+                                         const float* pos3, const float* x, const float* t, const float* w,
                                          float* y, float* loss, float* grad);
 
 #ifdef __cplusplus

@@ -66,7 +66,8 @@ struct UlArgs {
   Fields8 y;                    // y_n (storage order), updated at stage 4
   Fields8 K;                    // next stage state (storage order)
   Fields8 acc;                  // RK accumulator (storage order)
-  double* d8;                   // stage 0: derivatives by global id
+  double* d8;                   // stage 0: derivatives by global id (by sorted position if d8_sorted)
+  int d8_sorted;
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;

@@ -9,10 +9,12 @@
 //             H' = H + SiLU([H, A] Wo^T + bo)                   [N x 128]
 //   Y   = H3 Wd^T + bd                                          [N x 16] (8 used)
 // with w, gx, gy the dimensionless cubic-spline weight and gradient of each
-// edge of the radius-2h graph (h_ij = (h_i + h_j)/2, self excluded), so w is
-// symmetric and (gx, gy) antisymmetric.  Dense layers run on MFMA
-// (v_mfma_f32_16x16x32_bf16, FP32 accumulation); weights are kept in FP32
-// (Adam master copy) and in BF16 (W and W^T for the backward GEMMs).
+// edge of the radius-2h graph (h_ij = (h_i + h_j)/2, self excluded),
+// computed on the fly from the node positions: w is symmetric and (gx, gy)
+// antisymmetric bit for bit.  Dense layers run on MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) with the weight tile staged
+// in LDS; weights are kept in FP32 (Adam master copy) and in BF16 (W and W^T
+// for the backward GEMMs).
 #pragma once
 
 #include <stdint.h>
@@ -34,7 +36,8 @@ constexpr int kG = kH + 3 * kC;  // 224: [H, a0, ax, ay]
 constexpr int kOut = 8;    // dx, dy, dvx, dvy, dSxx, dSxy, dSyy, drho
 constexpr int kOutP = 16;  // padded output width
 constexpr int kLayers = 3;
-constexpr uint32_t kEdgesPerNode = 64;  // edge capacity per node on average
+constexpr uint32_t kMaxDeg = 64;  // edge slots per node
+constexpr float kSplineNorm = 0.45472840883398667f;  // 10/(7 pi): the 2D cubic spline's factor times h^2
 
 typedef unsigned short bf16_t;  // raw bfloat16 bits
 
@@ -53,12 +56,43 @@ __device__ __host__ inline float bf2f(bf16_t b) {
   return f;
 }
 
+// cubic-spline shape f(q) and f'(q) (src/kernel.cpp:20-55 without the
+// 10/(7 pi h^2) factor)
+__device__ inline void spline_shape(float q, float& f, float& df) {
+  if (q < 1.f) {
+    f = 1.f - 1.5f * q * q * (1.f - 0.5f * q);
+    df = -3.f * q * (1.f - 0.75f * q);
+  } else if (q < 2.f) {
+    const float t = 2.f - q;
+    f = 0.25f * t * t * t;
+    df = -0.75f * t * t;
+  } else {
+    f = 0.f;
+    df = 0.f;
+  }
+}
+
+// Edge features from node records (x, y, h, -): w = W(r, h_ij) h_ij^2 and
+// (gx, gy) = grad_i W(r, h_ij) h_ij^3.  fl(xi - xj) = -fl(xj - xi), so the
+// pair (j, i) gets the same w and the negated gradient.
+__device__ inline void edge_weights(const float4 pi, const float4 pj, float& w, float& gx, float& gy) {
+  const float dx = pi.x - pj.x, dy = pi.y - pj.y;
+  const float hij = 0.5f * (pi.z + pj.z);
+  const float r = sqrtf(dx * dx + dy * dy);
+  float f, df;
+  spline_shape(r / hij, f, df);
+  w = kSplineNorm * f;
+  const float sc = r > 0.f ? kSplineNorm * df / r : 0.f;
+  gx = sc * dx;
+  gy = sc * dy;
+}
+
 // One parameter tensor: FP32 master, Adam moments, gradient, BF16 W and W^T.
 struct MpTensor {
   int rows = 0, cols = 0;        // W is rows x cols (out x in)
   size_t off = 0;                // offset into the flat FP32 arrays
-  bf16_t* w16 = nullptr;         // rows x cols
-  bf16_t* wt16 = nullptr;        // cols x rows (only for matrices used in backward)
+  bf16_t* w16 = nullptr;         // rows_p x cols_p (rows to 16, cols to 32, zero padded)
+  bf16_t* wt16 = nullptr;        // cols x rows_pk (rows to 32, zero padded)
 };
 
 struct MpNet {
@@ -75,34 +109,30 @@ struct MpNet {
 // Activations of one forward pass over N nodes (kept for the backward).
 struct MpActs {
   uint32_t cap = 0;
+  bool train = false;                   // training buffers allocated
   bf16_t* X0 = nullptr;                 // N x 32
-  float* Z[kLayers + 1] = {};           // pre-activations: Z0 (enc), Z1..Z3 (layers), N x 128
-  float* Hf[kLayers + 1] = {};          // hidden states FP32, N x 128
+  float* Z[kLayers + 1] = {};           // pre-activations: Z0 (enc), Z1..Z3 (layers), N x 128 (training)
+  float* Hf[kLayers + 1] = {};          // hidden states FP32, N x 128 (training)
   bf16_t* G[kLayers + 1] = {};          // [H, A] bf16, N x 224 (G[l] holds H_l and A of layer l+1)
   bf16_t* U[kLayers] = {};              // N x 32 bf16
   float* Y = nullptr;                   // N x 16
-  // backward scratch
+  // backward scratch (training)
   float* dH = nullptr;                  // N x 128
   bf16_t* dZ16 = nullptr;               // N x 224 (bf16 dZ / dY / dU operands)
-  float* dG = nullptr;                  // N x 224
   bf16_t* dA16 = nullptr;               // N x 96
-  float* dU = nullptr;                  // N x 32
   float* part = nullptr;                // dW partials
   size_t part_cap = 0;
   std::vector<void*> allocs;
 };
 
-// Edges of the radius-2h graph over sorted positions, CSR: node p owns
-// edges [rowp[p], rowp[p+1]).  Capacity kEdgesPerNode per node on average;
-// single nodes may have any degree.
+// Radius-2h graph over sorted positions, slot-major (ELL): the k-th edge
+// of node p is col[k * cap + p], k < deg[p] <= kMaxDeg, so the lanes of a
+// wave that handle consecutive nodes read and write consecutive words.
 struct MpGraph {
-  uint32_t cap = 0;           // nodes
-  uint64_t ecap = 0;          // edges
-  uint32_t* deg = nullptr;    // cap + 1 (degrees, scanned into rowp)
-  uint32_t* rowp = nullptr;   // cap + 1
-  uint32_t* tsum = nullptr;   // scan tile sums
-  uint32_t* col = nullptr;    // ecap (sorted positions)
-  float4* ew = nullptr;       // ecap: (w, gx, gy, 0)
+  uint32_t cap = 0;           // nodes (the slot stride)
+  uint32_t* deg = nullptr;    // cap
+  uint32_t* col = nullptr;    // kMaxDeg * cap (sorted positions)
+  float4* npos = nullptr;     // cap: (x, y, h, 0) FP32 by sorted position
   float* wsum = nullptr;      // cap: sum of w over the edges and the node itself
   std::vector<void*> allocs;
 };
@@ -113,12 +143,13 @@ void mp_graph_free(MpGraph& g);
 mfree_status mp_net_init(MpNet& net, uint64_t seed, std::string& msg);
 void mp_net_free(MpNet& net);
 void mp_net_sync16(MpNet& net, hipStream_t st);  // FP32 master -> BF16 copies
-mfree_status mp_acts_reserve(MpActs& a, uint32_t n, std::string& msg);
+mfree_status mp_acts_reserve(MpActs& a, uint32_t n, bool train, std::string& msg);
 void mp_acts_free(MpActs& a);
 
 // Forward over n nodes: features in a.X0, graph g; outputs a.Y.  With
-// train == false the pre-activations (only needed by the backward) are not
-// stored.
+// train == false only the BF16 activations are kept (the residual stream
+// is the BF16 copy of H); with train == true the FP32 hidden states and the
+// pre-activations the backward needs are stored too.
 void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st);
 // Loss + dY from targets T (n x 8, normalised) -> accumulates weight
 // gradients into net.g (zeroed by the caller).  loss_out: device float.

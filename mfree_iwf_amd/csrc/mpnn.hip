@@ -1,15 +1,19 @@
 // Message-passing surrogate: forward and backward kernels on gfx950.
 //
 // Dense layers: k_gemm computes out[n][m] = sum_k X[n][k] W[m][k] with
-// v_mfma_f32_16x16x32_bf16 (a wave owns 16 nodes and every 16-wide output
-// tile; operands are 8 contiguous bf16 per lane: A row = lane%16, B column =
-// lane%16, result column = lane%16 / rows 4*(lane/16)+v) and fused
-// epilogues (bias, SiLU, residual, FP32 / BF16 stores).  Weight gradients
-// (k_gemm_dw) use the same instruction with the node axis as the reduction
-// axis and per-chunk partial sums.  The aggregation over the neighbour
-// graph is a gather per node; its backward is the same gather with the
-// signs of the antisymmetric gradient weights flipped (the graph and w are
-// symmetric, gx and gy antisymmetric).
+// v_mfma_f32_16x16x32_bf16.  Operands are 8 contiguous bf16 per lane (A row
+// = lane%16, B column = lane%16; result column = lane%16, rows
+// 4*(lane/16)+v).  A workgroup stages the whole weight (at most 128 x 224
+// bf16, rows padded by 16 bytes against LDS bank conflicts) in LDS once and
+// then walks node tiles (persistent grid); a wave owns ROWS x 16 nodes and
+// every 16-wide output tile, so each weight fragment read from LDS feeds
+// ROWS MFMAs.  Epilogues fuse bias, SiLU, residual and the FP32 / BF16
+// stores.  Weight gradients (k_gemm_dw) use the same instruction with the
+// node axis as the reduction axis and per-chunk partial sums.  The
+// aggregation over the neighbour graph is a gather per node (one node per
+// lane, all 32 channels in registers) that evaluates the edge weights from
+// the node positions; its backward is the same gather with the signs of the
+// antisymmetric gradient weights flipped.
 #include "mpnn.h"
 
 #include <math.h>
@@ -37,66 +41,124 @@ enum Epi : int {
   EPI_F32_ACC = 1,   // out += acc
   EPI_BF16 = 2,      // out16 = bf16(acc + bias)
   EPI_SILU_RES = 3,  // z = acc + bias -> zout; h = res + silu(z) -> out (f32), out16 (bf16)
+  EPI_SPLIT = 4,     // columns < kH: out[n][m] += acc (ld kH); others: out16[n][m - kH] = bf16(acc) (ld 3 kC)
 };
 
 struct GemmEpi {
   int mode;
   const float* bias;            // nbias entries (columns beyond get no bias)
   int nbias;
-  float* out;
+  float* out;                   // FP32 output (or null)
   int ldo;
-  bf16_t* out16;
+  bf16_t* out16;                // BF16 output
   int ld16;
-  float* zout;
-  const float* res;
+  float* zout;                  // pre-activation (EPI_SILU_RES, or null)
+  const float* res;             // FP32 residual (ld kH), or
+  const bf16_t* res16;          // BF16 residual (ld kG), or neither
 };
 
-template <int K, int M>
-__global__ __launch_bounds__(256) void k_gemm(const bf16_t* __restrict__ X, int ldx, const bf16_t* __restrict__ W,
-                                              uint32_t N, GemmEpi e) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t nb = blockIdx.x * 64u + wave * 16u;
-  if (nb >= N) return;
-  const uint32_t row = nb + (lane & 15);
-  const int kq = (lane >> 4) * 8;
+constexpr int kGemmBlocksMax = 512;  // 2 per CU
+constexpr int kGemmThreads = 512;    // 8 waves share the LDS weight tile
+
+__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) {
+  return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+}
+__device__ inline float4 unpack4_bf16(uint2 u) {
+  return make_float4(bf2f((bf16_t)(u.x & 0xFFFF)), bf2f((bf16_t)(u.x >> 16)), bf2f((bf16_t)(u.y & 0xFFFF)),
+                     bf2f((bf16_t)(u.y >> 16)));
+}
+
+// out = X W^T over N rows; W is M x K (bf16, row-major, K % 32 == 0,
+// M % 16 == 0).  The MFMA computes the transposed tile D = W X^T: A = 16
+// weight rows, B = 16 node rows (both 8 contiguous bf16 per lane), so a lane
+// ends with one node and 4 consecutive output channels (vector epilogue).
+// Persistent: block b handles node tiles b, b + grid, ...
+template <int K, int M, int ROWS, int MODE>
+__global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict__ X, int ldx,
+                                                       const bf16_t* __restrict__ W, uint32_t N, GemmEpi e) {
+  constexpr int KP = K + 8;  // padded LDS row
   constexpr int T = M / 16;
-  f32x4 acc[T];
-#pragma unroll
-  for (int t = 0; t < T; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 a;
-    if (row < N)
-      a = *(const bf16x8*)(X + (size_t)row * ldx + k0 + kq);
-    else
-      a = bf16x8{};
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-      const bf16x8 b = *(const bf16x8*)(W + (size_t)(t * 16 + (lane & 15)) * K + k0 + kq);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-    }
+  constexpr int kWaves = kGemmThreads / 64;
+  constexpr uint32_t kRowsPerBlock = kWaves * ROWS * 16;
+  __shared__ __attribute__((aligned(16))) bf16_t wl[M * KP];
+  for (int i = threadIdx.x * 8; i < M * K; i += kGemmThreads * 8) {
+    const int r = i / K, c = i - r * K;
+    *(uint4*)&wl[r * KP + c] = *(const uint4*)&W[i];
   }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kq = (lane >> 4) * 8;
+  const int mq = (lane >> 4) * 4;  // first of the lane's 4 output channels within a 16-wide tile
+  for (uint32_t tile = blockIdx.x; tile * kRowsPerBlock < N; tile += gridDim.x) {
+    const uint32_t nb = tile * kRowsPerBlock + wave * ROWS * 16;
+    if (nb >= N) continue;
+    f32x4 acc[ROWS][T];
 #pragma unroll
-  for (int t = 0; t < T; t++) {
-    const int m = t * 16 + (lane & 15);
+    for (int r = 0; r < ROWS; r++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const uint32_t n = nb + (lane >> 4) * 4 + v;
+      for (int t = 0; t < T; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // all node-row fragments of the tile in flight before the first MFMA
+    bf16x8 xb[K / 32][ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const uint32_t row = nb + r * 16 + (lane & 15);
+#pragma unroll
+      for (int kk = 0; kk < K / 32; kk++)
+        xb[kk][r] = row < N ? *(const bf16x8*)(X + (size_t)row * ldx + kk * 32 + kq) : bf16x8{};
+    }
+#pragma unroll
+    for (int kk = 0; kk < K / 32; kk++) {
+#pragma unroll
+      for (int t = 0; t < T; t++) {
+        const bf16x8 wa = *(const bf16x8*)&wl[(t * 16 + (lane & 15)) * KP + kk * 32 + kq];
+#pragma unroll
+        for (int r = 0; r < ROWS; r++)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb[kk][r], acc[r][t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const uint32_t n = nb + r * 16 + (lane & 15);
       if (n >= N) continue;
-      float r = acc[t][v];
-      const float bm = (e.bias && m < e.nbias) ? e.bias[m] : 0.f;
-      if (e.mode == EPI_F32) {
-        e.out[(size_t)n * e.ldo + m] = r + bm;
-      } else if (e.mode == EPI_F32_ACC) {
-        e.out[(size_t)n * e.ldo + m] += r;
-      } else if (e.mode == EPI_BF16) {
-        e.out16[(size_t)n * e.ld16 + m] = f2bf(r + bm);
-      } else {
-        const float z = r + bm;
-        if (e.zout) e.zout[(size_t)n * kH + m] = z;
-        const float h = (e.res ? e.res[(size_t)n * kH + m] : 0.f) + silu(z);
-        e.out[(size_t)n * kH + m] = h;
-        e.out16[(size_t)n * e.ld16 + m] = f2bf(h);
+#pragma unroll
+      for (int t = 0; t < T; t++) {
+        const int m = t * 16 + mq;
+        float4 x = make_float4(acc[r][t][0], acc[r][t][1], acc[r][t][2], acc[r][t][3]);
+        if (e.bias) {
+          x.x += m < e.nbias ? e.bias[m] : 0.f;
+          x.y += m + 1 < e.nbias ? e.bias[m + 1] : 0.f;
+          x.z += m + 2 < e.nbias ? e.bias[m + 2] : 0.f;
+          x.w += m + 3 < e.nbias ? e.bias[m + 3] : 0.f;
+        }
+        if constexpr (MODE == EPI_F32) {
+          *(float4*)&e.out[(size_t)n * e.ldo + m] = x;
+        } else if constexpr (MODE == EPI_F32_ACC) {
+          float4* o = (float4*)&e.out[(size_t)n * e.ldo + m];
+          const float4 y = *o;
+          *o = make_float4(y.x + x.x, y.y + x.y, y.z + x.z, y.w + x.w);
+        } else if constexpr (MODE == EPI_BF16) {
+          *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(x.x, x.y, x.z, x.w);
+        } else if constexpr (MODE == EPI_SPLIT) {
+          if (m < kH) {
+            float4* o = (float4*)&e.out[(size_t)n * kH + m];
+            const float4 y = *o;
+            *o = make_float4(y.x + x.x, y.y + x.y, y.z + x.z, y.w + x.w);
+          } else {
+            *(uint2*)&e.out16[(size_t)n * (3 * kC) + (m - kH)] = pack4_bf16(x.x, x.y, x.z, x.w);
+          }
+        } else {
+          if (e.zout) *(float4*)&e.zout[(size_t)n * kH + m] = x;
+          float4 h = make_float4(silu(x.x), silu(x.y), silu(x.z), silu(x.w));
+          if (e.res) {
+            const float4 rr = *(const float4*)&e.res[(size_t)n * kH + m];
+            h = make_float4(h.x + rr.x, h.y + rr.y, h.z + rr.z, h.w + rr.w);
+          } else if (e.res16) {
+            const float4 rr = unpack4_bf16(*(const uint2*)&e.res16[(size_t)n * kG + m]);
+            h = make_float4(h.x + rr.x, h.y + rr.y, h.z + rr.z, h.w + rr.w);
+          }
+          if (e.out) *(float4*)&e.out[(size_t)n * kH + m] = h;
+          *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(h.x, h.y, h.z, h.w);
+        }
       }
     }
   }
@@ -197,46 +259,97 @@ __global__ __launch_bounds__(256) void k_colsum(const bf16_t* __restrict__ dY, i
   part[(size_t)blockIdx.x * M + m] = s;
 }
 
-// Aggregation forward: for node i and channel c,
-//   G[i][128+c] = sum_e w U[j][c], G[i][160+c] = sum_e gx U[j][c], G[i][192+c] = sum_e gy U[j][c]
-// 32 lanes per node.
-__global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, const uint32_t* __restrict__ rowp,
-                                                 const uint32_t* __restrict__ col, const float4* __restrict__ ew,
-                                                 const bf16_t* __restrict__ U, bf16_t* __restrict__ G) {
-  const uint32_t i = blockIdx.x * 8u + (threadIdx.x >> 5);
-  const int c = threadIdx.x & 31;
-  if (i >= n) return;
-  float a0 = 0.f, ax = 0.f, ay = 0.f;
-  const uint32_t e0 = rowp[i], e1 = rowp[i + 1];
-  for (uint32_t e = e0; e < e1; e++) {
-    const float4 w = ew[e];
-    const float u = bf2f(U[(size_t)col[e] * kC + c]);
-    a0 += w.x * u;
-    ax += w.y * u;
-    ay += w.z * u;
+__device__ inline void unpack8_bf16(const uint4 u, float* f) {
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[2 * k] = bf2f((bf16_t)(w[k] & 0xFFFF));
+    f[2 * k + 1] = bf2f((bf16_t)(w[k] >> 16));
   }
-  bf16_t* g = G + (size_t)i * kG + kH;
-  g[c] = f2bf(a0);
-  g[kC + c] = f2bf(ax);
-  g[2 * kC + c] = f2bf(ay);
+}
+__device__ inline uint4 pack8_bf16(const float* f) {
+  uint32_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-// Aggregation backward: dU[j][c] = sum_{i in N(j)} (w dA0_i - gx_ji dAx_i - gy_ji dAy_i)
-// (w symmetric, gx/gy antisymmetric); dA = dG[:, 128:224].
-__global__ __launch_bounds__(256) void k_agg_bwd(uint32_t n, const uint32_t* __restrict__ rowp,
-                                                 const uint32_t* __restrict__ col, const float4* __restrict__ ew,
-                                                 const float* __restrict__ dG, float* __restrict__ dU) {
-  const uint32_t j = blockIdx.x * 8u + (threadIdx.x >> 5);
-  const int c = threadIdx.x & 31;
-  if (j >= n) return;
-  float s = 0.f;
-  const uint32_t e0 = rowp[j], e1 = rowp[j + 1];
-  for (uint32_t e = e0; e < e1; e++) {
-    const float4 w = ew[e];
-    const float* d = dG + (size_t)col[e] * kG + kH;
-    s += w.x * d[c] - w.y * d[kC + c] - w.z * d[2 * kC + c];
+// Aggregation forward, one node per lane (consecutive nodes on consecutive
+// lanes, so the slot-major edge lists load coalesced):
+//   G[i][128+c] = sum_e w U[j][c], G[i][160+c] = sum_e gx U[j][c], G[i][192+c] = sum_e gy U[j][c]
+__global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, const uint32_t* __restrict__ deg,
+                                                 const uint32_t* __restrict__ col, const float4* __restrict__ npos,
+                                                 const bf16_t* __restrict__ U, bf16_t* __restrict__ G) {
+  const uint32_t i = xcd_tile() * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = npos[i];
+  float a0[kC], ax[kC], ay[kC];
+#pragma unroll
+  for (int c = 0; c < kC; c++) a0[c] = ax[c] = ay[c] = 0.f;
+  const uint32_t d = deg[i];
+  for (uint32_t k = 0; k < d; k++) {
+    const uint32_t j = col[(size_t)k * stride + i];
+    const float4 pj = npos[j];
+    const uint4* u = (const uint4*)(U + (size_t)j * kC);
+    uint4 uv[kC / 8];
+#pragma unroll
+    for (int q = 0; q < kC / 8; q++) uv[q] = u[q];
+    float w, gx, gy;
+    edge_weights(pi, pj, w, gx, gy);
+#pragma unroll
+    for (int q = 0; q < kC / 8; q++) {
+      float f[8];
+      unpack8_bf16(uv[q], f);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        a0[8 * q + c] += w * f[c];
+        ax[8 * q + c] += gx * f[c];
+        ay[8 * q + c] += gy * f[c];
+      }
+    }
   }
-  dU[(size_t)j * kC + c] = s;
+  uint4* g = (uint4*)(G + (size_t)i * kG + kH);
+#pragma unroll
+  for (int q = 0; q < kC / 8; q++) {
+    g[q] = pack8_bf16(a0 + 8 * q);
+    g[kC / 8 + q] = pack8_bf16(ax + 8 * q);
+    g[2 * (kC / 8) + q] = pack8_bf16(ay + 8 * q);
+  }
+}
+
+// Aggregation backward, one node per lane:
+//   dU[j][c] = sum_{i in N(j)} (w dA0_i - gx_ji dAx_i - gy_ji dAy_i)
+// (w symmetric, gx/gy antisymmetric); dA = dA16 [n x 96] bf16.  Writes dU
+// as bf16 into dU16 (ld ldu).
+__global__ __launch_bounds__(256) void k_agg_bwd(uint32_t n, uint32_t stride, const uint32_t* __restrict__ deg,
+                                                 const uint32_t* __restrict__ col, const float4* __restrict__ npos,
+                                                 const bf16_t* __restrict__ dA16, bf16_t* __restrict__ dU16, int ldu) {
+  const uint32_t j = xcd_tile() * 256u + threadIdx.x;
+  if (j >= n) return;
+  const float4 pj = npos[j];
+  float s[kC];
+#pragma unroll
+  for (int c = 0; c < kC; c++) s[c] = 0.f;
+  const uint32_t d = deg[j];
+  for (uint32_t k = 0; k < d; k++) {
+    const uint32_t i = col[(size_t)k * stride + j];
+    const float4 pi = npos[i];
+    float w, gx, gy;
+    edge_weights(pj, pi, w, gx, gy);  // (gx, gy)_ji = -(gx, gy)_ij
+    const uint4* da = (const uint4*)(dA16 + (size_t)i * (3 * kC));
+#pragma unroll
+    for (int q = 0; q < kC / 8; q++) {
+      float f0[8], f1[8], f2[8];
+      unpack8_bf16(da[q], f0);
+      unpack8_bf16(da[kC / 8 + q], f1);
+      unpack8_bf16(da[2 * (kC / 8) + q], f2);
+#pragma unroll
+      for (int c = 0; c < 8; c++) s[8 * q + c] += w * f0[c] - gx * f1[c] - gy * f2[c];
+    }
+  }
+  uint4* o = (uint4*)(dU16 + (size_t)j * ldu);
+#pragma unroll
+  for (int q = 0; q < kC / 8; q++) o[q] = pack8_bf16(s + 8 * q);
 }
 
 // dZ = dH * silu'(Z) -> bf16 operand (ld 224 buffer, columns [0,128))
@@ -246,23 +359,6 @@ __global__ void k_silu_bwd(uint32_t nelem, const float* __restrict__ dH, const f
   if (idx >= nelem) return;
   const uint32_t n = idx / kH, m = idx % kH;
   dZ16[(size_t)n * ld16 + m] = f2bf(dH[idx] * silu_d(Z[idx]));
-}
-
-// f32 [N x cols] -> bf16 [N x ld16] (columns [0, cols))
-__global__ void k_to_bf16(uint32_t N, int cols, const float* __restrict__ src, int lds, bf16_t* __restrict__ dst,
-                          int ldd) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N * (uint32_t)cols) return;
-  const uint32_t n = idx / cols, m = idx % cols;
-  dst[(size_t)n * ldd + m] = f2bf(src[(size_t)n * lds + m]);
-}
-
-// dH += dG[:, :128]  (residual + the H part of [H, A])
-__global__ void k_add_dH(uint32_t nelem, const float* __restrict__ dG, float* __restrict__ dH) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nelem) return;
-  const uint32_t n = idx / kH, m = idx % kH;
-  dH[idx] += dG[(size_t)n * kG + m];
 }
 
 // MSE loss over the kOut outputs: dY = 2 (Y - T) / (kOut N) as bf16 (ld 32,
@@ -319,9 +415,28 @@ __global__ void k_sync16(const float* __restrict__ w, int rows, int cols, int co
 
 inline uint32_t blocks(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
 
-template <int K, int M>
+// Rows per wave: accumulators ROWS x M/16 x 4 registers stay within budget.
+template <int K, int M, int ROWS = (M > 128 ? 1 : (M > 32 ? 2 : 4))>
 void gemm(const bf16_t* X, int ldx, const bf16_t* W, uint32_t N, const GemmEpi& e, hipStream_t st) {
-  hipLaunchKernelGGL((k_gemm<K, M>), dim3(blocks(N, 64)), dim3(256), 0, st, X, ldx, W, N, e);
+  const uint32_t tiles = blocks(N, (kGemmThreads / 64) * ROWS * 16);
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles, kGemmBlocksMax));
+  switch (e.mode) {
+    case EPI_F32:
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      break;
+    case EPI_F32_ACC:
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32_ACC>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      break;
+    case EPI_BF16:
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_BF16>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      break;
+    case EPI_SPLIT:
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SPLIT>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SILU_RES>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      break;
+  }
 }
 
 GemmEpi epi(int mode) {
@@ -439,8 +554,11 @@ void mp_net_sync16(MpNet& net, hipStream_t st) {
   }
 }
 
-mfree_status mp_acts_reserve(MpActs& a, uint32_t n, std::string& msg) {
-  if (n <= a.cap) return MFREE_OK;
+mfree_status mp_acts_reserve(MpActs& a, uint32_t n, bool train, std::string& msg) {
+  if (n <= a.cap && (a.train || !train)) return MFREE_OK;
+  // grow to cover both the previous and the new request
+  n = std::max(n, a.cap);
+  train = train || a.train;
   mp_acts_free(a);
   auto al = [&](void** p, size_t bytes) -> bool {
     if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
@@ -448,20 +566,23 @@ mfree_status mp_acts_reserve(MpActs& a, uint32_t n, std::string& msg) {
     return true;
   };
   const size_t N = n;
-  bool ok = al((void**)&a.X0, N * kK0 * 2) && al((void**)&a.Y, N * kOutP * 4) && al((void**)&a.dH, N * kH * 4) &&
-            al((void**)&a.dZ16, N * kG * 2) && al((void**)&a.dG, N * kG * 4) && al((void**)&a.dA16, N * 96 * 2) &&
-            al((void**)&a.dU, N * kC * 4);
-  for (int l = 0; l <= kLayers && ok; l++)
-    ok = al((void**)&a.Z[l], N * kH * 4) && al((void**)&a.Hf[l], N * kH * 4) && al((void**)&a.G[l], N * kG * 2);
+  bool ok = al((void**)&a.X0, N * kK0 * 2) && al((void**)&a.Y, N * kOutP * 4);
+  for (int l = 0; l <= kLayers && ok; l++) ok = al((void**)&a.G[l], N * kG * 2);
   for (int l = 0; l < kLayers && ok; l++) ok = al((void**)&a.U[l], N * kC * 2);
-  a.part_cap = (size_t)blocks(N, kDwChunk) * kH * kG;
-  ok = ok && al((void**)&a.part, a.part_cap * 4);
+  if (train) {
+    ok = ok && al((void**)&a.dH, N * kH * 4) && al((void**)&a.dZ16, N * kG * 2) &&
+         al((void**)&a.dA16, N * 3 * kC * 2);
+    for (int l = 0; l <= kLayers && ok; l++) ok = al((void**)&a.Z[l], N * kH * 4) && al((void**)&a.Hf[l], N * kH * 4);
+    a.part_cap = (size_t)blocks(N, kDwChunk) * kH * kG;
+    ok = ok && al((void**)&a.part, a.part_cap * 4);
+  }
   if (!ok) {
     mp_acts_free(a);
     msg = "hipMalloc failed (activations)";
     return MFREE_ERR_HIP;
   }
   a.cap = n;
+  a.train = train;
   return MFREE_OK;
 }
 
@@ -469,6 +590,11 @@ void mp_acts_free(MpActs& a) {
   for (void* p : a.allocs) (void)hipFree(p);
   a.allocs.clear();
   a.cap = 0;
+  a.train = false;
+  for (int l = 0; l <= kLayers; l++) a.Z[l] = a.Hf[l] = nullptr;
+  a.dH = nullptr;
+  a.dZ16 = a.dA16 = nullptr;
+  a.part = nullptr;
 }
 
 void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st) {
@@ -477,7 +603,7 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
   e.bias = net.w + net.t[net.be].off;
   e.nbias = kH;
   e.zout = train ? a.Z[0] : nullptr;
-  e.out = a.Hf[0];
+  e.out = train ? a.Hf[0] : nullptr;
   e.out16 = a.G[0];
   e.ld16 = kG;
   gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
@@ -488,14 +614,16 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
     eu.ld16 = kC;
     gemm<kH, kC>(a.G[l], kG, net.t[net.Wu[l]].w16, n, eu, st);
     // A = aggregate(U) into G[l][:, 128:224]
-    hipLaunchKernelGGL(k_agg_fwd, dim3(blocks(n, 8)), dim3(256), 0, st, n, g.rowp, g.col, g.ew, a.U[l], a.G[l]);
+    hipLaunchKernelGGL(k_agg_fwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,
+                       a.U[l], a.G[l]);
     // H_{l+1} = H_l + SiLU([H, A] Wo^T + bo)
     GemmEpi eo = epi(EPI_SILU_RES);
     eo.bias = net.w + net.t[net.bo[l]].off;
     eo.nbias = kH;
     eo.zout = train ? a.Z[l + 1] : nullptr;
-    eo.res = a.Hf[l];
-    eo.out = a.Hf[l + 1];
+    eo.res = train ? a.Hf[l] : nullptr;
+    eo.res16 = train ? nullptr : a.G[l];
+    eo.out = train ? a.Hf[l + 1] : nullptr;
     eo.out16 = a.G[l + 1];
     eo.ld16 = kG;
     gemm<kG, kH>(a.G[l], kG, net.t[net.Wo[l]].w16, n, eo, st);
@@ -517,15 +645,12 @@ mfree_status mp_graph_reserve(MpGraph& g, uint32_t n, std::string& msg) {
     return true;
   };
   const size_t N = n;
-  const uint64_t E = (uint64_t)n * kEdgesPerNode;
-  if (!al((void**)&g.deg, (N + 1) * 4) || !al((void**)&g.rowp, (N + 1) * 4) ||
-      !al((void**)&g.tsum, ((N + 1 + kScanTile - 1) / kScanTile + 1) * 4) || !al((void**)&g.col, E * 4) ||
-      !al((void**)&g.ew, E * 16) || !al((void**)&g.wsum, N * 4)) {
+  if (!al((void**)&g.deg, N * 4) || !al((void**)&g.col, N * kMaxDeg * 4) || !al((void**)&g.npos, N * 16) ||
+      !al((void**)&g.wsum, N * 4)) {
     mp_graph_free(g);
     msg = "hipMalloc failed (graph)";
     return MFREE_ERR_HIP;
   }
-  g.ecap = E;
   g.cap = n;
   return MFREE_OK;
 }
@@ -534,7 +659,6 @@ void mp_graph_free(MpGraph& g) {
   for (void* p : g.allocs) (void)hipFree(p);
   g.allocs.clear();
   g.cap = 0;
-  g.ecap = 0;
 }
 
 void mp_zero_grad(MpNet& net, hipStream_t st) {
@@ -555,21 +679,20 @@ void mp_backward(MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, const floa
   e.ldo = kH;
   gemm<32, kH>(a.dZ16, 32, net.t[net.Wd].wt16, n, e, st);
   for (int l = kLayers - 1; l >= 0; l--) {
-    // dZ = dH * silu'(Z_{l+1}); residual keeps dH for H_l
+    // dZ = dH * silu'(Z_{l+1}); the residual keeps dH for H_l
     hipLaunchKernelGGL(k_silu_bwd, dim3(blocks((size_t)n * kH, 256)), dim3(256), 0, st, n * kH, a.dH, a.Z[l + 1],
                        a.dZ16, kG);
-    // dWo = dZ^T G_l, dbo = sum dZ, dG = dZ Wo
+    // dWo = dZ^T G_l, dbo = sum dZ
     gemm_dw<kG>(a.dZ16, kG, a.G[l], kG, n, kH, a.part, net.g + net.t[net.Wo[l]].off, st);
     bias_grad(a.dZ16, kG, n, kH, a.part, net.g + net.t[net.bo[l]].off, st);
-    GemmEpi eg = epi(EPI_F32);
-    eg.out = a.dG;
-    eg.ldo = kG;
+    // [dH_l += dG[:, :128], dA = dG[:, 128:224]] with dG = dZ Wo
+    GemmEpi eg = epi(EPI_SPLIT);
+    eg.out = a.dH;
+    eg.out16 = a.dA16;
     gemm<kH, kG>(a.dZ16, kG, net.t[net.Wo[l]].wt16, n, eg, st);
-    // dH_l = dH_{l+1} + dG[:, :128]
-    hipLaunchKernelGGL(k_add_dH, dim3(blocks((size_t)n * kH, 256)), dim3(256), 0, st, n * kH, a.dG, a.dH);
-    // dU = aggregate^T(dA)
-    hipLaunchKernelGGL(k_agg_bwd, dim3(blocks(n, 8)), dim3(256), 0, st, n, g.rowp, g.col, g.ew, a.dG, a.dU);
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks((size_t)n * kC, 256)), dim3(256), 0, st, n, kC, a.dU, kC, a.dZ16, kG);
+    // dU = aggregate^T(dA) -> bf16 operand (dZ16 columns 0..31)
+    hipLaunchKernelGGL(k_agg_bwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,
+                       a.dA16, a.dZ16, kG);
     // dWu = dU^T H_l, dH_l += dU Wu
     gemm_dw<kH>(a.dZ16, kG, a.G[l], kG, n, kC, a.part, net.g + net.t[net.Wu[l]].off, st);
     GemmEpi eu = epi(EPI_F32_ACC);
@@ -577,7 +700,7 @@ void mp_backward(MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, const floa
     eu.ldo = kH;
     gemm<kC, kH>(a.dZ16, kG, net.t[net.Wu[l]].wt16, n, eu, st);
   }
-  // encoder: dZ0 = dH0 * silu'(Z0); dWe = dZ0^T X0 (first 9 columns), dbe
+  // encoder: dZ0 = dH0 * silu'(Z0); dWe = dZ0^T X0 (first kNF columns), dbe
   hipLaunchKernelGGL(k_silu_bwd, dim3(blocks((size_t)n * kH, 256)), dim3(256), 0, st, n * kH, a.dH, a.Z[0], a.dZ16,
                      kG);
   hipLaunchKernelGGL((k_gemm_dw<kK0>), dim3(kH / 16, nch), dim3(256), 0, st, a.dZ16, kG, a.X0, kK0, n, kH, a.part);

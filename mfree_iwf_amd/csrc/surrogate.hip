@@ -5,10 +5,10 @@
 // One learned step of a UL context (every body advances by tau = 4 dt):
 //   1. D1 = time derivatives at the state (the stage-0 pass of the UL
 //      pipeline, src/body.h:125-148)
-//   2. rho and S advanced by tau*D1 (velocity and positions kept); D2 =
+//   2. rho and S advanced by D1 over tau (velocity and positions kept); D2 =
 //      derivatives at that state
-//   3. base update (FP64):  S += tau*dS1, rho += tau*drho1, v += tau*a2,
-//                           x += tau*xdot1 + tau^2/2*a2
+//   3. base update (FP64):  S += tau*dS1, rho *= exp(tau*drho1/rho),
+//                           v += tau*a2, x += tau*xdot1 + tau^2/2*a2
 //   4. the network adds a correction to all 8 fields (FP64 update), its
 //      inputs being 16 dimensionless node features built from the state,
 //      D1, D2 and the velocity one learned step earlier.
@@ -52,39 +52,20 @@ struct GraphArgs {
   const SceneConst* sc;
   const SceneGeom* geom;
   const double *x, *y, *h;  // storage order
-  uint32_t* deg;            // count pass output
-  const uint32_t* rowp;     // fill pass input
-  uint64_t ecap;
+  uint32_t stride;          // slot stride of col
+  uint32_t* deg;
   uint32_t* col;
-  float4* ew;
+  float4* npos;
   float* wsum;
   uint32_t* err;
 };
 
-// cubic-spline shape f(q) and f'(q) (src/kernel.cpp:20-55 without the
-// 10/(7 pi h^2) factor)
-__device__ inline void spline_shape(float q, float& f, float& df) {
-  if (q < 1.f) {
-    f = 1.f - 1.5f * q * q * (1.f - 0.5f * q);
-    df = -3.f * q * (1.f - 0.75f * q);
-  } else if (q < 2.f) {
-    const float t = 2.f - q;
-    f = 0.25f * t * t * t;
-    df = -0.75f * t * t;
-  } else {
-    f = 0.f;
-    df = 0.f;
-  }
-}
-
 // One thread per sorted position p: edges to every other particle of the
-// 3x3 cell block with r < 2 h_ij (h_ij = (h_i + h_j)/2).  The predicate and
-// the weights only use symmetric FP64 differences, so the graph is symmetric,
-// w_ij = w_ji and (gx, gy)_ij = -(gx, gy)_ji exactly.  Two passes: degrees
-// and kernel sums, then (after a scan into rowp) the edges.
-template <bool FILL>
+// 3x3 cell block with r < 2 h_ij (h_ij = (h_i + h_j)/2), decided in FP64 from
+// symmetric differences, so the graph is symmetric.  Writes the slot-major
+// edge list, the degree, the kernel sum and the FP32 node record.
 __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const uint32_t s = a.perm[p];
   const uint32_t b = a.scene[s];
@@ -95,7 +76,6 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   const double xi = a.x[s], yi = a.y[s], hi = a.h[s];
   uint32_t d = 0;
   float ws = kSplineNorm;  // the node itself (q = 0)
-  const uint64_t e0 = FILL ? a.rowp[p] : 0;
   const int cx0 = max(ix - 1, 0), cx1 = min(ix + 1, (int)g.nx - 1);
   const int cy0 = max(iy - 1, 0), cy1 = min(iy + 1, (int)g.ny - 1);
   for (int cx = cx0; cx <= cx1; cx++) {
@@ -108,44 +88,46 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
       const double r2 = dx * dx + dy * dy;
       const double hij = 0.5 * (hi + a.h[j]);
       if (!(r2 < 4.0 * hij * hij)) continue;
-      const double r = sqrt(r2);
       float f, df;
-      spline_shape((float)(r / hij), f, df);
-      const float w = kSplineNorm * f;
-      if (FILL) {
-        float gx = 0.f, gy = 0.f;
-        if (r > 0.0) {
-          const float sc = kSplineNorm * df / (float)r;
-          gx = sc * (float)dx;
-          gy = sc * (float)dy;
-        }
-        if (e0 + d < a.ecap) {
-          a.col[e0 + d] = q;
-          a.ew[e0 + d] = make_float4(w, gx, gy, 0.f);
-        }
-      }
+      spline_shape((float)(sqrt(r2) / hij), f, df);
+      ws += kSplineNorm * f;
+      if (d < kMaxDeg) a.col[(size_t)d * a.stride + p] = q;
       d++;
-      ws += w;
     }
   }
-  if (FILL) {
-    if (e0 + d > a.ecap) atomicOr(a.err, EB_NBH);
-  } else {
-    a.deg[p] = d;
-    a.wsum[p] = ws;
+  if (d > kMaxDeg) {
+    atomicOr(a.err, EB_NBH);
+    d = kMaxDeg;
   }
+  a.deg[p] = d;
+  a.wsum[p] = ws;
+  a.npos[p] = make_float4((float)xi, (float)yi, (float)hi, 0.f);
 }
 
 void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
+  g.stride = mg.cap;
   g.deg = mg.deg;
-  g.rowp = mg.rowp;
-  g.ecap = mg.ecap;
   g.col = mg.col;
-  g.ew = mg.ew;
+  g.npos = mg.npos;
   g.wsum = mg.wsum;
-  hipLaunchKernelGGL(k_graph<false>, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
-  launch_excl_scan(st, mg.deg, g.n, mg.tsum, mg.rowp);
-  hipLaunchKernelGGL(k_graph<true>, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
+  hipLaunchKernelGGL(k_graph, dim3(xcd_grid(nblk(g.n, 256))), dim3(256), 0, st, g);
+}
+
+// Edge features of every edge in CSR order (export for checks).
+__global__ void k_edge_export(uint32_t n, uint32_t stride, const uint32_t* __restrict__ deg,
+                              const uint32_t* __restrict__ col, const uint32_t* __restrict__ rowp,
+                              const float4* __restrict__ npos, float* __restrict__ e3) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = npos[i];
+  for (uint32_t k = 0; k < deg[i]; k++) {
+    float w, gx, gy;
+    edge_weights(pi, npos[col[(size_t)k * stride + i]], w, gx, gy);
+    const size_t e = rowp[i] + k;
+    e3[3 * e] = w;
+    e3[3 * e + 1] = gx;
+    e3[3 * e + 2] = gy;
+  }
 }
 
 // ---- learned-step kernels -------------------------------------------------
@@ -156,7 +138,7 @@ struct StepArgs {
   const SceneConst* sc;
   Fields8 y;                           // state, storage order
   const double* h;                     // storage order
-  const double *d1, *d2;               // derivatives by global id (8 per particle)
+  const double *d1, *d2;               // derivatives by sorted position (8 per particle)
   double* vprev;                       // by global id (2 per particle)
   const float* wsum;                   // by sorted position
   const float* norm;                   // kNorm floats (device), or null
@@ -174,17 +156,24 @@ struct StepArgs {
 
 __device__ inline double tau_of(const SceneConst& c) { return (double)kStepsPerLearned * c.dt; }
 
-// rho and S advanced by tau*D1, x and v unchanged (storage order).
+// Density over tau from its rate: rho exp(tau drho/rho), the exact solution
+// of the continuity equation for a constant velocity divergence; it stays
+// positive where the Euler step rho + tau drho would not.
+__device__ inline double advance_rho(double rho, double drho, double tau) { return rho * exp(tau * drho / rho); }
+
+// rho and S advanced over tau with D1, x and v unchanged (storage order).
 __global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const uint32_t i = a.perm[p];
   const SceneConst& c = a.sc[a.scene[i]];
   const double tau = tau_of(c);
-  const double* d = a.d1 + 8 * (size_t)a.gid[i];
+  const double* d = a.d1 + 8 * (size_t)p;
 #pragma unroll
   for (int k = 0; k < 4; k++) a.half.f[k][i] = a.y.f[k][i];
+  a.half.f[4][i] = advance_rho(a.y.f[4][i], d[4], tau);
 #pragma unroll
-  for (int k = 4; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
+  for (int k = 5; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
 }
 
 // One thread per sorted position: features, base update, optional network
@@ -200,8 +189,8 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   double st[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
-  const double* d1 = a.d1 + 8 * (size_t)g;
-  const double* d2 = a.d2 + 8 * (size_t)g;
+  const double* d1 = a.d1 + 8 * (size_t)p;
+  const double* d2 = a.d2 + 8 * (size_t)p;
   const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
   if (a.F || a.X0) {
     float f[kNF];
@@ -239,7 +228,7 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   nw[1] = st[1] + tau * d1[1] + 0.5 * tau * tau * d2[3];
   nw[2] = st[2] + tau * d2[2];
   nw[3] = st[3] + tau * d2[3];
-  nw[4] = st[4] + tau * d1[4];
+  nw[4] = advance_rho(st[4], d1[4], tau);
   nw[5] = st[5] + tau * d1[5];
   nw[6] = st[6] + tau * d1[6];
   nw[7] = st[7] + tau * d1[7];
@@ -564,7 +553,7 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   SCHK(s, salloc(s->callocs, &s->snap_sc, c->S));
   SCHK(s, salloc(s->callocs, &s->jsc, c->S));
   std::string msg;
-  if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, msg) != MFREE_OK)
+  if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, false, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
   s->bound = c;
   s->bN = c->N;
@@ -610,6 +599,7 @@ void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
   launch_ul_prepass(st, u);
   u.stage = 0;
   u.d8 = s->d1;
+  u.d8_sorted = 1;
   launch_ul_pairs(st, u);
   StepArgs a = step_args(s, c);
   hipLaunchKernelGGL(k_half_state, dim3(nblk(c->N, 256)), dim3(256), 0, st, a);
@@ -618,6 +608,7 @@ void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
   launch_ul_prepass(st, v);
   v.stage = 0;
   v.d8 = s->d2;
+  v.d8_sorted = 1;
   launch_ul_pairs(st, v);
 }
 
@@ -817,7 +808,7 @@ mfree_status build_windows(mfree_surrogate* s) {
     s->tchunk_cap = max_ch;
   }
   std::string msg;
-  if (mp_graph_reserve(s->graph, max_n, msg) != MFREE_OK || mp_acts_reserve(s->acts, max_n, msg) != MFREE_OK)
+  if (mp_graph_reserve(s->graph, max_n, msg) != MFREE_OK || mp_acts_reserve(s->acts, max_n, true, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
   s->wins_ready = true;
   return MFREE_OK;
@@ -1066,14 +1057,18 @@ mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* c, uint32_t 
   st = ensure_rows(s, s->rows + (uint64_t)ncapture * N);
   if (st != MFREE_OK) return st;
   for (uint32_t k = 0; k < ncapture; k++) {
-    if (gap) {
-      st = run_steps(c, gap);
+    // the velocity one learned step before the capture: recorded here at the
+    // first capture or after a gap, else before the previous capture's
+    // target steps
+    if (k == 0 || gap) {
+      if (gap && k > 0) {
+        st = run_steps(c, gap);
+        if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
+      }
+      record_vprev(s, c);
+      st = run_steps(c, kStepsPerLearned);
       if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
     }
-    // velocity one learned step before the capture
-    record_vprev(s, c);
-    st = run_steps(c, kStepsPerLearned);
-    if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
     // inputs and base update at the capture state
     const uint64_t row0 = s->rows;
     launch_fill_u32(c->stream, s->comp, 0u, c->S);
@@ -1091,6 +1086,7 @@ mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     hipLaunchKernelGGL(k_learned, dim3(nblk(N, 256)), dim3(256), 0, c->stream, a);
     st = sync_ctx(s, c);
     if (st != MFREE_OK) return st;
+    record_vprev(s, c);
     // truth: 4 RK4 steps
     st = run_steps(c, kStepsPerLearned);
     if (st != MFREE_OK) return sfail(s, st, std::string("collect: ") + c->err);
@@ -1143,7 +1139,7 @@ mfree_status mfree_surrogate_train(mfree_surrogate* s, uint32_t iters, float lr,
   if (e) {
     uint32_t z = 0;
     SCHK(s, hipMemcpy(s->terr, &z, sizeof(uint32_t), hipMemcpyHostToDevice));
-    if (e & EB_NBH) return sfail(s, MFREE_ERR_NBH_OVERFLOW, "training graph: more than 64 edges per node on average");
+    if (e & EB_NBH) return sfail(s, MFREE_ERR_NBH_OVERFLOW, "training graph: a node has more than 64 edges");
     return sfail(s, MFREE_ERR_CELLS, "training binning failed");
   }
   if (loss && iters) SCHK(s, hipMemcpy(loss, s->loss, iters * sizeof(float), hipMemcpyDeviceToHost));
@@ -1269,31 +1265,51 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
   st = sync_ctx(s, c);
   if (st != MFREE_OK) return st;
   const size_t N = c->N;
-  std::vector<uint32_t> rp(N + 1), perm(N), gid(N);
-  SCHK(s, hipMemcpy(rp.data(), s->graph.rowp, (N + 1) * 4, hipMemcpyDeviceToHost));
+  const size_t stride = s->graph.cap;  // >= N
+  std::vector<uint32_t> deg(N), perm(N), gid(N), colv(stride * kMaxDeg);
+  SCHK(s, hipMemcpy(deg.data(), s->graph.deg, N * 4, hipMemcpyDeviceToHost));
   SCHK(s, hipMemcpy(perm.data(), c->perm, N * 4, hipMemcpyDeviceToHost));
   SCHK(s, hipMemcpy(gid.data(), c->gid[b], N * 4, hipMemcpyDeviceToHost));
+  SCHK(s, hipMemcpy(colv.data(), s->graph.col, stride * kMaxDeg * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> rp(N + 1, 0);
+  for (size_t p = 0; p < N; p++) rp[p + 1] = rp[p] + deg[p];
   const uint64_t tot = rp[N];
   if (nedges) *nedges = tot;
   if (tot > cap) return sfail(s, MFREE_ERR_CAPACITY, "edge capacity too small");
-  std::vector<uint32_t> colv(tot);
-  std::vector<float4> ew(tot);
-  SCHK(s, hipMemcpy(colv.data(), s->graph.col, tot * 4, hipMemcpyDeviceToHost));
-  SCHK(s, hipMemcpy(ew.data(), s->graph.ew, tot * 16, hipMemcpyDeviceToHost));
+  std::vector<float> ew(3 * std::max<uint64_t>(tot, 1));
+  if (edge3 && tot) {
+    float* de = nullptr;
+    uint32_t* drp = nullptr;
+    SCHK(s, hipMalloc(&de, tot * 3 * sizeof(float)));
+    if (hipMalloc(&drp, (N + 1) * 4) != hipSuccess) {
+      (void)hipFree(de);
+      return sfail(s, MFREE_ERR_HIP, "hipMalloc failed");
+    }
+    hipError_t e = hipMemcpy(drp, rp.data(), (N + 1) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_edge_export, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, (uint32_t)stride,
+                         s->graph.deg, s->graph.col, drp, s->graph.npos, de);
+      e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpy(ew.data(), de, tot * 3 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(de);
+    (void)hipFree(drp);
+    if (e != hipSuccess) return sfail(s, MFREE_ERR_HIP, hipGetErrorString(e));
+  }
   // rows by global id, neighbours as global ids in sorted-position order
   std::vector<uint32_t> row_of(N);
   for (size_t p = 0; p < N; p++) row_of[gid[perm[p]]] = (uint32_t)p;
   row_ptr[0] = 0;
-  for (size_t g = 0; g < N; g++) row_ptr[g + 1] = row_ptr[g] + (rp[row_of[g] + 1] - rp[row_of[g]]);
+  for (size_t g = 0; g < N; g++) row_ptr[g + 1] = row_ptr[g] + deg[row_of[g]];
   for (size_t g = 0; g < N; g++) {
     const size_t p = row_of[g];
-    for (uint32_t e = rp[p]; e < rp[p + 1]; e++) {
-      const size_t k = row_ptr[g] + (e - rp[p]);
-      if (col) col[k] = gid[perm[colv[e]]];
+    for (uint32_t k = 0; k < deg[p]; k++) {
+      const size_t o = row_ptr[g] + k, e = rp[p] + k;
+      if (col) col[o] = gid[perm[colv[(size_t)k * stride + p]]];
       if (edge3) {
-        edge3[3 * k] = ew[e].x;
-        edge3[3 * k + 1] = ew[e].y;
-        edge3[3 * k + 2] = ew[e].z;
+        edge3[3 * o] = ew[3 * e];
+        edge3[3 * o + 1] = ew[3 * e + 1];
+        edge3[3 * o + 2] = ew[3 * e + 2];
       }
     }
   }
@@ -1301,12 +1317,12 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
 }
 
 mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
-                                         const float* edge3, const float* x, const float* t, const float* w,
+                                         const float* pos3, const float* x, const float* t, const float* w,
                                          float* y, float* loss, float* grad) {
-  if (!row_ptr || !col || !edge3 || !x || !t || !w || n == 0) return MFREE_ERR_ARG;
-  if (row_ptr[0] != 0 || (uint64_t)row_ptr[n] > (uint64_t)n * kEdgesPerNode) return MFREE_ERR_ARG;
+  if (!row_ptr || !col || !pos3 || !x || !t || !w || n == 0) return MFREE_ERR_ARG;
+  if (row_ptr[0] != 0) return MFREE_ERR_ARG;
   for (uint32_t i = 0; i < n; i++) {
-    if (row_ptr[i + 1] < row_ptr[i]) return MFREE_ERR_ARG;
+    if (row_ptr[i + 1] < row_ptr[i] || row_ptr[i + 1] - row_ptr[i] > kMaxDeg) return MFREE_ERR_ARG;
     for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; e++)
       if (col[e] >= n) return MFREE_ERR_ARG;
   }
@@ -1318,7 +1334,7 @@ mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint3
   MpGraph g;
   std::string msg;
   mfree_status st = mp_net_init(net, 0, msg);
-  if (st == MFREE_OK) st = mp_acts_reserve(acts, n, msg);
+  if (st == MFREE_OK) st = mp_acts_reserve(acts, n, true, msg);
   if (st == MFREE_OK) st = mp_graph_reserve(g, n, msg);
   float *dT = nullptr, *dL = nullptr;
   hipStream_t strm = nullptr;
@@ -1326,16 +1342,19 @@ mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint3
                          hipStreamCreateWithFlags(&strm, hipStreamNonBlocking) != hipSuccess))
     st = MFREE_ERR_HIP;
   if (st == MFREE_OK) {
-    const uint32_t E = row_ptr[n];
-    std::vector<float4> ew(std::max<uint32_t>(E, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float4> np(n);
+    std::vector<uint32_t> deg(n), cs((size_t)n * kMaxDeg, 0);
     std::vector<bf16_t> x16((size_t)n * kK0, 0);
-    for (uint32_t k = 0; k < E; k++) ew[k] = make_float4(edge3[3 * k], edge3[3 * k + 1], edge3[3 * k + 2], 0.f);
-    for (uint32_t i = 0; i < n; i++)
+    for (uint32_t i = 0; i < n; i++) {
+      np[i] = make_float4(pos3[3 * (size_t)i], pos3[3 * (size_t)i + 1], pos3[3 * (size_t)i + 2], 0.f);
+      deg[i] = row_ptr[i + 1] - row_ptr[i];
+      for (uint32_t k = 0; k < deg[i]; k++) cs[(size_t)k * n + i] = col[row_ptr[i] + k];
       for (int k = 0; k < kNF; k++) x16[(size_t)i * kK0 + k] = f2bf(x[(size_t)i * kNF + k]);
+    }
     const float zero = 0.f;
-    bool ok = hipMemcpy(g.rowp, row_ptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice) == hipSuccess &&
-              (E == 0 || hipMemcpy(g.col, col, (size_t)E * 4, hipMemcpyHostToDevice) == hipSuccess) &&
-              (E == 0 || hipMemcpy(g.ew, ew.data(), (size_t)E * 16, hipMemcpyHostToDevice) == hipSuccess) &&
+    bool ok = hipMemcpy(g.deg, deg.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(g.col, cs.data(), cs.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(g.npos, np.data(), (size_t)n * 16, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(acts.X0, x16.data(), x16.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(dT, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(dL, &zero, 4, hipMemcpyHostToDevice) == hipSuccess &&

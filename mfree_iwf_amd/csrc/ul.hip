@@ -164,12 +164,12 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
 }
 
 // Epilogue + RK4 stage update shared by both precisions.  d = derivatives.
-__device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t home, double* d,
-                                 const double* auxv) {
+__device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t i, uint32_t home,
+                                 double* d, const double* auxv) {
   if (!finite8(d)) atomicOr(a.err, EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t g = a.gid[home];
-    double* o = a.d8 + 8 * (size_t)g;
+    double* o = a.d8 + 8 * (size_t)(a.d8_sorted ? i : g);
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
     auxv[14] = 0.;
     auxv[15] = 0.;
   }
-  ul_finish(a, c, home, d, auxv);
+  ul_finish(a, c, i, home, d, auxv);
 }
 
 // Time advance of every body (src/rk4.h:107, simulation_time::increment_time).

@@ -195,7 +195,7 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
   if (!ok) atomicOr(a.err, EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t gid = a.gid[home];
-    double* o = a.d8 + 8 * (size_t)gid;
+    double* o = a.d8 + 8 * (size_t)(a.d8_sorted ? i : gid);
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {

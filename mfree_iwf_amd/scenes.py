@@ -221,7 +221,7 @@ def batch_specs(nscenes=256, seed=0, nmin=2000, nmax=60000):
     return out
 
 
-def subsonic_batch_specs(nscenes=256, seed=0, nmin=2000, nmax=60000, max_mach=0.9, nu=0.4, rho0=1010.0):
+def subsonic_batch_specs(nscenes=256, seed=0, nmin=2000, nmax=60000, max_mach=0.5, nu=0.4, rho0=1010.0):
     """The batched workload's generator restricted to impacts below max_mach
     (v0 / c0, c0 = sqrt(K / rho0)): the first nscenes such scenes of the
     seeded stream.  Faster impacts compress the rings far beyond an elastic

@@ -12,7 +12,7 @@ from . import _capi
 
 NF = 16       # node features
 NOUT = 8      # outputs: dx, dy, dvx, dvy, dSxx, dSxy, dSyy, drho (dimensionless)
-MAX_DEG = 64  # edges per node
+MAX_DEG = 64  # edge slots per node
 
 
 def _fp(a):
@@ -36,12 +36,13 @@ def layout():
     return [(int(sh[2 * k]), int(sh[2 * k + 1])) for k in range(nt)]
 
 
-def forward_backward(row_ptr, col, edge3, x, t, w, device=0):
-    """One forward + backward on the GPU: returns (y [n x 8], loss, grad)."""
+def forward_backward(row_ptr, col, pos3, x, t, w, device=0):
+    """One forward + backward on the GPU over nodes at pos3 = (x, y, h):
+    returns (y [n x 8], loss, grad)."""
     n = len(row_ptr) - 1
     row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
     col = np.ascontiguousarray(col, dtype=np.uint32)
-    edge3 = np.ascontiguousarray(edge3, dtype=np.float32)
+    pos3 = np.ascontiguousarray(pos3, dtype=np.float32)
     x = np.ascontiguousarray(x, dtype=np.float32)
     t = np.ascontiguousarray(t, dtype=np.float32)
     w = np.ascontiguousarray(w, dtype=np.float32)
@@ -50,8 +51,7 @@ def forward_backward(row_ptr, col, edge3, x, t, w, device=0):
     grad = np.zeros(param_count(), dtype=np.float32)
     if len(col) == 0:
         col = np.zeros(1, dtype=np.uint32)
-        edge3 = np.zeros((1, 3), dtype=np.float32)
-    st = _capi.lib().mfree_mpnn_forward_backward(device, n, _u32(row_ptr), _u32(col), _fp(edge3), _fp(x), _fp(t),
+    st = _capi.lib().mfree_mpnn_forward_backward(device, n, _u32(row_ptr), _u32(col), _fp(pos3), _fp(x), _fp(t),
                                                  _fp(w), _fp(y), _fp(loss), _fp(grad))
     _capi.check(st, what="mfree_mpnn_forward_backward")
     return y, float(loss[0]), grad

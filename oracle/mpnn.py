@@ -229,10 +229,16 @@ def features(state8, d1, d2, vprev, h, wsum, tau, c0, K, rho0):
     return f
 
 
+def advance_rho(rho, drho, tau):
+    """rho exp(tau drho / rho): continuity over tau at constant divergence."""
+    return rho * np.exp(tau * drho / rho)
+
+
 def half_state(state8, d1, tau):
-    """rho and S advanced by tau*D1; x, v unchanged."""
+    """rho and S advanced over tau with D1; x, v unchanged."""
     s = state8.copy()
-    s[:, 4:8] += tau * d1[:, 4:8]
+    s[:, 4] = advance_rho(state8[:, 4], d1[:, 4], tau)
+    s[:, 5:8] += tau * d1[:, 5:8]
     return s
 
 
@@ -241,7 +247,8 @@ def base_update(state8, d1, d2, tau):
     n = s.copy()
     n[:, 0:2] = s[:, 0:2] + tau * d1[:, 0:2] + 0.5 * tau * tau * d2[:, 2:4]
     n[:, 2:4] = s[:, 2:4] + tau * d2[:, 2:4]
-    n[:, 4:8] = s[:, 4:8] + tau * d1[:, 4:8]
+    n[:, 4] = advance_rho(s[:, 4], d1[:, 4], tau)
+    n[:, 5:8] = s[:, 5:8] + tau * d1[:, 5:8]
     return n
 
 

@@ -21,12 +21,14 @@ GRAD_TOL = 6e-2
 
 
 def lattice_graph(n, seed, hdx=1.3):
+    """Jittered lattice; positions rounded to float32 (what the GPU sees)."""
     rng = np.random.default_rng(seed)
     side = int(np.ceil(np.sqrt(n)))
     ij = np.array([(i, j) for i in range(side) for j in range(side)][:n], dtype=np.float64)
-    p = ij + 0.2 * rng.standard_normal(ij.shape)
+    p = (ij + 0.2 * rng.standard_normal(ij.shape)).astype(np.float32).astype(np.float64)
     h = np.full(n, hdx)
-    return om.radius_graph(p[:, 0], p[:, 1], h)
+    pos3 = np.stack([p[:, 0], p[:, 1], h], axis=1)
+    return pos3, om.radius_graph(p[:, 0], p[:, 1], h)
 
 
 def sorted_rows(row, col, e3):
@@ -46,10 +48,12 @@ def test_graph_matches_oracle():
         sv.add_body(ps, s.params)
         row, col, e3 = su.graph(sv)
     orow, ocol, oe3, _ = om.radius_graph(ps.x, ps.y, ps.h)
+    # the edge set is decided in FP64 (bit-exact); the weights are evaluated in
+    # FP32 from FP32 positions (a few 1e-6 relative to the weight scale)
     assert np.array_equal(row.astype(np.int64), orow)
     for (c, e), (oc, oe) in zip(sorted_rows(row, col, e3), sorted_rows(orow, ocol, oe3)):
         assert np.array_equal(c, oc)
-        assert np.allclose(e, oe, rtol=1e-5, atol=1e-6)
+        assert np.allclose(e, oe, rtol=1e-4, atol=2e-5)
 
 
 def test_graph_multibody_matches_oracle():
@@ -70,19 +74,19 @@ def test_graph_multibody_matches_oracle():
         got = sorted_rows(r, col[a:b].astype(np.int64) - off, e3[a:b])
         for (c, e), (oc, oe) in zip(got, sorted_rows(orow, ocol, oe3)):
             assert np.array_equal(c, oc)
-            assert np.allclose(e, oe, rtol=1e-5, atol=1e-6)
+            assert np.allclose(e, oe, rtol=1e-4, atol=2e-5)
         off += ps.n
 
 
 @pytest.mark.parametrize("n,seed", [(300, 0), (1000, 1)])
 def test_mpnn_forward_backward_matches_oracle(n, seed):
-    row, col, e3, _ = lattice_graph(n, seed)
+    pos3, (row, col, e3, _) = lattice_graph(n, seed)
     rng = np.random.default_rng(seed + 10)
     X = rng.standard_normal((n, om.NF))
     T = rng.standard_normal((n, om.NOUT))
     w = om.init_params(seed=seed + 20).astype(np.float32)
     print(f"[mpnn fwd/bwd] n={n} edges={len(col)}", flush=True)
-    y, loss, g = sg.forward_backward(row, col, e3, X, T, w)
+    y, loss, g = sg.forward_backward(row, col, pos3, X, T, w)
     # oracle on the same float32 weights (FP64 arithmetic)
     oy, oloss, og = om.loss_and_grad(row, col, e3, X, T, w.astype(np.float64))
     assert O.rel_l2(y, oy) < Y_TOL
