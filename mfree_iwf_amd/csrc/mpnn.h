@@ -107,18 +107,22 @@ struct MpNet {
 };
 
 // Activations of one forward pass over N nodes (kept for the backward).
+// Every array has whole rows of its own (no interleaving of differently
+// produced columns), so each kernel writes full cache lines.
 struct MpActs {
   uint32_t cap = 0;
   bool train = false;                   // training buffers allocated
   bf16_t* X0 = nullptr;                 // N x 32
-  float* Z[kLayers + 1] = {};           // pre-activations: Z0 (enc), Z1..Z3 (layers), N x 128 (training)
-  float* Hf[kLayers + 1] = {};          // hidden states FP32, N x 128 (training)
-  bf16_t* G[kLayers + 1] = {};          // [H, A] bf16, N x 224 (G[l] holds H_l and A of layer l+1)
+  bf16_t* H16[kLayers + 1] = {};        // hidden states, N x 128 bf16
+  bf16_t* A16[kLayers] = {};            // aggregated messages of layer l, N x 96 bf16
   bf16_t* U[kLayers] = {};              // N x 32 bf16
   float* Y = nullptr;                   // N x 16
-  // backward scratch (training)
+  // training
+  float* Z[kLayers + 1] = {};           // pre-activations: Z0 (enc), Z1..Z3 (layers), N x 128
+  float* Hf[kLayers + 1] = {};          // hidden states FP32, N x 128
   float* dH = nullptr;                  // N x 128
-  bf16_t* dZ16 = nullptr;               // N x 224 (bf16 dZ / dY / dU operands)
+  bf16_t* dZ16 = nullptr;               // N x 128 (bf16 dZ operand)
+  bf16_t* dU16 = nullptr;               // N x 32 (bf16 dY / dU operand)
   bf16_t* dA16 = nullptr;               // N x 96
   float* part = nullptr;                // dW partials
   size_t part_cap = 0;

@@ -54,11 +54,19 @@ struct GemmEpi {
   int ld16;
   float* zout;                  // pre-activation (EPI_SILU_RES, or null)
   const float* res;             // FP32 residual (ld kH), or
-  const bf16_t* res16;          // BF16 residual (ld kG), or neither
+  const bf16_t* res16;          // BF16 residual (ld kH), or neither
 };
 
-constexpr int kGemmBlocksMax = 512;  // 2 per CU
-constexpr int kGemmThreads = 512;    // 8 waves share the LDS weight tile
+// Launch shape: a weight tile above 32 KB is shared by 16 waves of one
+// workgroup per CU; smaller tiles use 4-wave workgroups, several per CU.
+template <int K, int M>
+constexpr int gemm_threads() {
+  return M * (K + 8) * 2 > 32 * 1024 ? 1024 : 256;
+}
+template <int K, int M>
+constexpr int gemm_blocks_max() {
+  return M * (K + 8) * 2 > 32 * 1024 ? 256 : 2048;
+}
 
 __device__ inline uint2 pack4_bf16(float a, float b, float c, float d) {
   return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
@@ -68,20 +76,27 @@ __device__ inline float4 unpack4_bf16(uint2 u) {
                      bf2f((bf16_t)(u.y >> 16)));
 }
 
-// out = X W^T over N rows; W is M x K (bf16, row-major, K % 32 == 0,
-// M % 16 == 0).  The MFMA computes the transposed tile D = W X^T: A = 16
-// weight rows, B = 16 node rows (both 8 contiguous bf16 per lane), so a lane
-// ends with one node and 4 consecutive output channels (vector epilogue).
-// Persistent: block b handles node tiles b, b + grid, ...
-template <int K, int M, int ROWS, int MODE>
-__global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict__ X, int ldx,
+// out = [X1, X2] W^T over N rows; W is M x K (bf16, row-major, K % 32 == 0,
+// M % 16 == 0); columns k < KS of the operand come from X1 (ld ldx), the
+// rest from X2 (ld ldx2).  The MFMA computes the transposed tile D = W X^T:
+// A = 16 weight rows, B = 16 node rows (both 8 contiguous bf16 per lane),
+// so a lane ends with one node and 4 consecutive output channels (vector
+// epilogue).  Persistent: block b handles node tiles b, b + grid, ...
+template <int K, int M, int ROWS, int MODE, int KS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_gemm(const bf16_t* __restrict__ X, int ldx,
+                                                       const bf16_t* __restrict__ X2, int ldx2,
                                                        const bf16_t* __restrict__ W, uint32_t N, GemmEpi e) {
   constexpr int KP = K + 8;  // padded LDS row
   constexpr int T = M / 16;
-  constexpr int kWaves = kGemmThreads / 64;
+  constexpr int kWaves = THREADS / 64;
   constexpr uint32_t kRowsPerBlock = kWaves * ROWS * 16;
   __shared__ __attribute__((aligned(16))) bf16_t wl[M * KP];
-  for (int i = threadIdx.x * 8; i < M * K; i += kGemmThreads * 8) {
+  // bf16 outputs leave through a per-wave LDS tile (16 rows x M) so that the
+  // stores are whole rows of 16-byte chunks
+  constexpr bool kStage = (MODE == EPI_BF16 || MODE == EPI_SILU_RES) && M <= 128;
+  constexpr int MP = M + 8;
+  __shared__ __attribute__((aligned(16))) bf16_t ostage[kStage ? kWaves * 16 * MP : 8];
+  for (int i = threadIdx.x * 8; i < M * K; i += THREADS * 8) {
     const int r = i / K, c = i - r * K;
     *(uint4*)&wl[r * KP + c] = *(const uint4*)&W[i];
   }
@@ -89,23 +104,30 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict_
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kq = (lane >> 4) * 8;
   const int mq = (lane >> 4) * 4;  // first of the lane's 4 output channels within a 16-wide tile
-  for (uint32_t tile = blockIdx.x; tile * kRowsPerBlock < N; tile += gridDim.x) {
+  // Operand rows of one tile; the next tile's are loaded before this tile's
+  // epilogue stores, so waiting for them does not wait for the stores.
+  bf16x8 xb[K / 32][ROWS];
+#define MP_LOAD_TILE(TILE)                                                                                   \
+  do {                                                                                                       \
+    const uint32_t nb_ = (TILE) * kRowsPerBlock + wave * ROWS * 16;                                          \
+    _Pragma("unroll") for (int r = 0; r < ROWS; r++) {                                                       \
+      const uint32_t row = nb_ + r * 16 + (lane & 15);                                                       \
+      _Pragma("unroll") for (int kk = 0; kk < K / 32; kk++) {                                                \
+        const bf16_t* src = kk * 32 < KS ? X + (size_t)row * ldx + kk * 32 + kq                              \
+                                         : X2 + (size_t)row * ldx2 + (kk * 32 - KS) + kq;                    \
+        xb[kk][r] = row < N ? *(const bf16x8*)src : bf16x8{};                                                \
+      }                                                                                                      \
+    }                                                                                                        \
+  } while (0)
+  uint32_t tile = blockIdx.x;
+  MP_LOAD_TILE(tile);
+  for (; tile * kRowsPerBlock < N; tile += gridDim.x) {
     const uint32_t nb = tile * kRowsPerBlock + wave * ROWS * 16;
-    if (nb >= N) continue;
     f32x4 acc[ROWS][T];
 #pragma unroll
     for (int r = 0; r < ROWS; r++)
 #pragma unroll
       for (int t = 0; t < T; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // all node-row fragments of the tile in flight before the first MFMA
-    bf16x8 xb[K / 32][ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; r++) {
-      const uint32_t row = nb + r * 16 + (lane & 15);
-#pragma unroll
-      for (int kk = 0; kk < K / 32; kk++)
-        xb[kk][r] = row < N ? *(const bf16x8*)(X + (size_t)row * ldx + kk * 32 + kq) : bf16x8{};
-    }
 #pragma unroll
     for (int kk = 0; kk < K / 32; kk++) {
 #pragma unroll
@@ -115,11 +137,26 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict_
         for (int r = 0; r < ROWS; r++)
           acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb[kk][r], acc[r][t], 0, 0, 0);
       }
+      // keep the weight fragments of one k-step live at a time
+      __builtin_amdgcn_sched_barrier(0);
     }
+    // residual rows of this tile, then the next tile's operands
+    uint2 rr16[ROWS][T];
+    if constexpr (MODE == EPI_SILU_RES) {
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        const uint32_t n = nb + r * 16 + (lane & 15);
+#pragma unroll
+        for (int t = 0; t < T; t++)
+          rr16[r][t] = (e.res16 && n < N) ? *(const uint2*)&e.res16[(size_t)n * kH + t * 16 + mq] : make_uint2(0u, 0u);
+      }
+    }
+    MP_LOAD_TILE(tile + gridDim.x);
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
       const uint32_t n = nb + r * 16 + (lane & 15);
-      if (n >= N) continue;
+      bf16_t* stg = ostage + wave * 16 * MP;
+      if (n < N) {
 #pragma unroll
       for (int t = 0; t < T; t++) {
         const int m = t * 16 + mq;
@@ -137,7 +174,10 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict_
           const float4 y = *o;
           *o = make_float4(y.x + x.x, y.y + x.y, y.z + x.z, y.w + x.w);
         } else if constexpr (MODE == EPI_BF16) {
-          *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(x.x, x.y, x.z, x.w);
+          if constexpr (kStage)
+            *(uint2*)&stg[(lane & 15) * MP + m] = pack4_bf16(x.x, x.y, x.z, x.w);
+          else
+            *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(x.x, x.y, x.z, x.w);
         } else if constexpr (MODE == EPI_SPLIT) {
           if (m < kH) {
             float4* o = (float4*)&e.out[(size_t)n * kH + m];
@@ -153,15 +193,35 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16_t* __restrict_
             const float4 rr = *(const float4*)&e.res[(size_t)n * kH + m];
             h = make_float4(h.x + rr.x, h.y + rr.y, h.z + rr.z, h.w + rr.w);
           } else if (e.res16) {
-            const float4 rr = unpack4_bf16(*(const uint2*)&e.res16[(size_t)n * kG + m]);
+            const float4 rr = unpack4_bf16(rr16[r][t]);
             h = make_float4(h.x + rr.x, h.y + rr.y, h.z + rr.z, h.w + rr.w);
           }
           if (e.out) *(float4*)&e.out[(size_t)n * kH + m] = h;
-          *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(h.x, h.y, h.z, h.w);
+          if constexpr (kStage)
+            *(uint2*)&stg[(lane & 15) * MP + m] = pack4_bf16(h.x, h.y, h.z, h.w);
+          else
+            *(uint2*)&e.out16[(size_t)n * e.ld16 + m] = pack4_bf16(h.x, h.y, h.z, h.w);
         }
+      }
+      }
+      if constexpr (kStage) {
+        // the wave's 16 staged rows -> global, 16-byte chunks, rows contiguous
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+        constexpr int CH = M / 8;
+        const uint32_t n0 = nb + r * 16;
+#pragma unroll
+        for (int c = lane; c < 16 * CH; c += 64) {
+          const int row = c / CH, ch = c - row * CH;
+          if (n0 + row < N)
+            *(uint4*)&e.out16[(size_t)(n0 + row) * e.ld16 + ch * 8] = *(const uint4*)&stg[row * MP + ch * 8];
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
       }
     }
   }
+#undef MP_LOAD_TILE
 }
 
 // Weight gradient partials: part[chunk][m][k] = sum_{n in chunk} dY[n][m] X[n][k]
@@ -227,25 +287,16 @@ __global__ __launch_bounds__(256) void k_gemm_dw(const bf16_t* __restrict__ dY, 
   }
 }
 
-// g[m][k] += sum over chunks of part[chunk][m][k]
-__global__ void k_reduce_parts(const float* __restrict__ part, uint32_t nchunk, int MK, float* __restrict__ g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= MK) return;
-  float s = 0.f;
-  for (uint32_t c = 0; c < nchunk; c++) s += part[(size_t)c * MK + i];
-  g[i] += s;
-}
-
-// g[r][k] += sum_chunks part[chunk][r][k] for r < rows, k < cols of a
-// partial laid out [chunk][Mp][Kp] (the padded GEMM shape).
+// g[r * gld + coff + k] += sum_chunks part[chunk][r][k] for r < rows, k < cols
+// of a partial laid out [chunk][Mp][Kp] (the padded GEMM shape).
 __global__ void k_reduce_rect(const float* __restrict__ part, uint32_t nchunk, int Mp, int Kp, int rows, int cols,
-                              float* __restrict__ g) {
+                              float* __restrict__ g, int gld, int coff) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows * cols) return;
   const int r = i / cols, k = i % cols;
   float s = 0.f;
   for (uint32_t c = 0; c < nchunk; c++) s += part[((size_t)c * Mp + r) * Kp + k];
-  g[i] += s;
+  g[(size_t)r * gld + coff + k] += s;
 }
 
 // bias gradient: g[m] += sum_n dY[n][m] (bf16 operand, M <= 256 columns)
@@ -276,10 +327,10 @@ __device__ inline uint4 pack8_bf16(const float* f) {
 
 // Aggregation forward, one node per lane (consecutive nodes on consecutive
 // lanes, so the slot-major edge lists load coalesced):
-//   G[i][128+c] = sum_e w U[j][c], G[i][160+c] = sum_e gx U[j][c], G[i][192+c] = sum_e gy U[j][c]
+//   A[i][c] = sum_e w U[j][c], A[i][32+c] = sum_e gx U[j][c], A[i][64+c] = sum_e gy U[j][c]
 __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, const uint32_t* __restrict__ deg,
                                                  const uint32_t* __restrict__ col, const float4* __restrict__ npos,
-                                                 const bf16_t* __restrict__ U, bf16_t* __restrict__ G) {
+                                                 const bf16_t* __restrict__ U, bf16_t* __restrict__ A) {
   const uint32_t i = xcd_tile() * 256u + threadIdx.x;
   if (i >= n) return;
   const float4 pi = npos[i];
@@ -308,7 +359,7 @@ __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, co
       }
     }
   }
-  uint4* g = (uint4*)(G + (size_t)i * kG + kH);
+  uint4* g = (uint4*)(A + (size_t)i * (3 * kC));
 #pragma unroll
   for (int q = 0; q < kC / 8; q++) {
     g[q] = pack8_bf16(a0 + 8 * q);
@@ -416,27 +467,39 @@ __global__ void k_sync16(const float* __restrict__ w, int rows, int cols, int co
 inline uint32_t blocks(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
 
 // Rows per wave: accumulators ROWS x M/16 x 4 registers stay within budget.
-template <int K, int M, int ROWS = (M > 128 ? 1 : (M > 32 ? 2 : 4))>
-void gemm(const bf16_t* X, int ldx, const bf16_t* W, uint32_t N, const GemmEpi& e, hipStream_t st) {
-  const uint32_t tiles = blocks(N, (kGemmThreads / 64) * ROWS * 16);
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles, kGemmBlocksMax));
+template <int K, int M, int KS = K, int ROWS = (M > 32 ? 1 : 4)>
+void gemm2(const bf16_t* X, int ldx, const bf16_t* X2, int ldx2, const bf16_t* W, uint32_t N, const GemmEpi& e,
+           hipStream_t st) {
+  constexpr int TH = gemm_threads<K, M>();
+  const uint32_t tiles = blocks(N, (TH / 64) * ROWS * 16);
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles, gemm_blocks_max<K, M>()));
   switch (e.mode) {
     case EPI_F32:
-      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2, W, N,
+                         e);
       break;
     case EPI_F32_ACC:
-      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32_ACC>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_F32_ACC, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2, W,
+                         N, e);
       break;
     case EPI_BF16:
-      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_BF16>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_BF16, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2, W, N,
+                         e);
       break;
     case EPI_SPLIT:
-      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SPLIT>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SPLIT, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2, W,
+                         N, e);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SILU_RES>), dim3(grid), dim3(kGemmThreads), 0, st, X, ldx, W, N, e);
+      hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SILU_RES, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2,
+                         W, N, e);
       break;
   }
+}
+
+template <int K, int M>
+void gemm(const bf16_t* X, int ldx, const bf16_t* W, uint32_t N, const GemmEpi& e, hipStream_t st) {
+  gemm2<K, M, K>(X, ldx, X, ldx, W, N, e, st);
 }
 
 GemmEpi epi(int mode) {
@@ -446,19 +509,21 @@ GemmEpi epi(int mode) {
   return e;
 }
 
-// g[M x K] += dY^T X over all N nodes
+// g[r * gld + coff + k] += (dY^T X)[r][k] over all N nodes, r < rows (of the
+// M-row padded product), k < K
 template <int K>
-void gemm_dw(const bf16_t* dY, int ldy, const bf16_t* X, int ldx, uint32_t N, int M, float* part, float* g,
-             hipStream_t st) {
+void gemm_dw(const bf16_t* dY, int ldy, const bf16_t* X, int ldx, uint32_t N, int M, int rows, float* part, float* g,
+             int gld, int coff, hipStream_t st) {
   const uint32_t nch = blocks(N, kDwChunk);
   hipLaunchKernelGGL((k_gemm_dw<K>), dim3(M / 16, nch), dim3(256), 0, st, dY, ldy, X, ldx, N, M, part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks((size_t)M * K, 256)), dim3(256), 0, st, part, nch, M * K, g);
+  hipLaunchKernelGGL(k_reduce_rect, dim3(blocks((size_t)rows * K, 256)), dim3(256), 0, st, part, nch, M, K, rows, K,
+                     g, gld, coff);
 }
 
 void bias_grad(const bf16_t* dY, int ldy, uint32_t N, int M, float* part, float* g, hipStream_t st) {
   const uint32_t nch = blocks(N, kDwChunk);
   hipLaunchKernelGGL(k_colsum, dim3(nch), dim3(256), 0, st, dY, ldy, N, M, part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(M, 256)), dim3(256), 0, st, part, nch, M, g);
+  hipLaunchKernelGGL(k_reduce_rect, dim3(blocks(M, 256)), dim3(256), 0, st, part, nch, 1, M, 1, M, g, M, 0);
 }
 
 }  // namespace
@@ -567,13 +632,13 @@ mfree_status mp_acts_reserve(MpActs& a, uint32_t n, bool train, std::string& msg
   };
   const size_t N = n;
   bool ok = al((void**)&a.X0, N * kK0 * 2) && al((void**)&a.Y, N * kOutP * 4);
-  for (int l = 0; l <= kLayers && ok; l++) ok = al((void**)&a.G[l], N * kG * 2);
-  for (int l = 0; l < kLayers && ok; l++) ok = al((void**)&a.U[l], N * kC * 2);
+  for (int l = 0; l <= kLayers && ok; l++) ok = al((void**)&a.H16[l], N * kH * 2);
+  for (int l = 0; l < kLayers && ok; l++) ok = al((void**)&a.A16[l], N * 3 * kC * 2) && al((void**)&a.U[l], N * kC * 2);
   if (train) {
-    ok = ok && al((void**)&a.dH, N * kH * 4) && al((void**)&a.dZ16, N * kG * 2) &&
+    ok = ok && al((void**)&a.dH, N * kH * 4) && al((void**)&a.dZ16, N * kH * 2) && al((void**)&a.dU16, N * kC * 2) &&
          al((void**)&a.dA16, N * 3 * kC * 2);
     for (int l = 0; l <= kLayers && ok; l++) ok = al((void**)&a.Z[l], N * kH * 4) && al((void**)&a.Hf[l], N * kH * 4);
-    a.part_cap = (size_t)blocks(N, kDwChunk) * kH * kG;
+    a.part_cap = (size_t)blocks(N, kDwChunk) * kH * kH;
     ok = ok && al((void**)&a.part, a.part_cap * 4);
   }
   if (!ok) {
@@ -593,7 +658,7 @@ void mp_acts_free(MpActs& a) {
   a.train = false;
   for (int l = 0; l <= kLayers; l++) a.Z[l] = a.Hf[l] = nullptr;
   a.dH = nullptr;
-  a.dZ16 = a.dA16 = nullptr;
+  a.dZ16 = a.dU16 = a.dA16 = nullptr;
   a.part = nullptr;
 }
 
@@ -604,36 +669,36 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
   e.nbias = kH;
   e.zout = train ? a.Z[0] : nullptr;
   e.out = train ? a.Hf[0] : nullptr;
-  e.out16 = a.G[0];
-  e.ld16 = kG;
+  e.out16 = a.H16[0];
+  e.ld16 = kH;
   gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
   for (int l = 0; l < kLayers; l++) {
     // U = H Wu^T
     GemmEpi eu = epi(EPI_BF16);
     eu.out16 = a.U[l];
     eu.ld16 = kC;
-    gemm<kH, kC>(a.G[l], kG, net.t[net.Wu[l]].w16, n, eu, st);
-    // A = aggregate(U) into G[l][:, 128:224]
+    gemm<kH, kC>(a.H16[l], kH, net.t[net.Wu[l]].w16, n, eu, st);
+    // A = aggregate(U)
     hipLaunchKernelGGL(k_agg_fwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,
-                       a.U[l], a.G[l]);
+                       a.U[l], a.A16[l]);
     // H_{l+1} = H_l + SiLU([H, A] Wo^T + bo)
     GemmEpi eo = epi(EPI_SILU_RES);
     eo.bias = net.w + net.t[net.bo[l]].off;
     eo.nbias = kH;
     eo.zout = train ? a.Z[l + 1] : nullptr;
     eo.res = train ? a.Hf[l] : nullptr;
-    eo.res16 = train ? nullptr : a.G[l];
+    eo.res16 = train ? nullptr : a.H16[l];
     eo.out = train ? a.Hf[l + 1] : nullptr;
-    eo.out16 = a.G[l + 1];
-    eo.ld16 = kG;
-    gemm<kG, kH>(a.G[l], kG, net.t[net.Wo[l]].w16, n, eo, st);
+    eo.out16 = a.H16[l + 1];
+    eo.ld16 = kH;
+    gemm2<kG, kH, kH>(a.H16[l], kH, a.A16[l], 3 * kC, net.t[net.Wo[l]].w16, n, eo, st);
   }
   GemmEpi ed = epi(EPI_F32);
   ed.bias = net.w + net.t[net.bd].off;
   ed.nbias = kOut;
   ed.out = a.Y;
   ed.ldo = kOutP;
-  gemm<kH, kOutP>(a.G[kLayers], kG, net.t[net.Wd].w16, n, ed, st);
+  gemm<kH, kOutP>(a.H16[kLayers], kH, net.t[net.Wd].w16, n, ed, st);
 }
 
 mfree_status mp_graph_reserve(MpGraph& g, uint32_t n, std::string& msg) {
@@ -667,46 +732,46 @@ void mp_zero_grad(MpNet& net, hipStream_t st) {
 
 void mp_backward(MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, const float* T, float* loss_out, hipStream_t st) {
   // loss and dY (bf16, ld 32, zero-padded)
-  hipLaunchKernelGGL(k_mse, dim3(blocks(n, 256)), dim3(256), 0, st, n, a.Y, T, a.dZ16, loss_out);
+  hipLaunchKernelGGL(k_mse, dim3(blocks(n, 256)), dim3(256), 0, st, n, a.Y, T, a.dU16, loss_out);
   // decoder: dWd = dY^T H3 (rows 0..kOut-1 of the padded 16), dbd = sum dY, dH3 = dY Wd
-  const uint32_t nch = blocks(n, kDwChunk);
-  hipLaunchKernelGGL((k_gemm_dw<kH>), dim3(1, nch), dim3(256), 0, st, a.dZ16, 32, a.G[kLayers], kG, n, 16, a.part);
-  hipLaunchKernelGGL(k_reduce_rect, dim3(blocks((size_t)kOut * kH, 256)), dim3(256), 0, st, a.part, nch, 16, kH, kOut,
-                     kH, net.g + net.t[net.Wd].off);
-  bias_grad(a.dZ16, 32, n, kOut, a.part, net.g + net.t[net.bd].off, st);
+  gemm_dw<kH>(a.dU16, kC, a.H16[kLayers], kH, n, 16, kOut, a.part, net.g + net.t[net.Wd].off, kH, 0, st);
+  bias_grad(a.dU16, kC, n, kOut, a.part, net.g + net.t[net.bd].off, st);
   GemmEpi e = epi(EPI_F32);
   e.out = a.dH;
   e.ldo = kH;
-  gemm<32, kH>(a.dZ16, 32, net.t[net.Wd].wt16, n, e, st);
+  gemm<kC, kH>(a.dU16, kC, net.t[net.Wd].wt16, n, e, st);
   for (int l = kLayers - 1; l >= 0; l--) {
     // dZ = dH * silu'(Z_{l+1}); the residual keeps dH for H_l
     hipLaunchKernelGGL(k_silu_bwd, dim3(blocks((size_t)n * kH, 256)), dim3(256), 0, st, n * kH, a.dH, a.Z[l + 1],
-                       a.dZ16, kG);
-    // dWo = dZ^T G_l, dbo = sum dZ
-    gemm_dw<kG>(a.dZ16, kG, a.G[l], kG, n, kH, a.part, net.g + net.t[net.Wo[l]].off, st);
-    bias_grad(a.dZ16, kG, n, kH, a.part, net.g + net.t[net.bo[l]].off, st);
+                       a.dZ16, kH);
+    // dWo = dZ^T [H_l, A_l], dbo = sum dZ
+    float* gWo = net.g + net.t[net.Wo[l]].off;
+    gemm_dw<kH>(a.dZ16, kH, a.H16[l], kH, n, kH, kH, a.part, gWo, kG, 0, st);
+    gemm_dw<3 * kC>(a.dZ16, kH, a.A16[l], 3 * kC, n, kH, kH, a.part, gWo, kG, kH, st);
+    bias_grad(a.dZ16, kH, n, kH, a.part, net.g + net.t[net.bo[l]].off, st);
     // [dH_l += dG[:, :128], dA = dG[:, 128:224]] with dG = dZ Wo
     GemmEpi eg = epi(EPI_SPLIT);
     eg.out = a.dH;
     eg.out16 = a.dA16;
-    gemm<kH, kG>(a.dZ16, kG, net.t[net.Wo[l]].wt16, n, eg, st);
-    // dU = aggregate^T(dA) -> bf16 operand (dZ16 columns 0..31)
+    gemm<kH, kG>(a.dZ16, kH, net.t[net.Wo[l]].wt16, n, eg, st);
+    // dU = aggregate^T(dA)
     hipLaunchKernelGGL(k_agg_bwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,
-                       a.dA16, a.dZ16, kG);
+                       a.dA16, a.dU16, kC);
     // dWu = dU^T H_l, dH_l += dU Wu
-    gemm_dw<kH>(a.dZ16, kG, a.G[l], kG, n, kC, a.part, net.g + net.t[net.Wu[l]].off, st);
+    gemm_dw<kH>(a.dU16, kC, a.H16[l], kH, n, kC, kC, a.part, net.g + net.t[net.Wu[l]].off, kH, 0, st);
     GemmEpi eu = epi(EPI_F32_ACC);
     eu.out = a.dH;
     eu.ldo = kH;
-    gemm<kC, kH>(a.dZ16, kG, net.t[net.Wu[l]].wt16, n, eu, st);
+    gemm<kC, kH>(a.dU16, kC, net.t[net.Wu[l]].wt16, n, eu, st);
   }
   // encoder: dZ0 = dH0 * silu'(Z0); dWe = dZ0^T X0 (first kNF columns), dbe
   hipLaunchKernelGGL(k_silu_bwd, dim3(blocks((size_t)n * kH, 256)), dim3(256), 0, st, n * kH, a.dH, a.Z[0], a.dZ16,
-                     kG);
-  hipLaunchKernelGGL((k_gemm_dw<kK0>), dim3(kH / 16, nch), dim3(256), 0, st, a.dZ16, kG, a.X0, kK0, n, kH, a.part);
+                     kH);
+  const uint32_t nch = blocks(n, kDwChunk);
+  hipLaunchKernelGGL((k_gemm_dw<kK0>), dim3(kH / 16, nch), dim3(256), 0, st, a.dZ16, kH, a.X0, kK0, n, kH, a.part);
   hipLaunchKernelGGL(k_reduce_rect, dim3(blocks((size_t)kH * kNF, 256)), dim3(256), 0, st, a.part, nch, kH, kK0, kH,
-                     kNF, net.g + net.t[net.We].off);
-  bias_grad(a.dZ16, kG, n, kH, a.part, net.g + net.t[net.be].off, st);
+                     kNF, net.g + net.t[net.We].off, kNF, 0);
+  bias_grad(a.dZ16, kH, n, kH, a.part, net.g + net.t[net.be].off, st);
 }
 
 void mp_adam(MpNet& net, float lr, float beta1, float beta2, float eps, hipStream_t st) {
