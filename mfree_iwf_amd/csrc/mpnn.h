@@ -137,6 +137,7 @@ struct MpGraph {
   uint32_t* deg = nullptr;    // cap
   uint32_t* col = nullptr;    // kMaxDeg * cap (sorted positions)
   float4* npos = nullptr;     // cap: (x, y, h, 0) FP32 by sorted position
+  double4* dpos = nullptr;    // cap: (x, y, h, 0) FP64 by sorted position (graph build)
   float* wsum = nullptr;      // cap: sum of w over the edges and the node itself
   std::vector<void*> allocs;
 };

@@ -711,7 +711,7 @@ mfree_status mp_graph_reserve(MpGraph& g, uint32_t n, std::string& msg) {
   };
   const size_t N = n;
   if (!al((void**)&g.deg, N * 4) || !al((void**)&g.col, N * kMaxDeg * 4) || !al((void**)&g.npos, N * 16) ||
-      !al((void**)&g.wsum, N * 4)) {
+      !al((void**)&g.dpos, N * 32) || !al((void**)&g.wsum, N * 4)) {
     mp_graph_free(g);
     msg = "hipMalloc failed (graph)";
     return MFREE_ERR_HIP;

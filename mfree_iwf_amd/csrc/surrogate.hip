@@ -56,14 +56,26 @@ struct GraphArgs {
   uint32_t* deg;
   uint32_t* col;
   float4* npos;
+  double4* dpos;
   float* wsum;
   uint32_t* err;
 };
 
+// Node records in sorted order: FP64 for the edge predicate (contiguous
+// candidate runs), FP32 for the network's edge features.
+__global__ __launch_bounds__(256) void k_graph_nodes(GraphArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const uint32_t s = a.perm[p];
+  const double x = a.x[s], y = a.y[s], h = a.h[s];
+  a.dpos[p] = make_double4(x, y, h, 0.0);
+  a.npos[p] = make_float4((float)x, (float)y, (float)h, 0.f);
+}
+
 // One thread per sorted position p: edges to every other particle of the
 // 3x3 cell block with r < 2 h_ij (h_ij = (h_i + h_j)/2), decided in FP64 from
 // symmetric differences, so the graph is symmetric.  Writes the slot-major
-// edge list, the degree, the kernel sum and the FP32 node record.
+// edge list, the degree and the kernel sum.
 __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   const uint32_t p = xcd_tile() * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
@@ -73,7 +85,8 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   const uint32_t co = a.sc[b].cell_offset;
   const uint32_t lk = a.key[s] - co;
   const int ix = (int)(lk / g.ny), iy = (int)(lk % g.ny);
-  const double xi = a.x[s], yi = a.y[s], hi = a.h[s];
+  const double4 pi = a.dpos[p];
+  const double xi = pi.x, yi = pi.y, hi = pi.z;
   uint32_t d = 0;
   float ws = kSplineNorm;  // the node itself (q = 0)
   const int cx0 = max(ix - 1, 0), cx1 = min(ix + 1, (int)g.nx - 1);
@@ -83,10 +96,10 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
     const uint32_t q0 = a.cell_start[c0 + cy0], q1 = a.cell_start[c0 + cy1 + 1];
     for (uint32_t q = q0; q < q1; q++) {
       if (q == p) continue;
-      const uint32_t j = a.perm[q];
-      const double dx = xi - a.x[j], dy = yi - a.y[j];
+      const double4 pj = a.dpos[q];
+      const double dx = xi - pj.x, dy = yi - pj.y;
       const double r2 = dx * dx + dy * dy;
-      const double hij = 0.5 * (hi + a.h[j]);
+      const double hij = 0.5 * (hi + pj.z);
       if (!(r2 < 4.0 * hij * hij)) continue;
       float f, df;
       spline_shape((float)(sqrt(r2) / hij), f, df);
@@ -101,7 +114,6 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   }
   a.deg[p] = d;
   a.wsum[p] = ws;
-  a.npos[p] = make_float4((float)xi, (float)yi, (float)hi, 0.f);
 }
 
 void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
@@ -109,7 +121,9 @@ void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
   g.deg = mg.deg;
   g.col = mg.col;
   g.npos = mg.npos;
+  g.dpos = mg.dpos;
   g.wsum = mg.wsum;
+  hipLaunchKernelGGL(k_graph_nodes, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
   hipLaunchKernelGGL(k_graph, dim3(xcd_grid(nblk(g.n, 256))), dim3(256), 0, st, g);
 }
 
@@ -495,6 +509,7 @@ struct mfree_surrogate {
   double* jsnap = nullptr;  // 8 x N: state before the judge step
   SceneConst* snap_sc = nullptr;
   SceneConst* jsc = nullptr;
+  hipGraphExec_t lexec[2] = {nullptr, nullptr};  // captured learned steps per storage buffer
 };
 
 namespace mfree {
@@ -527,6 +542,14 @@ void free_list(std::vector<void*>& l) {
   l.clear();
 }
 
+void drop_graphs(mfree_surrogate* s) {
+  for (int b = 0; b < 2; b++)
+    if (s->lexec[b]) {
+      (void)hipGraphExecDestroy(s->lexec[b]);
+      s->lexec[b] = nullptr;
+    }
+}
+
 // Per-context buffers (derivatives, base update, previous velocities,
 // snapshots); reallocated when the surrogate meets another context.
 mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
@@ -537,6 +560,7 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   if (c->opt.device != s->opt.device) return sfail(s, MFREE_ERR_ARG, "context and surrogate use different devices");
   if (s->bound == c && s->bN == c->N) return MFREE_OK;
   SCHK(s, hipDeviceSynchronize());
+  drop_graphs(s);
   free_list(s->callocs);
   const size_t N = c->N;
   SCHK(s, salloc(s->callocs, &s->d1, 8 * N));
@@ -630,15 +654,18 @@ void enqueue_graph(mfree_surrogate* s, mfree_ctx* c) {
   launch_graph(c->stream, g, s->graph);
 }
 
-void advance_time(mfree_ctx* c, uint32_t nsteps) {
+void advance_time_device(mfree_ctx* c, uint32_t nsteps) {
   for (uint32_t k = 0; k < nsteps; k++)
     hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc);
+}
+
+void advance_time_host(mfree_ctx* c, uint32_t nsteps) {
   for (uint32_t b = 0; b < c->S; b++)
     for (uint32_t k = 0; k < nsteps; k++) c->time[b] = c->time[b] + c->prm[b].dt;
 }
 
-// One learned step of every body (no synchronisation).
-void enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
+// Device work of one learned step of every body (capturable: kernels only).
+void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
   enqueue_derivs(s, c);
   StepArgs a = step_args(s, c);
   if (use_net) {
@@ -655,7 +682,33 @@ void enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
   a.apply = 1;
   a.save_vprev = 1;
   hipLaunchKernelGGL(k_learned, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, a);
-  advance_time(c, kStepsPerLearned);
+  advance_time_device(c, kStepsPerLearned);
+}
+
+// One learned step of every body (no synchronisation).  With graphs enabled
+// on the context, the network step is replayed from a graph captured once
+// per storage buffer (the step never reorders storage).
+mfree_status enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
+  const int b = c->cur;
+  if (use_net && c->opt.use_graph) {
+    if (!s->lexec[b]) {
+      hipGraph_t g;
+      SCHK(s, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_learned_device(s, c, true);
+      SCHK(s, hipStreamEndCapture(c->stream, &g));
+      hipError_t e = hipGraphInstantiate(&s->lexec[b], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        s->lexec[b] = nullptr;
+        return sfail(s, MFREE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    SCHK(s, hipGraphLaunch(s->lexec[b], c->stream));
+  } else {
+    enqueue_learned_device(s, c, use_net);
+  }
+  advance_time_host(c, kStepsPerLearned);
+  return MFREE_OK;
 }
 
 mfree_status ensure_rows(mfree_surrogate* s, uint64_t need) {
@@ -808,6 +861,7 @@ mfree_status build_windows(mfree_surrogate* s) {
     s->tchunk_cap = max_ch;
   }
   std::string msg;
+  drop_graphs(s);  // the reservations below may move the buffers a captured step uses
   if (mp_graph_reserve(s->graph, max_n, msg) != MFREE_OK || mp_acts_reserve(s->acts, max_n, true, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
   s->wins_ready = true;
@@ -978,6 +1032,7 @@ void mfree_surrogate_destroy(mfree_surrogate* s) {
   if (!s) return;
   (void)hipSetDevice(s->opt.device);
   (void)hipDeviceSynchronize();
+  drop_graphs(s);
   mp_net_free(s->net);
   mp_acts_free(s->acts);
   mp_graph_free(s->graph);
@@ -1153,7 +1208,8 @@ mfree_status mfree_surrogate_step(mfree_surrogate* s, mfree_ctx* c, int32_t use_
   if (st != MFREE_OK) return st;
   if (use_net && !s->have_norm) return sfail(s, MFREE_ERR_ARG, "network has no normalisation (collect or set_params)");
   if (!s->vprev_valid) record_vprev(s, c);
-  enqueue_learned(s, c, use_net != 0);
+  st = enqueue_learned(s, c, use_net != 0);
+  if (st != MFREE_OK) return st;
   return sync_ctx(s, c);
 }
 
@@ -1195,7 +1251,10 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
   while (done + span <= nsteps) {
     auto t0 = std::chrono::steady_clock::now();
     snapshot(s, c, times);
-    for (uint32_t k = 0; k + 1 < group; k++) enqueue_learned(s, c, true);
+    for (uint32_t k = 0; k + 1 < group; k++) {
+      st = enqueue_learned(s, c, true);
+      if (st != MFREE_OK) return st;
+    }
     st = sync_ctx(s, c);
     if (st != MFREE_OK) return st;
     js.ms_learned += ms_since(t0);
@@ -1205,7 +1264,8 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     for (int k = 0; k < 8; k++) copy_async(s->jsnap + k * N, c->y[b].f[k], N * 8, c->stream);
     copy_async(s->jsc, c->sc, c->S * sizeof(SceneConst), c->stream);
     const std::vector<double> jtimes = c->time;
-    enqueue_learned(s, c, true);  // also records the velocities of this state
+    st = enqueue_learned(s, c, true);  // also records the velocities of this state
+    if (st != MFREE_OK) return st;
     hipLaunchKernelGGL(k_pos_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b].f[0],
                        c->y[b].f[1], c->gid[b], s->cand);
     for (int k = 0; k < 8; k++) copy_async(c->y[b].f[k], s->jsnap + k * N, N * 8, c->stream);
