@@ -41,10 +41,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
 
 
-def run_mode(scene, precision, steps, warmup, log):
+def run_mode(scene, precision, steps, warmup, log, pipeline=0):
     from mfree_iwf_amd import Solver
     n = scene.particles.n
-    with Solver(precision=precision, use_graph=True) as sv:
+    with Solver(precision=precision, use_graph=True, pipeline=pipeline) as sv:
         sv.add_body(scene.particles, scene.params)
         t0 = time.perf_counter()
         sv.step(max(warmup, 1))
@@ -245,7 +245,9 @@ def main():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB mebibytes (HBM counter calibration)")
-    ap.add_argument("--modes", default="fp64,mixed,surrogate")
+    ap.add_argument("--modes", default="plain_fp64,fp64,mixed,surrogate")
+    ap.add_argument("--plain-steps", type=int, default=10,
+                    help="timed steps of the reference-structured FP64 baseline")
     ap.add_argument("--sur-scenes", type=int, default=256)
     ap.add_argument("--sur-captures", type=int, default=80)
     ap.add_argument("--sur-gap", type=int, default=0)
@@ -305,11 +307,14 @@ def main():
 
     modes = {}
     wanted = args.modes.split(",")
-    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+    from mfree_iwf_amd import _capi
+    for name, prec, pipe, nst in (("plain_fp64", PRECISION_FP64, _capi.PIPELINE_REFERENCE, args.plain_steps),
+                                  ("fp64", PRECISION_FP64, _capi.PIPELINE_FUSED, args.steps),
+                                  ("mixed", PRECISION_MIXED, _capi.PIPELINE_FUSED, args.steps)):
         if name not in wanted:
             continue
         log(f"mode {name}")
-        r = run_mode(scene, prec, args.steps, args.warmup, log)
+        r = run_mode(scene, prec, nst, min(args.warmup, nst), log, pipeline=pipe)
         r.pop("state")
         modes[name] = r
         log(f"  {r['particle_steps_per_s']:.4e} particle-steps/s ({r['ms_per_step']:.3f} ms/step), "
@@ -323,6 +328,10 @@ def main():
         log("mode surrogate")
         modes["surrogate"] = bench_surrogate(scene, args, ref_state, log)
 
+    if "plain_fp64" in modes:
+        base = modes["plain_fp64"]["particle_steps_per_s"]
+        for k, r in modes.items():
+            r["speedup_vs_plain_fp64"] = r["particle_steps_per_s"] / base
     head = modes.get("mixed") or modes.get("fp64")
     if head is None:  # a surrogate-only run: no SPH pair-kernel timing
         print(json.dumps({"metric": METRIC, "modes": modes, "accuracy": acc}), flush=True)

@@ -59,6 +59,13 @@ typedef int32_t mfree_status;
 #define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
 #define MFREE_METHOD_TL_RL 1       /* particle_tl, tensile_tl action list */
 
+/* Pipelines of a UL FP64 context. */
+#define MFREE_PIPELINE_FUSED 0     /* fused per-stage kernels (default) */
+#define MFREE_PIPELINE_REFERENCE 1 /* the reference's structure on the GPU: materialised
+                                      MAX_NBH neighbour and kernel lists per stage, one
+                                      kernel per action (src/body.h:125-148); the
+                                      "plain FP64 GPU" baseline */
+
 /* Arithmetic of the pair sums. */
 #define MFREE_PRECISION_FP64 0  /* everything FP64 (reference arithmetic) */
 #define MFREE_PRECISION_MIXED 1 /* FP32 pair sums with compensated accumulation,
@@ -86,6 +93,7 @@ typedef struct mfree_options {
   int32_t deterministic;    /* 1: within-cell order by idx (bit-reproducible) */
   double tl_h_search;       /* TL: h passed to find_neighbors; <= 0: per-particle h */
   int32_t tile_cells;       /* mixed UL: cells per LDS pair tile; 0 = autotune, < 0 = no tiles */
+  int32_t pipeline;         /* MFREE_PIPELINE_* (REFERENCE: UL + FP64 only) */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y

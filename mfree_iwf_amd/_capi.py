@@ -25,6 +25,8 @@ METHOD_UL_MONAGHAN = 0
 METHOD_TL_RL = 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PIPELINE_FUSED = 0
+PIPELINE_REFERENCE = 1
 
 dptr = C.POINTER(C.c_double)
 u32ptr = C.POINTER(C.c_uint32)
@@ -41,7 +43,7 @@ class Params(C.Structure):
 class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
-                ("tl_h_search", C.c_double), ("tile_cells", C.c_int32)]
+                ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")

@@ -172,6 +172,10 @@ mfree_status commit(mfree_ctx* c) {
     mfree_status st = tl_alloc(c->tl, (uint32_t)N, c->hX.data(), c->hY.data(), c->bc_left, c->bc_right, c->off);
     if (st != MFREE_OK) return fail(c, st, "TL allocation failed");
   }
+  if (c->opt.pipeline == MFREE_PIPELINE_REFERENCE) {
+    mfree_status st = ref_alloc(c);
+    if (st != MFREE_OK) return st;
+  }
   c->committed = true;
   if (!tl && c->opt.precision == MFREE_PRECISION_MIXED) {
     if (c->opt.tile_cells > 0) {
@@ -403,6 +407,10 @@ mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {
     HIPCHK(c, hipEventRecord(e0, c->stream));
   }
   for (uint32_t k = 0; k < nsteps; k++) {
+    if (c->ref) {
+      ref_enqueue_step(c);
+      continue;
+    }
     if (tl) {
       tl_enqueue_step(c->tl, tl_shared(c), c->prof ? &evs : nullptr, [c]() { return take_event(c); });
       hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc);
@@ -519,6 +527,7 @@ void mfree_default_options(mfree_options* o) {
   o->deterministic = 1;
   o->tl_h_search = 0.;
   o->tile_cells = 0;
+  o->pipeline = MFREE_PIPELINE_FUSED;
 }
 
 mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
@@ -537,6 +546,16 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     delete c;
     return MFREE_ERR_ARG;
   }
+  if (c->opt.pipeline != MFREE_PIPELINE_FUSED && c->opt.pipeline != MFREE_PIPELINE_REFERENCE) {
+    delete c;
+    return MFREE_ERR_ARG;
+  }
+  if (c->opt.pipeline == MFREE_PIPELINE_REFERENCE &&
+      (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.precision != MFREE_PRECISION_FP64 ||
+       c->opt.max_nbh > 200)) {
+    delete c;
+    return MFREE_ERR_UNSUPPORTED;
+  }
   *out = c;
   return MFREE_OK;
 }
@@ -549,6 +568,7 @@ void mfree_destroy(mfree_ctx* c) {
     if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   tl_free(c->tl);
+  ref_free(c);
   for (void* p : c->allocs) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -704,6 +724,8 @@ mfree_status mfree_eval_derivatives(mfree_ctx* c, double* d8, double* aux) {
     st = tl_ensure_precomputed(c->tl, tl_shared(c));
     if (st == MFREE_OK)
       tl_enqueue_eval(c->tl, tl_shared(c), dd, da);
+  } else if (c->ref) {
+    ref_enqueue_eval(c, dd);
   } else {
     const int b = c->cur;
     launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));

@@ -18,6 +18,10 @@ using mfree::SceneConst;
 using mfree::SceneGeom;
 using mfree::TlState;
 
+namespace mfree {
+struct RefPipe;  // reference-structured FP64 pipeline (baseline.hip)
+}
+
 struct mfree_ctx {
   mfree_options opt;
   std::string err;
@@ -58,6 +62,7 @@ struct mfree_ctx {
   uint32_t tile_grid = 0;
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
+  mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
 
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
@@ -91,5 +96,11 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b);
 UlArgs ul_args(mfree_ctx* c, int b);
 // nsteps RK4 steps of every body (graphs when enabled).
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps);
+
+// Reference-structured FP64 pipeline (baseline.hip).
+mfree_status ref_alloc(mfree_ctx* c);
+void ref_free(mfree_ctx* c);
+void ref_enqueue_step(mfree_ctx* c);
+void ref_enqueue_eval(mfree_ctx* c, double* d8);
 
 }  // namespace mfree
