@@ -306,8 +306,9 @@ typedef struct mfree_judge_stats {
  * warm-up steps, then groups of `group` learned steps; the last one of each
  * group is judged against 4 plain RK4 steps from the same state (the plain
  * result is kept).  If the relative L2 deviation of all positions exceeds
- * `threshold` the group is rolled back and redone with plain steps.  Steps
- * that do not fill a group run plain. */
+ * `threshold` the group is rolled back and redone with plain steps.  The
+ * last group may be shorter (at least 2 learned steps); fewer than 8
+ * remaining steps run plain. */
 mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* ctx, uint32_t nsteps, uint32_t group,
                                      double threshold, mfree_judge_stats* stats);
 

@@ -1232,7 +1232,6 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
   if (!s->have_norm) return sfail(s, MFREE_ERR_ARG, "network has no normalisation (collect or set_params)");
   mfree_judge_stats js;
   memset(&js, 0, sizeof(js));
-  const uint32_t span = group * kStepsPerLearned;
   uint32_t done = 0;
   auto plain = [&](uint32_t k) -> mfree_status {
     mfree_status r = run_steps(c, k);
@@ -1240,7 +1239,7 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     return r == MFREE_OK ? MFREE_OK : sfail(s, r, std::string("plain steps: ") + c->err);
   };
   // warm start: the features need the velocity one learned step back
-  if (nsteps >= span + kStepsPerLearned) {
+  if (nsteps >= 3 * kStepsPerLearned) {
     record_vprev(s, c);
     st = plain(kStepsPerLearned);
     if (st != MFREE_OK) return st;
@@ -1248,10 +1247,13 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
   }
   std::vector<double> times;
   const size_t N = c->N;
-  while (done + span <= nsteps) {
+  // groups of `group` learned steps; the last one may be shorter (>= 2)
+  while (nsteps - done >= 2 * kStepsPerLearned) {
+    const uint32_t glen = std::min(group, (nsteps - done) / kStepsPerLearned);
+    const uint32_t gspan = glen * kStepsPerLearned;
     auto t0 = std::chrono::steady_clock::now();
     snapshot(s, c, times);
-    for (uint32_t k = 0; k + 1 < group; k++) {
+    for (uint32_t k = 0; k + 1 < glen; k++) {
       st = enqueue_learned(s, c, true);
       if (st != MFREE_OK) return st;
     }
@@ -1282,21 +1284,21 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     js.groups++;
     js.last_deviation = dev;
     js.max_deviation = std::max(js.max_deviation, dev);
-    js.learned_steps += group;
+    js.learned_steps += glen;
     if (!(dev <= threshold)) {
       // roll back the group and redo it with plain RK4 steps
       auto t2 = std::chrono::steady_clock::now();
       js.fallbacks++;
-      js.learned_steps -= group;
+      js.learned_steps -= glen;
       restore(s, c, times);
-      st = plain(span - kStepsPerLearned);
+      st = plain(gspan - kStepsPerLearned);
       if (st != MFREE_OK) return st;
       record_vprev(s, c);
       st = plain(kStepsPerLearned);
       if (st != MFREE_OK) return st;
       js.ms_fallback += ms_since(t2);
     }
-    done += span;
+    done += gspan;
   }
   if (done < nsteps) {
     if (nsteps - done >= kStepsPerLearned) {
