@@ -4,7 +4,7 @@
 // Architecture (hidden width H = 128, message width C = 32, L = 3 layers):
 //   X0  = node features (NF = 16, zero-padded to 32)            [N x 32]  bf16
 //   H0  = SiLU(X0 We^T + be)                                    [N x 128]
-//   layer l:  U  = H Wu^T                                       [N x 32]
+//   layer l:  U  = H Wu^T  (fused into the GEMM that produces H) [N x 32]
 //             A  = [sum_j w_ij U_j, sum_j gx_ij U_j, sum_j gy_ij U_j]   [N x 96]
 //             H' = H + SiLU([H, A] Wo^T + bo)                   [N x 128]
 //   Y   = H3 Wd^T + bd                                          [N x 16] (8 used)

@@ -55,6 +55,9 @@ struct GemmEpi {
   float* zout;                  // pre-activation (EPI_SILU_RES, or null)
   const float* res;             // FP32 residual (ld kH), or
   const bf16_t* res16;          // BF16 residual (ld kH), or neither
+  // EPI_SILU_RES with M == kH: also U = bf16(h) Wu^T of the next layer
+  const bf16_t* wu;             // kC x kH bf16 (or null)
+  bf16_t* u_out;                // N x kC bf16
 };
 
 // Launch shape: a weight tile above 32 KB is shared by 16 waves of one
@@ -82,7 +85,7 @@ __device__ inline float4 unpack4_bf16(uint2 u) {
 // A = 16 weight rows, B = 16 node rows (both 8 contiguous bf16 per lane),
 // so a lane ends with one node and 4 consecutive output channels (vector
 // epilogue).  Persistent: block b handles node tiles b, b + grid, ...
-template <int K, int M, int ROWS, int MODE, int KS, int THREADS>
+template <int K, int M, int ROWS, int MODE, int KS, int THREADS, bool FU = false>
 __global__ __launch_bounds__(THREADS) void k_gemm(const bf16_t* __restrict__ X, int ldx,
                                                        const bf16_t* __restrict__ X2, int ldx2,
                                                        const bf16_t* __restrict__ W, uint32_t N, GemmEpi e) {
@@ -96,6 +99,15 @@ __global__ __launch_bounds__(THREADS) void k_gemm(const bf16_t* __restrict__ X, 
   constexpr bool kStage = (MODE == EPI_BF16 || MODE == EPI_SILU_RES) && M <= 128;
   constexpr int MP = M + 8;
   __shared__ __attribute__((aligned(16))) bf16_t ostage[kStage ? kWaves * 16 * MP : 8];
+  // fused message projection of the next layer (FU): Wu staged like W
+  constexpr int KU = kH + 8;
+  __shared__ __attribute__((aligned(16))) bf16_t wul[FU ? kC * KU : 8];
+  if constexpr (FU) {
+    for (int i = threadIdx.x * 8; i < kC * kH; i += THREADS * 8) {
+      const int r = i / kH, c = i - r * kH;
+      *(uint4*)&wul[r * KU + c] = *(const uint4*)&e.wu[i];
+    }
+  }
   for (int i = threadIdx.x * 8; i < M * K; i += THREADS * 8) {
     const int r = i / K, c = i - r * K;
     *(uint4*)&wl[r * KP + c] = *(const uint4*)&W[i];
@@ -215,6 +227,28 @@ __global__ __launch_bounds__(THREADS) void k_gemm(const bf16_t* __restrict__ X, 
           const int row = c / CH, ch = c - row * CH;
           if (n0 + row < N)
             *(uint4*)&e.out16[(size_t)(n0 + row) * e.ld16 + ch * 8] = *(const uint4*)&stg[row * MP + ch * 8];
+        }
+        if constexpr (FU) {
+          // U of the staged rows: D = Wu H^T (2 tiles of 16 channels, 4 k-steps)
+          f32x4 ua[kC / 16];
+#pragma unroll
+          for (int t2 = 0; t2 < kC / 16; t2++) ua[t2] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kk = 0; kk < kH / 32; kk++) {
+            const bf16x8 hb = *(const bf16x8*)&stg[(lane & 15) * MP + kk * 32 + kq];
+#pragma unroll
+            for (int t2 = 0; t2 < kC / 16; t2++) {
+              const bf16x8 wa = *(const bf16x8*)&wul[(t2 * 16 + (lane & 15)) * KU + kk * 32 + kq];
+              ua[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hb, ua[t2], 0, 0, 0);
+            }
+          }
+          const uint32_t nu = n0 + (lane & 15);
+          if (nu < N) {
+#pragma unroll
+            for (int t2 = 0; t2 < kC / 16; t2++)
+              *(uint2*)&e.u_out[(size_t)nu * kC + t2 * 16 + mq] =
+                  pack4_bf16(ua[t2][0], ua[t2][1], ua[t2][2], ua[t2][3]);
+          }
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
         __builtin_amdgcn_wave_barrier();
@@ -491,6 +525,13 @@ void gemm2(const bf16_t* X, int ldx, const bf16_t* X2, int ldx2, const bf16_t* W
                          N, e);
       break;
     default:
+      if constexpr (M == kH) {
+        if (e.wu) {
+          hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SILU_RES, KS, TH, true>), dim3(grid), dim3(TH), 0, st, X, ldx,
+                             X2, ldx2, W, N, e);
+          break;
+        }
+      }
       hipLaunchKernelGGL((k_gemm<K, M, ROWS, EPI_SILU_RES, KS, TH>), dim3(grid), dim3(TH), 0, st, X, ldx, X2, ldx2,
                          W, N, e);
       break;
@@ -671,13 +712,10 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
   e.out = train ? a.Hf[0] : nullptr;
   e.out16 = a.H16[0];
   e.ld16 = kH;
+  e.wu = net.t[net.Wu[0]].w16;  // U_0 = H_0 Wu_0^T from the staged rows
+  e.u_out = a.U[0];
   gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
   for (int l = 0; l < kLayers; l++) {
-    // U = H Wu^T
-    GemmEpi eu = epi(EPI_BF16);
-    eu.out16 = a.U[l];
-    eu.ld16 = kC;
-    gemm<kH, kC>(a.H16[l], kH, net.t[net.Wu[l]].w16, n, eu, st);
     // A = aggregate(U)
     hipLaunchKernelGGL(k_agg_fwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,
                        a.U[l], a.A16[l]);
@@ -691,6 +729,10 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
     eo.out = train ? a.Hf[l + 1] : nullptr;
     eo.out16 = a.H16[l + 1];
     eo.ld16 = kH;
+    if (l + 1 < kLayers) {  // U_{l+1} = H_{l+1} Wu_{l+1}^T
+      eo.wu = net.t[net.Wu[l + 1]].w16;
+      eo.u_out = a.U[l + 1];
+    }
     gemm2<kG, kH, kH>(a.H16[l], kH, a.A16[l], 3 * kC, net.t[net.Wo[l]].w16, n, eo, st);
   }
   GemmEpi ed = epi(EPI_F32);
