@@ -6,7 +6,7 @@
 //   1. D1 = time derivatives at the state (the stage-0 pass of the UL
 //      pipeline, src/body.h:125-148)
 //   2. rho and S advanced by D1 over tau (velocity and positions kept); D2 =
-//      derivatives at that state
+//      the acceleration at that state
 //   3. base update (FP64):  S += tau*dS1, rho *= exp(tau*drho1/rho),
 //                           v += tau*a2, x += tau*xdot1 + tau^2/2*a2
 //   4. the network adds a correction to all 8 fields (FP64 update), its
@@ -633,6 +633,7 @@ void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
   v.stage = 0;
   v.d8 = s->d2;
   v.d8_sorted = 1;
+  v.accel_only = 1;  // the step uses only the acceleration of D2
   launch_ul_pairs(st, v);
 }
 

@@ -110,6 +110,9 @@ __device__ inline Nb nb_from_rec(const PairRec32& r) {
 }
 
 // All four pair sums of one neighbour at separation (xij, yij), r2 = |x_ij|^2.
+// ACC: only the sums the acceleration needs (stress divergence, artificial
+// viscosity); the velocity gradient and XSPH sums stay zero.
+template <bool ACC>
 __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
   const float rij = sqrtf(r2);
   const float q = rij * me.h1;
@@ -139,10 +142,12 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
   acc_add(S.Sxy_x, mj * (me.sxy + nj.sxy + Rxy) * wx);
   acc_add(S.Syy_y, mj * (me.syy + nj.syy + Ryy) * wy);
   const float dvx = nj.vx - me.vx, dvy = nj.vy - me.vy;
-  acc_add(S.vx_x, dvx * wx * nj.qw);
-  acc_add(S.vx_y, dvx * wy * nj.qw);
-  acc_add(S.vy_x, dvy * wx * nj.qw);
-  acc_add(S.vy_y, dvy * wy * nj.qw);
+  if constexpr (!ACC) {
+    acc_add(S.vx_x, dvx * wx * nj.qw);
+    acc_add(S.vx_y, dvx * wy * nj.qw);
+    acc_add(S.vy_x, dvy * wx * nj.qw);
+    acc_add(S.vy_y, dvy * wy * nj.qw);
+  }
   const float vp = -(dvx * xij + dvy * yij);
   if (!(vp >= 0.f)) {
     const float cij = 0.5f * (me.c + nj.c);
@@ -153,10 +158,12 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
     acc_add(S.ax, -mj * piij * wx);
     acc_add(S.ay, -mj * piij * wy);
   }
-  const float rhoij = 0.5f * (me.rho + nj.rho);
-  const float fx = -me.xeps_mi * w / rhoij;
-  acc_add(S.xs, -fx * dvx);
-  acc_add(S.ys, -fx * dvy);
+  if constexpr (!ACC) {
+    const float rhoij = 0.5f * (me.rho + nj.rho);
+    const float fx = -me.xeps_mi * w / rhoij;
+    acc_add(S.xs, -fx * dvx);
+    acc_add(S.ys, -fx * dvy);
+  }
 }
 
 __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
@@ -246,6 +253,7 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
 
 // Sorted particle i straight from global memory, no buffering (the
 // tile kernel's path for tiles whose candidates do not fit in LDS).
+template <bool ACC>
 __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -273,7 +281,7 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
         const float yij = (pi.y - pj.y) + offy;
         const float r2 = xij * xij + yij * yij;
         if (r2 > me.cut2) continue;
-        pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
+        pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
       }
     }
   }
@@ -288,6 +296,7 @@ constexpr int kNbCap = 32;  // compacted neighbours buffered per thread
 // inside the cutoff in LDS (sorted position + cell offsets packed in 32
 // bits), then evaluate the pair terms on the compact list with full SIMD
 // occupancy.
+template <bool ACC>
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
@@ -312,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
       const float2 pj = a.pos32[j];
       const float xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
       const float yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
-      pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, xij * xij + yij * yij, S);
+      pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, xij * xij + yij * yij, S);
     }
     nb = 0;
   };
@@ -346,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
 constexpr int kTileCap = 768;  // staged candidates per tile (10-bit LDS index)
 constexpr int kTileNb = 32;    // compacted neighbours buffered per thread
 
+template <bool ACC>
 __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
   __shared__ float s_px[kTileCap], s_py[kTileCap];
   __shared__ float s_vx[kTileCap], s_vy[kTileCap], s_rho[kTileCap], s_qw[kTileCap], s_m[kTileCap];
@@ -430,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
     const float cellf = (float)g.cell;
     for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
       if (!staged) {
-        particle_global(a, i);
+        particle_global<ACC>(a, i);
         continue;
       }
       const uint32_t li = s_off[1] + (i - s_rs[1]);
@@ -474,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
           nj.Rxx = s_Rxx[lj];
           nj.Rxy = s_Rxy[lj];
           nj.Ryy = s_Ryy[lj];
-          pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
+          pair_terms<ACC>(me, nj, xij, yij, xij * xij + yij * yij, S);
         }
         nb = 0;
       };
@@ -510,10 +520,17 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
 }
 
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {
-  if (a.tile_cells && a.tile_off)
-    hipLaunchKernelGGL(k_ul_tiles32, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_ul_pairs32, dim3(xcd_grid((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
+  const dim3 grid(xcd_grid((a.n + kBlock - 1) / kBlock));
+  if (a.tile_cells && a.tile_off) {
+    if (a.accel_only)
+      hipLaunchKernelGGL(k_ul_tiles32<true>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_ul_tiles32<false>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
+  } else if (a.accel_only) {
+    hipLaunchKernelGGL(k_ul_pairs32<true>, grid, dim3(kBlock), 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_ul_pairs32<false>, grid, dim3(kBlock), 0, st, a);
+  }
 }
 
 }  // namespace mfree

@@ -141,6 +141,21 @@ def test_base_step_matches_oracle():
     assert t == pytest.approx(4 * s.params.dt, rel=1e-12)
 
 
+def test_base_step_mixed_close_to_fp64():
+    """The mixed path evaluates D2 with the acceleration-only pair pass."""
+    s = perturbed_rings(nbox=41, seed=8, dpos=0.2)
+    out = []
+    for prec in (PRECISION_FP64, PRECISION_MIXED):
+        with _ctx(s, prec) as sv, sg.Surrogate() as su:
+            su.record_velocity(sv)
+            su.step(sv, use_net=False)
+            out.append(sv.state(0).state8())
+    base = s.particles.state8()
+    for k in range(8):
+        # increments agree to the mixed mode's FP32 pair sums
+        assert O.rel_l2(out[1][:, k] - base[:, k], out[0][:, k] - base[:, k]) < 1e-4, scenes.STATE_FIELDS[k]
+
+
 def test_learned_step_matches_oracle():
     """Network step on random weights: the correction added to the base
     update agrees with the FP64 oracle's features -> forward -> correction."""
