@@ -329,7 +329,9 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* ctx, uint32_t*
  * host graph (CSR, at most 64 edges per node) over nodes at pos3 = (x, y, h) per
  * node (the edge features are evaluated from these as in a learned step),
  * normalised node features x (n x 16) and targets t (n x 8).  The graph
- * must be symmetric.  y: n x 8 outputs; loss: MSE; grad: dloss/dw. */
+ * must be symmetric.  y: n x 8 outputs; loss: MSE; grad: dloss/dw.  With
+ * grad == NULL only the inference forward runs (the fused layer kernels a
+ * learned step uses): t may be NULL and loss is left untouched. */
 mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
                                          const float* pos3, const float* x, const float* t, const float* w,
                                          float* y, float* loss, float* grad);

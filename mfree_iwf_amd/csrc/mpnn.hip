@@ -402,6 +402,183 @@ __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, co
   }
 }
 
+// One message-passing layer of the inference forward, fused into one kernel:
+//   A  = [sum_e w U_j, sum_e gx U_j, sum_e gy U_j] over the edges of each node
+//   H' = H + SiLU([H, A] Wo^T + bo)
+//   then U' = H' Wu'^T (OUT == LO_U: the next layer's messages; H' stored) or
+//   Y = H' Wd^T + bd (OUT == LO_DEC: the decoder; H' is not stored).
+// A wave owns 16 nodes.  The aggregation runs in the MFMA operand layout:
+// lane (node lane%16, channel block 8*(lane/16)) sums its 8 channels of the
+// three messages, so the packed sums are the B operands of the three
+// message k-steps without a round trip through memory.  Edges go in batches
+// of 8: each lane loads the 8 neighbour ids of its node and its 16-byte
+// slice of their U rows; the 4 lanes of a node split the 8 edge-weight
+// evaluations (2 each) and exchange them through the wave's LDS stage.
+// Every node sums its edges in edge order, as k_agg_fwd does.  Waves are
+// persistent and XCD-major: the waves of XCD x walk the x-th eighth of the
+// 16-node tiles.
+enum LayerOut : int { LO_U = 0, LO_DEC = 1 };
+constexpr int kLayerThreads = 1024;
+
+template <int OUT>
+__global__ __launch_bounds__(kLayerThreads) void k_layer(
+    uint32_t N, const bf16_t* __restrict__ H, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ col,
+    uint32_t stride, const float4* __restrict__ npos, const bf16_t* __restrict__ U, const bf16_t* __restrict__ Wo,
+    const float* __restrict__ bo, const bf16_t* __restrict__ Wp, const float* __restrict__ bp,
+    bf16_t* __restrict__ Hout, bf16_t* __restrict__ Uout, float* __restrict__ Y) {
+  constexpr int K = kG, M = kH, KP = K + 8, T = M / 16;
+  constexpr int kWaves = kLayerThreads / 64;
+  constexpr int MP = M + 8;
+  constexpr int PR = OUT == LO_U ? kC : kOutP;  // projection rows
+  constexpr int KU = kH + 8;
+  __shared__ __attribute__((aligned(16))) bf16_t wl[M * KP];
+  __shared__ __attribute__((aligned(16))) bf16_t ostage[kWaves * 16 * MP];
+  __shared__ __attribute__((aligned(16))) bf16_t wpl[PR * KU];
+  for (int i = threadIdx.x * 8; i < PR * kH; i += kLayerThreads * 8) {
+    const int r = i / kH, c = i - r * kH;
+    *(uint4*)&wpl[r * KU + c] = *(const uint4*)&Wp[i];
+  }
+  for (int i = threadIdx.x * 8; i < M * K; i += kLayerThreads * 8) {
+    const int r = i / K, c = i - r * K;
+    *(uint4*)&wl[r * KP + c] = *(const uint4*)&Wo[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane & 15, kg = lane >> 4;
+  const int kq = kg * 8, mq = kg * 4;
+  bf16_t* stg = ostage + wave * 16 * MP;
+  float4* ew = (float4*)stg;  // 16 nodes x 8 edges of (w, gx, gy, 0) while aggregating
+  const uint32_t ntile = (N + 15) / 16;
+  const uint32_t xcd = blockIdx.x % kXcds, bx = blockIdx.x / kXcds, nbx = gridDim.x / kXcds;
+  const uint32_t t_lo = (uint32_t)((uint64_t)ntile * xcd / kXcds);
+  const uint32_t t_hi = (uint32_t)((uint64_t)ntile * (xcd + 1) / kXcds);
+  for (uint32_t tile = t_lo + bx * kWaves + wave; tile < t_hi; tile += nbx * kWaves) {
+    const uint32_t n0 = tile * 16, n = n0 + q;
+    const bool valid = n < N;
+    const uint32_t nn = valid ? n : N - 1;  // clamped row for the loads of a partial tile
+    const float4 pi = npos[nn];
+    const uint32_t d = valid ? deg[nn] : 0u;
+    uint32_t dm = d;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) dm = max(dm, (uint32_t)__shfl_xor((int)dm, o, 64));
+    dm = __builtin_amdgcn_readfirstlane(dm);
+    float a0[8], ax[8], ay[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) a0[c] = ax[c] = ay[c] = 0.f;
+    for (uint32_t k0 = 0; k0 < dm; k0 += 8) {
+      uint32_t j[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) j[b] = k0 + b < d ? col[(size_t)(k0 + b) * stride + nn] : nn;
+      uint4 uv[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) uv[b] = *(const uint4*)(U + (size_t)j[b] * kC + kq);
+      // this lane's two edge weights: batch slots kg and kg + 4
+      uint32_t jA = j[0], jB = j[4];
+      if (kg == 1) {
+        jA = j[1];
+        jB = j[5];
+      } else if (kg == 2) {
+        jA = j[2];
+        jB = j[6];
+      } else if (kg == 3) {
+        jA = j[3];
+        jB = j[7];
+      }
+      const float4 pA = npos[jA], pB = npos[jB];
+      float w, gx, gy;
+      edge_weights(pi, pA, w, gx, gy);
+      ew[q * 8 + kg] = k0 + kg < d ? make_float4(w, gx, gy, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      edge_weights(pi, pB, w, gx, gy);
+      ew[q * 8 + kg + 4] = k0 + kg + 4 < d ? make_float4(w, gx, gy, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const float4 e4 = ew[q * 8 + b];
+        float f[8];
+        unpack8_bf16(uv[b], f);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          a0[c] += e4.x * f[c];
+          ax[c] += e4.y * f[c];
+          ay[c] += e4.z * f[c];
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+    }
+    bf16x8 xh[kH / 32];
+#pragma unroll
+    for (int kk = 0; kk < kH / 32; kk++) xh[kk] = *(const bf16x8*)(H + (size_t)nn * kH + kk * 32 + kq);
+    bf16x8 xa[3];
+    xa[0] = __builtin_bit_cast(bf16x8, pack8_bf16(a0));
+    xa[1] = __builtin_bit_cast(bf16x8, pack8_bf16(ax));
+    xa[2] = __builtin_bit_cast(bf16x8, pack8_bf16(ay));
+    f32x4 acc[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < K / 32; kk++) {
+      const bf16x8 xb = kk < kH / 32 ? xh[kk] : xa[kk - kH / 32];
+#pragma unroll
+      for (int t = 0; t < T; t++) {
+        const bf16x8 wa = *(const bf16x8*)&wl[(t * 16 + q) * KP + kk * 32 + kq];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb, acc[t], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // H' = H + SiLU(z) into the stage (rows of this tile)
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+      const int m = t * 16 + mq;
+      const float4 rr = unpack4_bf16(*(const uint2*)&H[(size_t)nn * kH + m]);
+      const float4 bb = *(const float4*)&bo[m];
+      const float4 z = make_float4(acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w);
+      *(uint2*)&stg[q * MP + m] =
+          pack4_bf16(rr.x + silu(z.x), rr.y + silu(z.y), rr.z + silu(z.z), rr.w + silu(z.w));
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    if constexpr (OUT == LO_U) {
+      constexpr int CH = M / 8;
+#pragma unroll
+      for (int c = lane; c < 16 * CH; c += 64) {
+        const int row = c / CH, ch = c - row * CH;
+        if (n0 + row < N) *(uint4*)&Hout[(size_t)(n0 + row) * kH + ch * 8] = *(const uint4*)&stg[row * MP + ch * 8];
+      }
+    }
+    // projection of the staged rows: D = Wp H'^T
+    f32x4 pa[PR / 16];
+#pragma unroll
+    for (int t2 = 0; t2 < PR / 16; t2++) pa[t2] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < kH / 32; kk++) {
+      const bf16x8 hb = *(const bf16x8*)&stg[q * MP + kk * 32 + kq];
+#pragma unroll
+      for (int t2 = 0; t2 < PR / 16; t2++) {
+        const bf16x8 wa = *(const bf16x8*)&wpl[(t2 * 16 + q) * KU + kk * 32 + kq];
+        pa[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hb, pa[t2], 0, 0, 0);
+      }
+    }
+    if (valid) {
+      if constexpr (OUT == LO_U) {
+#pragma unroll
+        for (int t2 = 0; t2 < PR / 16; t2++)
+          *(uint2*)&Uout[(size_t)n * kC + t2 * 16 + mq] = pack4_bf16(pa[t2][0], pa[t2][1], pa[t2][2], pa[t2][3]);
+      } else {
+        float4 y = make_float4(pa[0][0], pa[0][1], pa[0][2], pa[0][3]);
+        y.x += mq < kOut ? bp[mq] : 0.f;
+        y.y += mq + 1 < kOut ? bp[mq + 1] : 0.f;
+        y.z += mq + 2 < kOut ? bp[mq + 2] : 0.f;
+        y.w += mq + 3 < kOut ? bp[mq + 3] : 0.f;
+        *(float4*)&Y[(size_t)n * kOutP + mq] = y;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // Aggregation backward, one node per lane:
 //   dU[j][c] = sum_{i in N(j)} (w dA0_i - gx_ji dAx_i - gy_ji dAy_i)
 // (w symmetric, gx/gy antisymmetric); dA = dA16 [n x 96] bf16.  Writes dU
@@ -704,6 +881,7 @@ void mp_acts_free(MpActs& a) {
 }
 
 void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st) {
+  if (n == 0) return;
   // encoder: H0 = SiLU(X0 We^T + be)
   GemmEpi e = epi(EPI_SILU_RES);
   e.bias = net.w + net.t[net.be].off;
@@ -715,6 +893,23 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
   e.wu = net.t[net.Wu[0]].w16;  // U_0 = H_0 Wu_0^T from the staged rows
   e.u_out = a.U[0];
   gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
+  if (!train) {
+    // fused layers; the last one ends in the decoder
+    const uint32_t grid = std::min<uint32_t>(256, xcd_grid(blocks(blocks(n, 16), kLayerThreads / 64)));
+    for (int l = 0; l < kLayers; l++) {
+      const bool last = l + 1 == kLayers;
+      const bf16_t* wp = last ? net.t[net.Wd].w16 : net.t[net.Wu[l + 1]].w16;
+      const float* bo = net.w + net.t[net.bo[l]].off;
+      if (last)
+        hipLaunchKernelGGL(k_layer<LO_DEC>, dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[l], g.deg, g.col, g.cap,
+                           g.npos, a.U[l], net.t[net.Wo[l]].w16, bo, wp, net.w + net.t[net.bd].off, nullptr, nullptr,
+                           a.Y);
+      else
+        hipLaunchKernelGGL(k_layer<LO_U>, dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[l], g.deg, g.col, g.cap,
+                           g.npos, a.U[l], net.t[net.Wo[l]].w16, bo, wp, nullptr, a.H16[l + 1], a.U[l + 1], nullptr);
+    }
+    return;
+  }
   for (int l = 0; l < kLayers; l++) {
     // A = aggregate(U)
     hipLaunchKernelGGL(k_agg_fwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,

@@ -1382,7 +1382,8 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
 mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
                                          const float* pos3, const float* x, const float* t, const float* w,
                                          float* y, float* loss, float* grad) {
-  if (!row_ptr || !col || !pos3 || !x || !t || !w || n == 0) return MFREE_ERR_ARG;
+  if (!row_ptr || !col || !pos3 || !x || !w || n == 0 || (grad && !t)) return MFREE_ERR_ARG;
+  const bool train = grad != nullptr;  // without grad: the inference forward only
   if (row_ptr[0] != 0) return MFREE_ERR_ARG;
   for (uint32_t i = 0; i < n; i++) {
     if (row_ptr[i + 1] < row_ptr[i] || row_ptr[i + 1] - row_ptr[i] > kMaxDeg) return MFREE_ERR_ARG;
@@ -1397,7 +1398,7 @@ mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint3
   MpGraph g;
   std::string msg;
   mfree_status st = mp_net_init(net, 0, msg);
-  if (st == MFREE_OK) st = mp_acts_reserve(acts, n, true, msg);
+  if (st == MFREE_OK) st = mp_acts_reserve(acts, n, train, msg);
   if (st == MFREE_OK) st = mp_graph_reserve(g, n, msg);
   float *dT = nullptr, *dL = nullptr;
   hipStream_t strm = nullptr;
@@ -1419,14 +1420,16 @@ mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint3
               hipMemcpy(g.col, cs.data(), cs.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(g.npos, np.data(), (size_t)n * 16, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(acts.X0, x16.data(), x16.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(dT, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              (!train || hipMemcpy(dT, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice) == hipSuccess) &&
               hipMemcpy(dL, &zero, 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(net.w, w, net.nparam * 4, hipMemcpyHostToDevice) == hipSuccess;
     if (ok) {
       mp_net_sync16(net, strm);
-      mp_forward(net, acts, g, n, true, strm);
-      mp_zero_grad(net, strm);
-      mp_backward(net, acts, g, n, dT, dL, strm);
+      mp_forward(net, acts, g, n, train, strm);
+      if (train) {
+        mp_zero_grad(net, strm);
+        mp_backward(net, acts, g, n, dT, dL, strm);
+      }
       ok = hipStreamSynchronize(strm) == hipSuccess;
     }
     if (ok && y) {
@@ -1435,7 +1438,7 @@ mfree_status mfree_mpnn_forward_backward(int32_t device, uint32_t n, const uint3
       for (uint32_t i = 0; ok && i < n; i++)
         for (int k = 0; k < kOut; k++) y[(size_t)i * kOut + k] = yy[(size_t)i * kOutP + k];
     }
-    if (ok && loss) ok = hipMemcpy(loss, dL, 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (ok && loss && train) ok = hipMemcpy(loss, dL, 4, hipMemcpyDeviceToHost) == hipSuccess;
     if (ok && grad) ok = hipMemcpy(grad, net.g, net.nparam * 4, hipMemcpyDeviceToHost) == hipSuccess;
     if (!ok) st = MFREE_ERR_HIP;
   }

@@ -36,9 +36,10 @@ def layout():
     return [(int(sh[2 * k]), int(sh[2 * k + 1])) for k in range(nt)]
 
 
-def forward_backward(row_ptr, col, pos3, x, t, w, device=0):
+def forward_backward(row_ptr, col, pos3, x, t, w, device=0, train=True):
     """One forward + backward on the GPU over nodes at pos3 = (x, y, h):
-    returns (y [n x 8], loss, grad)."""
+    returns (y [n x 8], loss, grad).  With train=False only the inference
+    forward runs (t is ignored): returns (y, None, None)."""
     n = len(row_ptr) - 1
     row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
     col = np.ascontiguousarray(col, dtype=np.uint32)
@@ -51,6 +52,11 @@ def forward_backward(row_ptr, col, pos3, x, t, w, device=0):
     grad = np.zeros(param_count(), dtype=np.float32)
     if len(col) == 0:
         col = np.zeros(1, dtype=np.uint32)
+    if not train:
+        st = _capi.lib().mfree_mpnn_forward_backward(device, n, _u32(row_ptr), _u32(col), _fp(pos3), _fp(x), None,
+                                                     _fp(w), _fp(y), None, None)
+        _capi.check(st, what="mfree_mpnn_forward_backward")
+        return y, None, None
     st = _capi.lib().mfree_mpnn_forward_backward(device, n, _u32(row_ptr), _u32(col), _fp(pos3), _fp(x), _fp(t),
                                                  _fp(w), _fp(y), _fp(loss), _fp(grad))
     _capi.check(st, what="mfree_mpnn_forward_backward")

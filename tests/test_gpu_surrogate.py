@@ -100,6 +100,41 @@ def test_mpnn_forward_backward_matches_oracle(n, seed):
     assert cos > 0.998
 
 
+def ragged_graph(n, seed):
+    """Jittered lattice with per-node smoothing lengths: isolated nodes,
+    degrees up to the 64-edge limit, a partial last tile of 16 nodes."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    ij = np.array([(i, j) for i in range(side) for j in range(side)][:n], dtype=np.float64)
+    p = (ij + 0.2 * rng.standard_normal(ij.shape)).astype(np.float32).astype(np.float64)
+    h = rng.uniform(0.25, 2.0, n)
+    h[(ij[:, 0] < 8) & (ij[:, 1] < 8)] = 2.2  # a block of dense rows
+    p[-5:] = [(side + 50.0 + 10.0 * k, 0.0) for k in range(5)]  # isolated nodes
+    h = h.astype(np.float32).astype(np.float64)
+    g = om.radius_graph(p[:, 0], p[:, 1], h)
+    return np.stack([p[:, 0], p[:, 1], h], axis=1), g
+
+
+@pytest.mark.parametrize("n,seed", [(300, 0), (1000, 1), (4099, 2)])
+def test_mpnn_inference_forward_matches_oracle(n, seed):
+    """The fused layer kernels (aggregation in the MFMA operand layout, decoder
+    in the last layer's epilogue) against the oracle and the training forward."""
+    pos3, (row, col, e3, _) = ragged_graph(n, seed)
+    deg = np.diff(row)
+    assert deg.max() <= sg.MAX_DEG and (deg == 0).any() and deg.max() > 40
+    rng = np.random.default_rng(seed + 10)
+    X = rng.standard_normal((n, om.NF))
+    w = om.init_params(seed=seed + 20).astype(np.float32)
+    print(f"[mpnn inference fwd] n={n} edges={len(col)} max deg={deg.max()}", flush=True)
+    y, _, _ = sg.forward_backward(row, col, pos3, X, None, w, train=False)
+    yt, _, _ = sg.forward_backward(row, col, pos3, X, np.zeros((n, om.NOUT)), w)
+    oy, _ = om.forward(row, col, e3, X, w.astype(np.float64))
+    assert np.isfinite(y).all()
+    assert O.rel_l2(y, oy) < Y_TOL
+    # same sums; the training forward keeps an FP32 residual stream
+    assert O.rel_l2(y, yt) < 1e-2
+
+
 def _ctx(scene, precision=PRECISION_FP64):
     sv = Solver(precision=precision)
     sv.add_body(scene.particles, scene.params)
