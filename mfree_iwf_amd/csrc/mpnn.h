@@ -74,15 +74,16 @@ __device__ inline void spline_shape(float q, float& f, float& df) {
 
 // Edge features from node records (x, y, h, -): w = W(r, h_ij) h_ij^2 and
 // (gx, gy) = grad_i W(r, h_ij) h_ij^3.  fl(xi - xj) = -fl(xj - xi), so the
-// pair (j, i) gets the same w and the negated gradient.
+// pair (j, i) gets the same w and the negated gradient.  Square root and
+// reciprocals are the hardware's single instructions (about 1 ulp).
 __device__ inline void edge_weights(const float4 pi, const float4 pj, float& w, float& gx, float& gy) {
   const float dx = pi.x - pj.x, dy = pi.y - pj.y;
   const float hij = 0.5f * (pi.z + pj.z);
-  const float r = sqrtf(dx * dx + dy * dy);
+  const float r = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
   float f, df;
-  spline_shape(r / hij, f, df);
+  spline_shape(r * __builtin_amdgcn_rcpf(hij), f, df);
   w = kSplineNorm * f;
-  const float sc = r > 0.f ? kSplineNorm * df / r : 0.f;
+  const float sc = r > 0.f ? kSplineNorm * df * __builtin_amdgcn_rcpf(r) : 0.f;
   gx = sc * dx;
   gy = sc * dy;
 }

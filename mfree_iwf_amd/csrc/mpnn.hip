@@ -29,7 +29,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
+__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 __device__ inline float silu(float z) { return z * sigm(z); }
 __device__ inline float silu_d(float z) {
   const float s = sigm(z);
@@ -71,9 +71,17 @@ constexpr int gemm_blocks_max() {
   return M * (K + 8) * 2 > 32 * 1024 ? 256 : 2048;
 }
 
-__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) {
-  return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> packed bf16 pair (low = a), round to nearest even (v_cvt_pk_bf16_f32)
+__device__ inline uint32_t pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
+__device__ inline bf16_t bf16_bits(float a) { return __builtin_bit_cast(bf16_t, (__bf16)a); }
+
+__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) { return make_uint2(pk_bf16(a, b), pk_bf16(c, d)); }
 __device__ inline float4 unpack4_bf16(uint2 u) {
   return make_float4(bf2f((bf16_t)(u.x & 0xFFFF)), bf2f((bf16_t)(u.x >> 16)), bf2f((bf16_t)(u.y & 0xFFFF)),
                      bf2f((bf16_t)(u.y >> 16)));
@@ -353,10 +361,17 @@ __device__ inline void unpack8_bf16(const uint4 u, float* f) {
   }
 }
 __device__ inline uint4 pack8_bf16(const float* f) {
-  uint32_t w[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
-  return make_uint4(w[0], w[1], w[2], w[3]);
+  return make_uint4(pk_bf16(f[0], f[1]), pk_bf16(f[2], f[3]), pk_bf16(f[4], f[5]), pk_bf16(f[6], f[7]));
+}
+
+// Edge weights rounded to bf16: the operand precision of the fused layer's
+// aggregation MFMA, used by the training kernels as well so that training
+// and inference evaluate the same function.
+__device__ inline void edge_weights16(const float4 pi, const float4 pj, float& w, float& gx, float& gy) {
+  edge_weights(pi, pj, w, gx, gy);
+  w = bf2f(bf16_bits(w));
+  gx = bf2f(bf16_bits(gx));
+  gy = bf2f(bf16_bits(gy));
 }
 
 // Aggregation forward, one node per lane (consecutive nodes on consecutive
@@ -380,7 +395,7 @@ __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, co
 #pragma unroll
     for (int q = 0; q < kC / 8; q++) uv[q] = u[q];
     float w, gx, gy;
-    edge_weights(pi, pj, w, gx, gy);
+    edge_weights16(pi, pj, w, gx, gy);
 #pragma unroll
     for (int q = 0; q < kC / 8; q++) {
       float f[8];
@@ -407,14 +422,14 @@ __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, co
 //   H' = H + SiLU([H, A] Wo^T + bo)
 //   then U' = H' Wu'^T (OUT == LO_U: the next layer's messages; H' stored) or
 //   Y = H' Wd^T + bd (OUT == LO_DEC: the decoder; H' is not stored).
-// A wave owns 16 nodes.  The aggregation runs in the MFMA operand layout:
-// lane (node lane%16, channel block 8*(lane/16)) sums its 8 channels of the
-// three messages, so the packed sums are the B operands of the three
-// message k-steps without a round trip through memory.  Edges go in batches
-// of 8: each lane loads the 8 neighbour ids of its node and its 16-byte
-// slice of their U rows; the 4 lanes of a node split the 8 edge-weight
-// evaluations (2 each) and exchange them through the wave's LDS stage.
-// Every node sums its edges in edge order, as k_agg_fwd does.  Waves are
+// A wave owns 16 nodes.  The aggregation is a batch of 16 small products on
+// v_mfma_f32_4x4x4_16b_bf16 (block b = lanes 4b..4b+3 = one node): per batch
+// of 4 edges, A = the 3 edge coefficients (w, gx, gy; row 3 zero) x 4 edges
+// and B = 4 edges x 4 channels, one MFMA per channel offset m, where lane
+// 4b + c loads the 16-byte slice (channels 8c..8c+7) of each neighbour's U
+// row and computes the weights of edge c (exchanged through the wave's LDS
+// stage).  The packed sums move to the dense GEMM's operand layout with
+// ds_bpermute and feed its three message k-steps directly.  Waves are
 // persistent and XCD-major: the waves of XCD x walk the x-th eighth of the
 // 16-node tiles.
 enum LayerOut : int { LO_U = 0, LO_DEC = 1 };
@@ -434,6 +449,9 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
   __shared__ __attribute__((aligned(16))) bf16_t wl[M * KP];
   __shared__ __attribute__((aligned(16))) bf16_t ostage[kWaves * 16 * MP];
   __shared__ __attribute__((aligned(16))) bf16_t wpl[PR * KU];
+  // bias in LDS: read per tile rather than held in registers across the tile loop
+  __shared__ __attribute__((aligned(16))) float bl[kH];
+  if (threadIdx.x < kH) bl[threadIdx.x] = bo[threadIdx.x];
   for (int i = threadIdx.x * 8; i < PR * kH; i += kLayerThreads * 8) {
     const int r = i / kH, c = i - r * kH;
     *(uint4*)&wpl[r * KU + c] = *(const uint4*)&Wp[i];
@@ -447,7 +465,7 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
   const int q = lane & 15, kg = lane >> 4;
   const int kq = kg * 8, mq = kg * 4;
   bf16_t* stg = ostage + wave * 16 * MP;
-  float4* ew = (float4*)stg;  // 16 nodes x 8 edges of (w, gx, gy, 0) while aggregating
+  bf16_t* ewt = stg;  // while aggregating: edge weights [16 nodes][4 rows][4 edges] bf16
   const uint32_t ntile = (N + 15) / 16;
   const uint32_t xcd = blockIdx.x % kXcds, bx = blockIdx.x / kXcds, nbx = gridDim.x / kXcds;
   const uint32_t t_lo = (uint32_t)((uint64_t)ntile * xcd / kXcds);
@@ -456,64 +474,96 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
     const uint32_t n0 = tile * 16, n = n0 + q;
     const bool valid = n < N;
     const uint32_t nn = valid ? n : N - 1;  // clamped row for the loads of a partial tile
-    const float4 pi = npos[nn];
-    const uint32_t d = valid ? deg[nn] : 0u;
-    uint32_t dm = d;
+    // --- aggregation: lane = 4 * node + c (node an, channel block / coefficient row c)
+    const int an = lane >> 2, ac = lane & 3;
+    const uint32_t na = min(n0 + an, N - 1);
+    const bool va = n0 + an < N;
+    const float4 pia = *(const float4*)((const char*)npos + (na << 4));
+    const uint32_t da = va ? deg[na] : 0u;
+    uint32_t dm = da;
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) dm = max(dm, (uint32_t)__shfl_xor((int)dm, o, 64));
+    for (int o = 4; o < 64; o <<= 1) dm = max(dm, (uint32_t)__shfl_xor((int)dm, o, 64));
     dm = __builtin_amdgcn_readfirstlane(dm);
-    float a0[8], ax[8], ay[8];
+    f32x4 ag[8];  // ag[m][r]: sum of coefficient r (w, gx, gy, -) times channel 8 c + m
 #pragma unroll
-    for (int c = 0; c < 8; c++) a0[c] = ax[c] = ay[c] = 0.f;
-    for (uint32_t k0 = 0; k0 < dm; k0 += 8) {
-      uint32_t j[8];
+    for (int m = 0; m < 8; m++) ag[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Software pipeline over batches of 4 edges: the neighbour ids of batch
+    // b + 2 and the message slices and positions of batch b + 1 are in flight
+    // while batch b is reduced.  Slots past the node's degree load slot 63
+    // (in bounds) and use the node itself.
+    uint32_t jr[4];  // raw slot contents of the next batch (selected when used)
+    auto load_ids = [&](uint32_t kb) {
 #pragma unroll
-      for (int b = 0; b < 8; b++) j[b] = k0 + b < d ? col[(size_t)(k0 + b) * stride + nn] : nn;
-      uint4 uv[8];
-#pragma unroll
-      for (int b = 0; b < 8; b++) uv[b] = *(const uint4*)(U + (size_t)j[b] * kC + kq);
-      // this lane's two edge weights: batch slots kg and kg + 4
-      uint32_t jA = j[0], jB = j[4];
-      if (kg == 1) {
-        jA = j[1];
-        jB = j[5];
-      } else if (kg == 2) {
-        jA = j[2];
-        jB = j[6];
-      } else if (kg == 3) {
-        jA = j[3];
-        jB = j[7];
+      for (int e = 0; e < 4; e++) {
+        const uint32_t slot = min(kb + e, kMaxDeg - 1);
+        jr[e] = *(const uint32_t*)((const char*)col + ((slot * stride + na) << 2));
       }
-      const float4 pA = npos[jA], pB = npos[jB];
+    };
+    uint4 uv[4];  // channels 8c..8c+7 of the 4 neighbours' messages
+    float4 pc;    // position of the lane's edge (edge c of the batch)
+    auto load_batch = [&](uint32_t kb) {
+      uint32_t j[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) j[e] = kb + e < da ? jr[e] : na;
+#pragma unroll
+      for (int e = 0; e < 4; e++) uv[e] = *(const uint4*)((const char*)U + (j[e] * (kC * 2u) + ac * 16u));
+      const uint32_t jc = ac == 0 ? j[0] : ac == 1 ? j[1] : ac == 2 ? j[2] : j[3];
+      pc = *(const float4*)((const char*)npos + (jc << 4));
+    };
+    if (dm > 0) {
+      load_ids(0);
+      load_batch(0);
+      load_ids(4);
+    }
+    for (uint32_t k0 = 0; k0 < dm; k0 += 4) {
+      const uint4 cu[4] = {uv[0], uv[1], uv[2], uv[3]};
+      const float4 cp = pc;
+      // (unconditional: past the last batch these are in-bounds loads of the node's own row)
+      load_batch(k0 + 4);
+      load_ids(k0 + 8);
+      // weights of edge c, as bf16, into the wave's [node][row][edge] table
       float w, gx, gy;
-      edge_weights(pi, pA, w, gx, gy);
-      ew[q * 8 + kg] = k0 + kg < d ? make_float4(w, gx, gy, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
-      edge_weights(pi, pB, w, gx, gy);
-      ew[q * 8 + kg + 4] = k0 + kg + 4 < d ? make_float4(w, gx, gy, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      edge_weights(pia, cp, w, gx, gy);
+      if (k0 + ac >= da) w = gx = gy = 0.f;
+      ewt[(an * 4 + 0) * 4 + ac] = bf16_bits(w);
+      ewt[(an * 4 + 1) * 4 + ac] = bf16_bits(gx);
+      ewt[(an * 4 + 2) * 4 + ac] = bf16_bits(gy);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
       __builtin_amdgcn_wave_barrier();
+      // A operand: row c = coefficient c over the 4 edges (row 3 zero)
+      const s16x4 ea = ac < 3 ? *(const s16x4*)&ewt[(an * 4 + ac) * 4] : s16x4{0, 0, 0, 0};
+      // B operands: channel 8c + m of the 4 edges; D += A B per node (16 blocks)
+      const uint32_t u0[4] = {cu[0].x, cu[0].y, cu[0].z, cu[0].w}, u1[4] = {cu[1].x, cu[1].y, cu[1].z, cu[1].w};
+      const uint32_t u2[4] = {cu[2].x, cu[2].y, cu[2].z, cu[2].w}, u3[4] = {cu[3].x, cu[3].y, cu[3].z, cu[3].w};
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const float4 e4 = ew[q * 8 + b];
-        float f[8];
-        unpack8_bf16(uv[b], f);
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-          a0[c] += e4.x * f[c];
-          ax[c] += e4.y * f[c];
-          ay[c] += e4.z * f[c];
-        }
+      for (int dw = 0; dw < 4; dw++) {
+        const uint2 lo = make_uint2(__builtin_amdgcn_perm(u1[dw], u0[dw], 0x05040100u),
+                                    __builtin_amdgcn_perm(u3[dw], u2[dw], 0x05040100u));
+        const uint2 hi = make_uint2(__builtin_amdgcn_perm(u1[dw], u0[dw], 0x07060302u),
+                                    __builtin_amdgcn_perm(u3[dw], u2[dw], 0x07060302u));
+        ag[2 * dw] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(ea, __builtin_bit_cast(s16x4, lo), ag[2 * dw], 0, 0, 0);
+        ag[2 * dw + 1] =
+            __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(ea, __builtin_bit_cast(s16x4, hi), ag[2 * dw + 1], 0, 0, 0);
       }
-      __builtin_amdgcn_s_waitcnt(0xc07f);
       __builtin_amdgcn_wave_barrier();
     }
-    bf16x8 xh[kH / 32];
+    bf16x8 xh[kH / 32];  // the GEMM's H operand
 #pragma unroll
     for (int kk = 0; kk < kH / 32; kk++) xh[kk] = *(const bf16x8*)(H + (size_t)nn * kH + kk * 32 + kq);
+    // packed sums, then lane 4 q + g -> lane 16 g + q (the GEMM operand layout)
+    uint32_t pk[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int dw = 0; dw < 4; dw++) pk[r][dw] = pk_bf16(ag[2 * dw][r], ag[2 * dw + 1][r]);
+    const int src = (4 * q + kg) << 2;
     bf16x8 xa[3];
-    xa[0] = __builtin_bit_cast(bf16x8, pack8_bf16(a0));
-    xa[1] = __builtin_bit_cast(bf16x8, pack8_bf16(ax));
-    xa[2] = __builtin_bit_cast(bf16x8, pack8_bf16(ay));
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const uint4 v = make_uint4(__builtin_amdgcn_ds_bpermute(src, pk[r][0]), __builtin_amdgcn_ds_bpermute(src, pk[r][1]),
+                                 __builtin_amdgcn_ds_bpermute(src, pk[r][2]), __builtin_amdgcn_ds_bpermute(src, pk[r][3]));
+      xa[r] = __builtin_bit_cast(bf16x8, v);
+    }
     f32x4 acc[T];
 #pragma unroll
     for (int t = 0; t < T; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -532,7 +582,7 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
     for (int t = 0; t < T; t++) {
       const int m = t * 16 + mq;
       const float4 rr = unpack4_bf16(*(const uint2*)&H[(size_t)nn * kH + m]);
-      const float4 bb = *(const float4*)&bo[m];
+      const float4 bb = *(const float4*)&bl[m];
       const float4 z = make_float4(acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w);
       *(uint2*)&stg[q * MP + m] =
           pack4_bf16(rr.x + silu(z.x), rr.y + silu(z.y), rr.z + silu(z.z), rr.w + silu(z.w));
@@ -597,7 +647,7 @@ __global__ __launch_bounds__(256) void k_agg_bwd(uint32_t n, uint32_t stride, co
     const uint32_t i = col[(size_t)k * stride + j];
     const float4 pi = npos[i];
     float w, gx, gy;
-    edge_weights(pj, pi, w, gx, gy);  // (gx, gy)_ji = -(gx, gy)_ij
+    edge_weights16(pj, pi, w, gx, gy);  // (gx, gy)_ji = -(gx, gy)_ij
     const uint4* da = (const uint4*)(dA16 + (size_t)i * (3 * kC));
 #pragma unroll
     for (int q = 0; q < kC / 8; q++) {
@@ -940,6 +990,10 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
 
 mfree_status mp_graph_reserve(MpGraph& g, uint32_t n, std::string& msg) {
   if (n <= g.cap) return MFREE_OK;
+  if ((uint64_t)n * kMaxDeg * 4 >= (1ull << 32)) {  // the layer kernels address edge slots with 32-bit offsets
+    msg = "learned-step graph limited to 16M nodes";
+    return MFREE_ERR_CAPACITY;
+  }
   mp_graph_free(g);
   auto al = [&](void** p, size_t bytes) -> bool {
     if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
