@@ -230,11 +230,13 @@ __global__ __launch_bounds__(THREADS) void k_gemm(const bf16_t* __restrict__ X, 
         __builtin_amdgcn_wave_barrier();
         constexpr int CH = M / 8;
         const uint32_t n0 = nb + r * 16;
+        if (e.out16) {  // (null: rows only staged, for the fused projection)
 #pragma unroll
-        for (int c = lane; c < 16 * CH; c += 64) {
-          const int row = c / CH, ch = c - row * CH;
-          if (n0 + row < N)
-            *(uint4*)&e.out16[(size_t)(n0 + row) * e.ld16 + ch * 8] = *(const uint4*)&stg[row * MP + ch * 8];
+          for (int c = lane; c < 16 * CH; c += 64) {
+            const int row = c / CH, ch = c - row * CH;
+            if (n0 + row < N)
+              *(uint4*)&e.out16[(size_t)(n0 + row) * e.ld16 + ch * 8] = *(const uint4*)&stg[row * MP + ch * 8];
+          }
         }
         if constexpr (FU) {
           // U of the staged rows: D = Wu H^T (2 tiles of 16 channels, 4 k-steps)
@@ -435,23 +437,37 @@ __global__ __launch_bounds__(256) void k_agg_fwd(uint32_t n, uint32_t stride, co
 enum LayerOut : int { LO_U = 0, LO_DEC = 1 };
 constexpr int kLayerThreads = 1024;
 
-template <int OUT>
+// FIRST: the layer's input H = SiLU(X0 We^T + be) is recomputed from the
+// node features X0 (the encoder kernel stores only U), so the hidden state
+// of the encoder never goes through memory.
+template <int OUT, bool FIRST>
 __global__ __launch_bounds__(kLayerThreads) void k_layer(
     uint32_t N, const bf16_t* __restrict__ H, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ col,
     uint32_t stride, const float4* __restrict__ npos, const bf16_t* __restrict__ U, const bf16_t* __restrict__ Wo,
     const float* __restrict__ bo, const bf16_t* __restrict__ Wp, const float* __restrict__ bp,
-    bf16_t* __restrict__ Hout, bf16_t* __restrict__ Uout, float* __restrict__ Y) {
+    bf16_t* __restrict__ Hout, bf16_t* __restrict__ Uout, float* __restrict__ Y, const bf16_t* __restrict__ X0,
+    const bf16_t* __restrict__ We, const float* __restrict__ be) {
   constexpr int K = kG, M = kH, KP = K + 8, T = M / 16;
   constexpr int kWaves = kLayerThreads / 64;
   constexpr int MP = M + 8;
   constexpr int PR = OUT == LO_U ? kC : kOutP;  // projection rows
   constexpr int KU = kH + 8;
+  constexpr int KE = kK0 + 8;
   __shared__ __attribute__((aligned(16))) bf16_t wl[M * KP];
   __shared__ __attribute__((aligned(16))) bf16_t ostage[kWaves * 16 * MP];
   __shared__ __attribute__((aligned(16))) bf16_t wpl[PR * KU];
-  // bias in LDS: read per tile rather than held in registers across the tile loop
+  __shared__ __attribute__((aligned(16))) bf16_t wel[FIRST ? kH * KE : 8];
+  // biases in LDS: read per tile rather than held in registers across the tile loop
   __shared__ __attribute__((aligned(16))) float bl[kH];
+  __shared__ __attribute__((aligned(16))) float bel[FIRST ? kH : 4];
   if (threadIdx.x < kH) bl[threadIdx.x] = bo[threadIdx.x];
+  if constexpr (FIRST) {
+    if (threadIdx.x < kH) bel[threadIdx.x] = be[threadIdx.x];
+    for (int i = threadIdx.x * 8; i < kH * kK0; i += kLayerThreads * 8) {
+      const int r = i / kK0, c = i - r * kK0;
+      *(uint4*)&wel[r * KE + c] = *(const uint4*)&We[i];
+    }
+  }
   for (int i = threadIdx.x * 8; i < PR * kH; i += kLayerThreads * 8) {
     const int r = i / kH, c = i - r * kH;
     *(uint4*)&wpl[r * KU + c] = *(const uint4*)&Wp[i];
@@ -548,8 +564,26 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
       __builtin_amdgcn_wave_barrier();
     }
     bf16x8 xh[kH / 32];  // the GEMM's H operand
+    if constexpr (FIRST) {
+      // H = SiLU(X0 We^T + be) for this tile, staged as bf16 rows (also the residual)
+      const bf16x8 xb0 = *(const bf16x8*)(X0 + (size_t)nn * kK0 + kq);
 #pragma unroll
-    for (int kk = 0; kk < kH / 32; kk++) xh[kk] = *(const bf16x8*)(H + (size_t)nn * kH + kk * 32 + kq);
+      for (int t = 0; t < T; t++) {
+        const bf16x8 wa = *(const bf16x8*)&wel[(t * 16 + q) * KE + kq];
+        const f32x4 z0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const int m = t * 16 + mq;
+        const float4 bb = *(const float4*)&bel[m];
+        *(uint2*)&stg[q * MP + m] =
+            pack4_bf16(silu(z0[0] + bb.x), silu(z0[1] + bb.y), silu(z0[2] + bb.z), silu(z0[3] + bb.w));
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int kk = 0; kk < kH / 32; kk++) xh[kk] = *(const bf16x8*)&stg[q * MP + kk * 32 + kq];
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < kH / 32; kk++) xh[kk] = *(const bf16x8*)(H + (size_t)nn * kH + kk * 32 + kq);
+    }
     // packed sums, then lane 4 q + g -> lane 16 g + q (the GEMM operand layout)
     uint32_t pk[3][4];
 #pragma unroll
@@ -581,7 +615,7 @@ __global__ __launch_bounds__(kLayerThreads) void k_layer(
 #pragma unroll
     for (int t = 0; t < T; t++) {
       const int m = t * 16 + mq;
-      const float4 rr = unpack4_bf16(*(const uint2*)&H[(size_t)nn * kH + m]);
+      const float4 rr = unpack4_bf16(FIRST ? *(const uint2*)&stg[q * MP + m] : *(const uint2*)&H[(size_t)nn * kH + m]);
       const float4 bb = *(const float4*)&bl[m];
       const float4 z = make_float4(acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w);
       *(uint2*)&stg[q * MP + m] =
@@ -942,24 +976,28 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
   e.ld16 = kH;
   e.wu = net.t[net.Wu[0]].w16;  // U_0 = H_0 Wu_0^T from the staged rows
   e.u_out = a.U[0];
-  gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
   if (!train) {
-    // fused layers; the last one ends in the decoder
+    // the encoder stores only U_0; the first fused layer recomputes H_0 from
+    // X0, the last one ends in the decoder
+    e.out16 = nullptr;
+    gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
+    static_assert(kLayers == 3, "fused inference forward: first, middle and last layer");
     const uint32_t grid = std::min<uint32_t>(256, xcd_grid(blocks(blocks(n, 16), kLayerThreads / 64)));
-    for (int l = 0; l < kLayers; l++) {
-      const bool last = l + 1 == kLayers;
-      const bf16_t* wp = last ? net.t[net.Wd].w16 : net.t[net.Wu[l + 1]].w16;
-      const float* bo = net.w + net.t[net.bo[l]].off;
-      if (last)
-        hipLaunchKernelGGL(k_layer<LO_DEC>, dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[l], g.deg, g.col, g.cap,
-                           g.npos, a.U[l], net.t[net.Wo[l]].w16, bo, wp, net.w + net.t[net.bd].off, nullptr, nullptr,
-                           a.Y);
-      else
-        hipLaunchKernelGGL(k_layer<LO_U>, dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[l], g.deg, g.col, g.cap,
-                           g.npos, a.U[l], net.t[net.Wo[l]].w16, bo, wp, nullptr, a.H16[l + 1], a.U[l + 1], nullptr);
-    }
+    const bf16_t* we = net.t[net.We].w16;
+    const float* be = net.w + net.t[net.be].off;
+    auto bo = [&](int l) { return net.w + net.t[net.bo[l]].off; };
+    hipLaunchKernelGGL((k_layer<LO_U, true>), dim3(grid), dim3(kLayerThreads), 0, st, n, nullptr, g.deg, g.col, g.cap,
+                       g.npos, a.U[0], net.t[net.Wo[0]].w16, bo(0), net.t[net.Wu[1]].w16, nullptr, a.H16[1], a.U[1],
+                       nullptr, a.X0, we, be);
+    hipLaunchKernelGGL((k_layer<LO_U, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[1], g.deg, g.col,
+                       g.cap, g.npos, a.U[1], net.t[net.Wo[1]].w16, bo(1), net.t[net.Wu[2]].w16, nullptr, a.H16[2],
+                       a.U[2], nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_layer<LO_DEC, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[2], g.deg, g.col,
+                       g.cap, g.npos, a.U[2], net.t[net.Wo[2]].w16, bo(2), net.t[net.Wd].w16,
+                       net.w + net.t[net.bd].off, nullptr, nullptr, a.Y, nullptr, nullptr, nullptr);
     return;
   }
+  gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
   for (int l = 0; l < kLayers; l++) {
     // A = aggregate(U)
     hipLaunchKernelGGL(k_agg_fwd, dim3(xcd_grid(blocks(n, 256))), dim3(256), 0, st, n, g.cap, g.deg, g.col, g.npos,

@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
       const double r2 = dx * dx + dy * dy;
       const double hij = 0.5 * (hi + pj.z);
       if (!(r2 < 4.0 * hij * hij)) continue;
+      // the weight sum is an FP32 feature: the shape in single precision
       float f, df;
-      spline_shape((float)(sqrt(r2) / hij), f, df);
+      spline_shape(__builtin_amdgcn_sqrtf((float)r2) * __builtin_amdgcn_rcpf((float)hij), f, df);
       ws += kSplineNorm * f;
       if (d < kMaxDeg) a.col[(size_t)d * a.stride + p] = q;
       d++;
