@@ -91,17 +91,21 @@ def accuracy(scene, nsteps, log):
     """Mixed vs FP64 GPU after nsteps RK4 steps from the same state.
     Returns (summary, FP64 end state)."""
     from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver
-    out = {}
+    out, energy = {}, {}
     for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
         with Solver(precision=prec) as sv:
             sv.add_body(scene.particles, scene.params)
+            e0 = sum(sv.energy(0))
             t0 = time.perf_counter()
             sv.step(nsteps)
             out[name] = sv.state(0)
-            log(f"  accuracy run {name}: {nsteps} steps in {time.perf_counter() - t0:.2f} s")
+            energy[name] = sum(sv.energy(0)) / e0
+            log(f"  accuracy run {name}: {nsteps} steps in {time.perf_counter() - t0:.2f} s, "
+                f"energy E/E0 = {energy[name]:.9f}")
     a, b = out["mixed"], out["fp64"]
     res = dict(steps=nsteps, **errors(a, b),
-               max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()))
+               max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()),
+               energy_ratio=energy)
     return res, b
 
 
@@ -138,10 +142,12 @@ def bench_surrogate(scene, args, ref_state, log):
         with Solver(precision=PRECISION_MIXED) as sv:
             sv.add_body(scene.particles, scene.params)
             sv.step(0)  # commit (autotuner) outside the timed region
+            e0 = sum(sv.energy(0))
             t0 = time.perf_counter()
             st = su.rollout(sv, nsteps, group=args.sur_group, threshold=args.sur_threshold)
             wall = time.perf_counter() - t0
             state = sv.state(0)
+            res["energy_ratio"] = sum(sv.energy(0)) / e0
         res.update(particle_steps_per_s=n * nsteps / wall, rollout_s=wall, steps=nsteps, judge=st)
         log(f"  rollout {nsteps} steps in {wall:.2f} s: {res['particle_steps_per_s']:.4e} particle-steps/s, "
             f"judge {st}")
@@ -226,6 +232,35 @@ def bench_batch(steps, warmup, nscenes, log):
     return ntot, out
 
 
+def bench_plate(steps, warmup, log):
+    """Total-Lagrangian corrected SPH: the tension plate with a hole
+    (0.1 x 0.04 m, r = 0.01 m, dx = 8e-5 m), strip velocity BCs."""
+    from mfree_iwf_amd import METHOD_TL_RL, PRECISION_FP64, PRECISION_MIXED, Solver, scenes
+    t0 = time.perf_counter()
+    pl = scenes.plate_config3()
+    n = pl.particles.n
+    log(f"plate: {n} particles (setup {time.perf_counter() - t0:.1f} s)")
+    out = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        with Solver(method=METHOD_TL_RL, precision=prec) as sv:
+            sv.add_body(pl.particles, pl.params)
+            sv.set_strip_bc(0, pl.left, pl.right, pl.v_bc)
+            sv.step(max(warmup, 1))  # includes the one-time neighbour search and correction
+            sv.profile(True)
+            sv.step(steps)
+            prof = sv.profile_read()
+            sv.profile(False)
+            t0 = time.perf_counter()
+            sv.step(steps)
+            wall_graph = time.perf_counter() - t0
+        out[name] = dict(particle_steps_per_s=n * steps / (prof["step_ms"] / 1e3),
+                         particle_steps_per_s_graph=n * steps / wall_graph,
+                         ms_per_step=prof["step_ms"] / steps)
+        log(f"  {name}: {out[name]['particle_steps_per_s']:.4e} particle-steps/s "
+            f"({out[name]['ms_per_step']:.3f} ms/step), graph {out[name]['particle_steps_per_s_graph']:.4e}")
+    return n, out
+
+
 def traffic_from_profiles():
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     if os.path.exists(p):
@@ -256,7 +291,7 @@ def main():
     ap.add_argument("--sur-batch", type=int, default=1 << 20)
     ap.add_argument("--sur-group", type=int, default=16)
     ap.add_argument("--sur-threshold", type=float, default=1e-4)
-    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256"])
+    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
     args = ap.parse_args()
@@ -282,6 +317,18 @@ def main():
             "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": f"256 ring scenes from seed 0, {ntot} particles, 2k-60k per scene, "
                                    "mixed precision", "particles": ntot, "mode": "mixed"},
+            "modes": out}), flush=True)
+        return
+    if args.workload == "plate578k":
+        n, out = bench_plate(args.steps, args.warmup, log)
+        head = out["mixed"]
+        print(json.dumps({
+            "metric": "particle-steps/s, Total-Lagrangian corrected SPH tension plate",
+            "value": head["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"tension plate 0.1 x 0.04 m, hole r = 0.01 m, dx = 8e-5 m, {n} particles, "
+                                   "strips 3 dx at +-1 m/s, mixed precision", "particles": n, "mode": "mixed"},
             "modes": out}), flush=True)
         return
     t0 = time.perf_counter()

@@ -151,6 +151,14 @@ mfree_status mfree_tl_precompute(mfree_ctx* ctx);
  * access to *particles[i] (src/body.h:196-198, get_particles). */
 mfree_status mfree_set_state(mfree_ctx* ctx, uint32_t body, const mfree_particles* host);
 mfree_status mfree_get_state(mfree_ctx* ctx, uint32_t body, mfree_particles* host);
+
+/* Energy of one body in FP64 (no counterpart in the reference: the path's
+ * energy-conservation check).  energy[0] = kinetic sum m|v|^2/2,
+ * energy[1] = volumetric sum m c0^2 (ln(rho/rho0) + rho0/rho - 1) (the work
+ * of the EOS p = c0^2 (rho - rho0) of src/material.h:37-64), energy[2] =
+ * deviatoric sum (m/rho) S:S/(4G).  Artificial viscosity dissipates, so the
+ * total decays; the check compares its trajectory across precision modes. */
+mfree_status mfree_energy(mfree_ctx* ctx, uint32_t body, double* energy /* 3 */);
 mfree_status mfree_get_time(mfree_ctx* ctx, uint32_t body, double* time);
 
 /* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mfree_set_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_time": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
+    "mfree_energy": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
     "mfree_step": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "mfree_eval_derivatives": (C.c_int32, [C.c_void_p, dptr, dptr]),
     "mfree_get_pairs": (C.c_int32, [C.c_void_p, C.c_uint32, u32ptr, u32ptr, C.c_uint64,

@@ -63,6 +63,7 @@ struct mfree_ctx {
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
+  double* energy_buf = nullptr;   // mfree_energy partials (in allocs)
 
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
 

@@ -96,6 +96,13 @@ class Solver:
         self._chk(self._L.mfree_get_time(self._h, body, C.byref(t)), "mfree_get_time")
         return t.value
 
+    def energy(self, body=0):
+        """FP64 energy of a body: (kinetic, volumetric, deviatoric) in J per
+        unit thickness (see mfree_energy)."""
+        e = np.zeros(3)
+        self._chk(self._L.mfree_energy(self._h, body, e.ctypes.data_as(_capi.dptr)), "mfree_energy")
+        return float(e[0]), float(e[1]), float(e[2])
+
     # -- parity helpers
     def derivatives(self, with_aux=True):
         n = self.n_total
