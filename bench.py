@@ -120,6 +120,10 @@ def bench_surrogate(scene, args, ref_state, log):
     log(f"surrogate: {len(train)} training bodies, {ntrain} particles (setup {time.perf_counter() - t0:.1f} s)")
     res = {}
     with Surrogate(seed=0, max_batch_nodes=args.sur_batch) as su:
+        if args.sur_iters == 0:
+            # kernel timing only: the seeded network, unit feature normalisation
+            su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.full(8, 1e-6)]))
+            return _rollout(su, scene, args, None, res, log)
         with Solver(precision=PRECISION_MIXED) as gen:
             for s in train:
                 gen.add_body(s.particles, s.params)
@@ -137,23 +141,32 @@ def bench_surrogate(scene, args, ref_state, log):
         res["loss_last"] = float(np.mean(loss[-100:]))
         log(f"  trained {args.sur_iters} iterations in {res['train_s']:.1f} s, loss {res['loss_first']:.4f} -> "
             f"{res['loss_last']:.4f}")
-        n = scene.particles.n
-        nsteps = args.accuracy_steps
-        with Solver(precision=PRECISION_MIXED) as sv:
-            sv.add_body(scene.particles, scene.params)
-            sv.step(0)  # commit (autotuner) outside the timed region
-            e0 = sum(sv.energy(0))
-            t0 = time.perf_counter()
-            st = su.rollout(sv, nsteps, group=args.sur_group, threshold=args.sur_threshold)
-            wall = time.perf_counter() - t0
-            state = sv.state(0)
+        return _rollout(su, scene, args, ref_state, res, log)
+
+
+def _rollout(su, scene, args, ref_state, res, log):
+    """The judged rollout of the full workload, timed end to end (the energy
+    check only for a trained network)."""
+    from mfree_iwf_amd import PRECISION_MIXED, Solver
+    n = scene.particles.n
+    nsteps = args.sur_steps if args.sur_steps > 0 else args.accuracy_steps
+    trained = args.sur_iters > 0
+    with Solver(precision=PRECISION_MIXED) as sv:
+        sv.add_body(scene.particles, scene.params)
+        sv.step(0)  # commit (autotuner) outside the timed region
+        e0 = sum(sv.energy(0)) if trained else None
+        t0 = time.perf_counter()
+        st = su.rollout(sv, nsteps, group=args.sur_group, threshold=args.sur_threshold)
+        wall = time.perf_counter() - t0
+        state = sv.state(0)
+        if trained:
             res["energy_ratio"] = sum(sv.energy(0)) / e0
-        res.update(particle_steps_per_s=n * nsteps / wall, rollout_s=wall, steps=nsteps, judge=st)
-        log(f"  rollout {nsteps} steps in {wall:.2f} s: {res['particle_steps_per_s']:.4e} particle-steps/s, "
-            f"judge {st}")
-        if ref_state is not None:
-            res["error_vs_fp64"] = errors(state, ref_state)
-            log(f"  surrogate vs fp64 after {nsteps} steps: {res['error_vs_fp64']}")
+    res.update(particle_steps_per_s=n * nsteps / wall, rollout_s=wall, steps=nsteps, judge=st)
+    log(f"  rollout {nsteps} steps in {wall:.2f} s: {res['particle_steps_per_s']:.4e} particle-steps/s, "
+        f"judge {st}")
+    if ref_state is not None and nsteps == args.accuracy_steps:
+        res["error_vs_fp64"] = errors(state, ref_state)
+        log(f"  surrogate vs fp64 after {nsteps} steps: {res['error_vs_fp64']}")
     return res
 
 
@@ -291,6 +304,8 @@ def main():
     ap.add_argument("--sur-batch", type=int, default=1 << 20)
     ap.add_argument("--sur-group", type=int, default=16)
     ap.add_argument("--sur-threshold", type=float, default=1e-4)
+    ap.add_argument("--sur-steps", type=int, default=0,
+                    help="rollout length (default: the accuracy horizon, --accuracy-steps)")
     ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")

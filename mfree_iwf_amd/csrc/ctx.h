@@ -2,6 +2,7 @@
 // host helpers shared by capi.hip and surrogate.hip.
 #pragma once
 
+#include <atomic>
 #include <string>
 #include <utility>
 #include <vector>
@@ -64,6 +65,13 @@ struct mfree_ctx {
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
   double* energy_buf = nullptr;   // mfree_energy partials (in allocs)
+  // process-unique id: a learned stepper bound to a context compares ids, not
+  // addresses (a new context may reuse a destroyed one's address)
+  uint64_t serial = next_serial();
+  static uint64_t next_serial() {
+    static std::atomic<uint64_t> n{0};
+    return ++n;
+  }
 
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
 

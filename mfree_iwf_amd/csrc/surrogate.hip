@@ -497,7 +497,7 @@ struct mfree_surrogate {
   uint32_t loss_cap = 0;
 
   // per-context scratch
-  const mfree_ctx* bound = nullptr;
+  uint64_t bound = 0;  // serial of the bound context (0: none)
   uint32_t bN = 0;
   std::vector<void*> callocs;
   double *d1 = nullptr, *d2 = nullptr, *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
@@ -559,7 +559,7 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   if (c->opt.method != MFREE_METHOD_UL_MONAGHAN)
     return sfail(s, MFREE_ERR_UNSUPPORTED, "the learned stepper drives UL (Gray-Monaghan) contexts");
   if (c->opt.device != s->opt.device) return sfail(s, MFREE_ERR_ARG, "context and surrogate use different devices");
-  if (s->bound == c && s->bN == c->N) return MFREE_OK;
+  if (s->bound == c->serial && s->bN == c->N) return MFREE_OK;
   SCHK(s, hipDeviceSynchronize());
   drop_graphs(s);
   free_list(s->callocs);
@@ -580,7 +580,7 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   std::string msg;
   if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, false, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
-  s->bound = c;
+  s->bound = c->serial;
   s->bN = c->N;
   s->vprev_valid = false;
   return MFREE_OK;

@@ -228,6 +228,26 @@ def test_judge_always_rejecting_equals_plain_solver():
         assert a.time(0) == pytest.approx(b.time(0), rel=1e-12)
 
 
+def test_rebind_to_a_new_context_of_the_same_size():
+    """A learned stepper that outlives its context rebinds to the next one
+    (contexts are told apart by id, not by address)."""
+    s1 = perturbed_rings(nbox=41, seed=3, dpos=0.2)
+    s2 = perturbed_rings(nbox=41, seed=4, dpos=0.2)
+    assert s1.particles.n == s2.particles.n
+    norm = np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.full(8, 1e-3)])
+    with sg.Surrogate(seed=5) as su:
+        su.set_params(norm=norm)
+        for sc in (s1, s2):
+            with _ctx(sc, PRECISION_MIXED) as sv:
+                su.step(sv, use_net=True)
+                got = sv.state(0).state8()
+            with _ctx(sc, PRECISION_MIXED) as sv, sg.Surrogate(seed=5) as fresh:
+                fresh.set_params(norm=norm)
+                fresh.step(sv, use_net=True)
+                want = sv.state(0).state8()
+            assert np.array_equal(got, want)
+
+
 def test_rollout_accepting_counts_steps():
     s = scenes.rings_config1()
     with _ctx(s, PRECISION_MIXED) as a, sg.Surrogate() as su:
