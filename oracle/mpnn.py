@@ -17,9 +17,13 @@ restates the project's own definitions, in float64 numpy:
 
 Pins (tests/test_mpnn_oracle.py): finite differences of the loss against
 backward(); torch.autograd in float64 against forward() and backward(); a
-per-edge Python loop against the vectorised aggregation; radius_graph's
-weights against the reference-pinned cubic spline of the CPU oracle; the
-graph's symmetry.
+per-edge Python loop against the vectorised aggregation and the adjoint
+identity of its transpose; radius_graph's weights against the
+reference-pinned cubic spline of the CPU oracle; the graph's symmetry;
+base_update / half_state / advance_rho against the exact solutions for
+constant rates (uniformly accelerated motion, exponential compression);
+features against their invariance under a change of units; targets and
+correction as inverse maps.
 """
 import numpy as np
 
@@ -252,11 +256,23 @@ def base_update(state8, d1, d2, tau):
     return n
 
 
+def _field_scales(h, c0, K, rho0):
+    one = np.ones_like(h)
+    return np.stack([h, h, c0 * one, c0 * one, rho0 * one, K * one, K * one, K * one], axis=1)
+
+
+def targets(delta, h, c0, K, rho0):
+    """Training targets: field increments (truth - base update, state-field
+    order) made dimensionless, in network-output order (the library's
+    k_targets)."""
+    t = np.zeros_like(delta)
+    t[:, OUT_OF_FIELD] = delta / _field_scales(h, c0, K, rho0)
+    return t
+
+
 def correction(Y, norm, h, c0, K, rho0):
     """Network outputs (normalised) -> increments of the 8 state fields."""
     tm = norm[2 * NF:2 * NF + NOUT]
     ts = norm[2 * NF + NOUT:]
     r = np.asarray(Y, dtype=np.float64) * ts + tm
-    scale = np.stack([h, h, np.full_like(h, c0), np.full_like(h, c0), np.full_like(h, rho0), np.full_like(h, K),
-                      np.full_like(h, K), np.full_like(h, K)], axis=1)
-    return r[:, OUT_OF_FIELD] * scale
+    return r[:, OUT_OF_FIELD] * _field_scales(h, c0, K, rho0)

@@ -125,3 +125,73 @@ def test_forward_and_gradient_match_torch_autograd():
     assert np.allclose(Yt.detach().numpy(), Y, rtol=1e-12, atol=1e-12)
     assert float(lt.detach()) == pytest.approx(loss, rel=1e-12)
     assert np.allclose(wt.grad.numpy(), g, rtol=1e-9, atol=1e-13)
+
+
+def _random_state(n, rng):
+    s8 = np.zeros((n, 8))
+    s8[:, 0:2] = rng.standard_normal((n, 2))
+    s8[:, 2:4] = rng.standard_normal((n, 2))
+    s8[:, 4] = 1000.0 * (1.0 + 0.01 * rng.standard_normal(n))
+    s8[:, 5:8] = 1e5 * rng.standard_normal((n, 3))
+    return s8
+
+
+def test_base_update_is_exact_for_constant_rates():
+    """Uniformly accelerated motion, constant stress rate, constant velocity
+    divergence: the base update is the exact solution over tau."""
+    rng = np.random.default_rng(21)
+    n, tau = 50, 3e-3
+    s8 = _random_state(n, rng)
+    a = rng.standard_normal((n, 2))
+    Sdot = 1e6 * rng.standard_normal((n, 3))
+    div = 5.0 * rng.standard_normal(n)  # drho/dt = -rho div v
+    d1 = np.zeros((n, 8))
+    d1[:, 0:2] = s8[:, 2:4]
+    d1[:, 2:4] = a
+    d1[:, 4] = -s8[:, 4] * div
+    d1[:, 5:8] = Sdot
+    d2 = d1.copy()  # constant rates: the same at the half state
+    new = om.base_update(s8, d1, d2, tau)
+    np.testing.assert_allclose(new[:, 0:2], s8[:, 0:2] + tau * s8[:, 2:4] + 0.5 * tau * tau * a, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(new[:, 2:4], s8[:, 2:4] + tau * a, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(new[:, 4], s8[:, 4] * np.exp(-div * tau), rtol=1e-14)
+    np.testing.assert_allclose(new[:, 5:8], s8[:, 5:8] + tau * Sdot, rtol=1e-14)
+    half = om.half_state(s8, d1, tau)
+    np.testing.assert_array_equal(half[:, 0:4], s8[:, 0:4])
+    np.testing.assert_array_equal(half[:, 4:8], new[:, 4:8])
+
+
+def test_features_are_unit_invariant():
+    """Every feature is dimensionless: rescaling length, time, mass-density
+    and stress units leaves the 16 features unchanged (a feature normalised
+    by the wrong quantity would not be)."""
+    rng = np.random.default_rng(22)
+    n = 40
+    s8 = _random_state(n, rng)
+    d1 = rng.standard_normal((n, 8)) * np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
+    d2 = rng.standard_normal((n, 8)) * np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
+    vprev = s8[:, 2:4] + 0.01 * rng.standard_normal((n, 2))
+    h = 1e-3 * (1.0 + 0.1 * rng.random(n))
+    wsum = rng.random(n)
+    tau, c0, K, rho0 = 4e-7, 128.0, 1.6e7, 1010.0
+    f = om.features(s8, d1, d2, vprev, h, wsum, tau, c0, K, rho0)
+    L, T, R, P = 3.7, 0.29, 2.3, 11.0  # length, time, density, stress units
+    sc = np.array([L, L, L / T, L / T, R, P, P, P])   # state fields
+    rt = sc / T                                       # their rates
+    g = om.features(s8 * sc, d1 * rt, d2 * rt, vprev * L / T, h * L, wsum, tau * T, c0 * L / T, K * P, rho0 * R)
+    np.testing.assert_allclose(g, f, rtol=1e-12, atol=1e-15)
+
+
+def test_targets_and_correction_are_inverse():
+    rng = np.random.default_rng(23)
+    n = 30
+    delta = rng.standard_normal((n, 8)) * np.array([1e-5, 1e-5, 0.1, 0.1, 1.0, 1e4, 1e4, 1e4])
+    h = 1e-3 * (1.0 + rng.random(n))
+    c0, K, rho0 = 128.0, 1.6e7, 1010.0
+    norm = np.concatenate([np.zeros(om.NF), np.ones(om.NF), 0.1 * rng.standard_normal(8), 1.0 + rng.random(8)])
+    t = om.targets(delta, h, c0, K, rho0)
+    Y = (t - norm[2 * om.NF:2 * om.NF + 8]) / norm[2 * om.NF + 8:]
+    np.testing.assert_allclose(om.correction(Y, norm, h, c0, K, rho0), delta, rtol=1e-10, atol=1e-20)
+    # the position increments are in units of h, the stresses in units of K
+    assert t[:, 0] == pytest.approx(delta[:, 0] / h)
+    assert t[:, om.OUT_OF_FIELD[5]] == pytest.approx(delta[:, 5] / K)
