@@ -112,9 +112,13 @@ __device__ inline Nb nb_from_rec(const PairRec32& r) {
 // All four pair sums of one neighbour at separation (xij, yij), r2 = |x_ij|^2.
 // ACC: only the sums the acceleration needs (stress divergence, artificial
 // viscosity); the velocity gradient and XSPH sums stay zero.
+// Square roots and reciprocals are the hardware's single instructions
+// (v_sqrt_f32, v_rcp_f32: about 1 ulp) rather than the correctly rounded
+// sequences; each term then carries about one rounding more, well inside the
+// compensated sums' error.
 template <bool ACC>
 __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
-  const float rij = sqrtf(r2);
+  const float rij = __builtin_amdgcn_sqrtf(r2);
   const float q = rij * me.h1;
   if (q >= 2.f) return;
   float w, der;
@@ -126,7 +130,7 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
     w = me.fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
     der = -3.0f * q * (1.f - 0.75f * q) * me.h1;
   }
-  const float gsc = rij > 1e-12f ? der * me.fac / rij : 0.f;
+  const float gsc = rij > 1e-12f ? der * me.fac * __builtin_amdgcn_rcpf(rij) : 0.f;
   const float wx = xij * gsc, wy = yij * gsc;
   float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
   if (me.use_as) {
@@ -149,18 +153,17 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
     acc_add(S.vy_y, dvy * wy * nj.qw);
   }
   const float vp = -(dvx * xij + dvy * yij);
+  const float irhoij = __builtin_amdgcn_rcpf(0.5f * (me.rho + nj.rho));
   if (!(vp >= 0.f)) {
     const float cij = 0.5f * (me.c + nj.c);
     const float hij = 0.5f * (me.h + nj.h);
-    const float rhoij = 0.5f * (me.rho + nj.rho);
-    const float muij = (hij * vp) / (r2 + me.eta2 * hij * hij);
-    const float piij = (-me.alpha * cij * muij + me.beta * muij * muij) / rhoij;
+    const float muij = (hij * vp) * __builtin_amdgcn_rcpf(r2 + me.eta2 * hij * hij);
+    const float piij = (-me.alpha * cij * muij + me.beta * muij * muij) * irhoij;
     acc_add(S.ax, -mj * piij * wx);
     acc_add(S.ay, -mj * piij * wy);
   }
   if constexpr (!ACC) {
-    const float rhoij = 0.5f * (me.rho + nj.rho);
-    const float fx = -me.xeps_mi * w / rhoij;
+    const float fx = -me.xeps_mi * w * irhoij;
     acc_add(S.xs, -fx * dvx);
     acc_add(S.ys, -fx * dvy);
   }
