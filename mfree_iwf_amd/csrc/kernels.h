@@ -68,7 +68,7 @@ struct UlArgs {
   Fields8 acc;                  // RK accumulator (storage order)
   double* d8;                   // stage 0: derivatives by global id (by sorted position if d8_sorted)
   int d8_sorted;
-  int accel_only;               // stage 0, mixed: only d8[2], d8[3] are needed (other rates incomplete)
+  int accel_only;               // stage 0: only the acceleration, as 2 doubles per particle in d8
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;

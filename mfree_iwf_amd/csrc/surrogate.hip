@@ -205,12 +205,12 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
 #pragma unroll
   for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
   const double* d1 = a.d1 + 8 * (size_t)p;
-  const double* d2 = a.d2 + 8 * (size_t)p;
+  const double* d2 = a.d2 + 2 * (size_t)p;  // acceleration at the half state
   const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
   if (a.F || a.X0) {
     float f[kNF];
-    f[0] = (float)(tau * d2[2] / c0);
-    f[1] = (float)(tau * d2[3] / c0);
+    f[0] = (float)(tau * d2[0] / c0);
+    f[1] = (float)(tau * d2[1] / c0);
     f[2] = (float)(tau * d1[5] / K);
     f[3] = (float)(tau * d1[6] / K);
     f[4] = (float)(tau * d1[7] / K);
@@ -239,10 +239,10 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   }
   // base update
   double nw[8];
-  nw[0] = st[0] + tau * d1[0] + 0.5 * tau * tau * d2[2];
-  nw[1] = st[1] + tau * d1[1] + 0.5 * tau * tau * d2[3];
-  nw[2] = st[2] + tau * d2[2];
-  nw[3] = st[3] + tau * d2[3];
+  nw[0] = st[0] + tau * d1[0] + 0.5 * tau * tau * d2[0];
+  nw[1] = st[1] + tau * d1[1] + 0.5 * tau * tau * d2[1];
+  nw[2] = st[2] + tau * d2[0];
+  nw[3] = st[3] + tau * d2[1];
   nw[4] = advance_rho(st[4], d1[4], tau);
   nw[5] = st[5] + tau * d1[5];
   nw[6] = st[6] + tau * d1[6];
@@ -565,7 +565,7 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   free_list(s->callocs);
   const size_t N = c->N;
   SCHK(s, salloc(s->callocs, &s->d1, 8 * N));
-  SCHK(s, salloc(s->callocs, &s->d2, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->d2, 2 * N));
   SCHK(s, salloc(s->callocs, &s->base, 8 * N));
   SCHK(s, salloc(s->callocs, &s->vprev, 2 * N));
   SCHK(s, salloc(s->callocs, &s->cand, 2 * N));
