@@ -72,6 +72,13 @@ struct UlArgs {
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;
+  // learned-step graph emitted by the mixed per-particle pass (stage 0, not
+  // accel_only, bodies of uniform h): edges r < 2h over sorted positions
+  uint32_t* gcol;               // slot-major edge list (null: no graph)
+  uint32_t* gdeg;
+  float* gwsum;
+  const double4* gdpos;         // FP64 (x, y, h) by sorted position (predicate near the cutoff)
+  uint32_t gstride;
   // LDS-tiled pair pass (mixed precision)
   uint32_t nscene;
   uint32_t tile_cells;          // 0: per-particle kernel without tiles

@@ -98,9 +98,10 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
       if (q == p) continue;
       const double4 pj = a.dpos[q];
       const double dx = xi - pj.x, dy = yi - pj.y;
-      const double r2 = dx * dx + dy * dy;
+      // no FMA contraction: the predicate rounds like the FP64 oracle's
+      const double r2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
       const double hij = 0.5 * (hi + pj.z);
-      if (!(r2 < 4.0 * hij * hij)) continue;
+      if (!(r2 < __dmul_rn(__dmul_rn(4.0, hij), hij))) continue;
       // the weight sum is an FP32 feature: the shape in single precision
       float f, df;
       spline_shape(__builtin_amdgcn_sqrtf((float)r2) * __builtin_amdgcn_rcpf((float)hij), f, df);
@@ -117,7 +118,9 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   a.wsum[p] = ws;
 }
 
-void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
+// Node records, then (edges) the edge lists; without edges the mixed D1
+// pass emits them.
+void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg, bool edges = true) {
   g.stride = mg.cap;
   g.deg = mg.deg;
   g.col = mg.col;
@@ -125,7 +128,7 @@ void launch_graph(hipStream_t st, GraphArgs g, MpGraph& mg) {
   g.dpos = mg.dpos;
   g.wsum = mg.wsum;
   hipLaunchKernelGGL(k_graph_nodes, dim3(nblk(g.n, 256)), dim3(256), 0, st, g);
-  hipLaunchKernelGGL(k_graph, dim3(xcd_grid(nblk(g.n, 256))), dim3(256), 0, st, g);
+  if (edges) hipLaunchKernelGGL(k_graph, dim3(xcd_grid(nblk(g.n, 256))), dim3(256), 0, st, g);
 }
 
 // Edge features of every edge in CSR order (export for checks).
@@ -499,6 +502,7 @@ struct mfree_surrogate {
   // per-context scratch
   uint64_t bound = 0;  // serial of the bound context (0: none)
   uint32_t bN = 0;
+  bool uniform_h = false;  // every body of the bound context has one smoothing length
   std::vector<void*> callocs;
   double *d1 = nullptr, *d2 = nullptr, *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
   uint32_t* inv = nullptr;
@@ -580,6 +584,15 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   std::string msg;
   if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, false, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
+  {
+    // h is constant over a run: one check at binding
+    std::vector<double> hh(N);
+    SCHK(s, hipMemcpy(hh.data(), c->h[c->cur], N * sizeof(double), hipMemcpyDeviceToHost));
+    bool u = true;
+    for (size_t b = 0; b < c->S && u; b++)
+      for (uint32_t k = c->off[b] + 1; k < c->off[b + 1] && u; k++) u = hh[k] == hh[c->off[b]];
+    s->uniform_h = u;
+  }
   s->bound = c->serial;
   s->bN = c->N;
   s->vprev_valid = false;
@@ -615,16 +628,29 @@ void record_vprev(mfree_surrogate* s, mfree_ctx* c) {
 
 // Steps 1-2 of a learned step: cell list of the current state, D1, the
 // half state, D2.
-void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
+void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges = true);
+
+// Binning, D1 at the state and D2 at the half state.  fused_graph: the node
+// records first and the edge lists from the mixed D1 pass (bodies of
+// uniform h).
+void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c, bool fused_graph = false) {
   hipStream_t st = c->stream;
   const int b = c->cur;
   launch_binning(st, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+  if (fused_graph) enqueue_graph(s, c, false);
   UlArgs u = ul_args(c, b);
   u.src = c->y[b];
   launch_ul_prepass(st, u);
   u.stage = 0;
   u.d8 = s->d1;
   u.d8_sorted = 1;
+  if (fused_graph) {
+    u.gcol = s->graph.col;
+    u.gdeg = s->graph.deg;
+    u.gwsum = s->graph.wsum;
+    u.gdpos = s->graph.dpos;
+    u.gstride = s->graph.cap;
+  }
   launch_ul_pairs(st, u);
   StepArgs a = step_args(s, c);
   hipLaunchKernelGGL(k_half_state, dim3(nblk(c->N, 256)), dim3(256), 0, st, a);
@@ -638,7 +664,7 @@ void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c) {
   launch_ul_pairs(st, v);
 }
 
-void enqueue_graph(mfree_surrogate* s, mfree_ctx* c) {
+void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges) {
   const int b = c->cur;
   GraphArgs g;
   memset(&g, 0, sizeof(g));
@@ -653,7 +679,7 @@ void enqueue_graph(mfree_surrogate* s, mfree_ctx* c) {
   g.y = c->y[b].f[1];
   g.h = c->h[b];
   g.err = c->errb;
-  launch_graph(c->stream, g, s->graph);
+  launch_graph(c->stream, g, s->graph, edges);
 }
 
 void advance_time_device(mfree_ctx* c, uint32_t nsteps) {
@@ -668,10 +694,11 @@ void advance_time_host(mfree_ctx* c, uint32_t nsteps) {
 
 // Device work of one learned step of every body (capturable: kernels only).
 void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
-  enqueue_derivs(s, c);
+  const bool fused = use_net && c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h;
+  enqueue_derivs(s, c, fused);
   StepArgs a = step_args(s, c);
   if (use_net) {
-    enqueue_graph(s, c);
+    if (!fused) enqueue_graph(s, c);
     a.X0 = s->acts.X0;
     // features first (they read the state), then the network, then the update
     StepArgs f = a;
@@ -1324,8 +1351,12 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
   mfree_status st = bind_ctx(s, c);
   if (st != MFREE_OK) return st;
   const int b = c->cur;
-  launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
-  enqueue_graph(s, c);
+  if (c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h) {
+    enqueue_derivs(s, c, true);  // as in a learned step: the D1 pass emits the edges
+  } else {
+    launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+    enqueue_graph(s, c);
+  }
   st = sync_ctx(s, c);
   if (st != MFREE_OK) return st;
   const size_t N = c->N;

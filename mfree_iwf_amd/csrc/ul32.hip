@@ -300,12 +300,14 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
 }  // namespace
 
 constexpr int kNbCap = 32;  // compacted neighbours buffered per thread
+constexpr uint32_t kMaxGraphDeg = 64;                     // learned-step graph: edge slots per node
+constexpr float kSplineNormF = 0.45472840883398667f;      // 10/(7 pi): 2D cubic spline factor times h^2
 
 // Per particle from global memory: scan the 3x3 block, buffer the candidates
 // inside the cutoff in LDS (sorted position + cell offsets packed in 32
 // bits), then evaluate the pair terms on the compact list with full SIMD
 // occupancy.
-template <bool ACC>
+template <bool ACC, bool GRAPH = false>
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
@@ -323,6 +325,15 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   const Me me = make_me(c, nb_from_rec(a.rec32[i]));
   Sums S;
   int nb = 0;
+  // learned-step graph (D1 of a learned step): the compacted candidates are
+  // a superset of its edges r < 2h (uniform h: h_ij = h); classified in FP32
+  // from the cell-relative positions (about 1e-6 relative on r^2), with the
+  // FP64 predicate of the graph builder within 4e-6 of the cutoff, so the
+  // edge set and order are those of k_graph
+  constexpr bool graph = GRAPH && !ACC;
+  uint32_t gd = 0;
+  float gws = kSplineNormF;
+  const float gcut = 4.f * me.h * me.h;
   auto flush = [&]() {
     for (int k = 0; k < nb; k++) {
       const uint32_t e = my[k * kBlock];
@@ -330,7 +341,23 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
       const float2 pj = a.pos32[j];
       const float xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
       const float yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
-      pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, xij * xij + yij * yij, S);
+      const float r2 = xij * xij + yij * yij;
+      if (graph && !(r2 > gcut * (1.f + 4e-6f))) {
+        bool edge = true;
+        if (r2 >= gcut * (1.f - 4e-6f)) {
+          const double4 di = a.gdpos[i], dj = a.gdpos[j];
+          const double dx = di.x - dj.x, dy = di.y - dj.y, hij = 0.5 * (di.z + dj.z);
+          edge = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)) < __dmul_rn(__dmul_rn(4.0, hij), hij);
+        }
+        if (edge) {
+          const float q = __builtin_amdgcn_sqrtf(r2) * me.h1;
+          const float t = fmaxf(2.f - q, 0.f);
+          gws += q < 1.f ? kSplineNormF * (1.f - 1.5f * q * q * (1.f - 0.5f * q)) : kSplineNormF * 0.25f * t * t * t;
+          if (gd < kMaxGraphDeg) a.gcol[(size_t)gd * a.gstride + i] = j;
+          gd++;
+        }
+      }
+      pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
     }
     nb = 0;
   };
@@ -355,6 +382,14 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
     }
   }
   flush();
+  if (graph) {
+    if (gd > kMaxGraphDeg) {
+      atomicOr(a.err, EB_NBH);
+      gd = kMaxGraphDeg;
+    }
+    a.gdeg[i] = gd;
+    a.gwsum[i] = gws;
+  }
   finish(a, c, i, S);
 }
 
@@ -530,11 +565,13 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
 
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {
   const dim3 grid(xcd_grid((a.n + kBlock - 1) / kBlock));
-  if (a.tile_cells && a.tile_off) {
+  if (a.tile_cells && a.tile_off && !a.gcol) {
     if (a.accel_only)
       hipLaunchKernelGGL(k_ul_tiles32<true>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
     else
       hipLaunchKernelGGL(k_ul_tiles32<false>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
+  } else if (a.gcol && !a.accel_only) {
+    hipLaunchKernelGGL((k_ul_pairs32<false, true>), grid, dim3(kBlock), 0, st, a);
   } else if (a.accel_only) {
     hipLaunchKernelGGL(k_ul_pairs32<true>, grid, dim3(kBlock), 0, st, a);
   } else {

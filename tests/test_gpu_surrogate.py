@@ -56,6 +56,31 @@ def test_graph_matches_oracle():
         assert np.allclose(e, oe, rtol=1e-4, atol=2e-5)
 
 
+@pytest.mark.parametrize("case", ["perturbed", "lattice_ties"])
+def test_graph_from_the_mixed_pair_pass_equals_graph_builder(case):
+    """Mixed contexts of uniform-h bodies take the learned step's edges from
+    the D1 pair pass; they equal the FP64 graph builder's edge for edge, in
+    order, also on a lattice whose neighbours sit exactly at r = 2h."""
+    if case == "perturbed":
+        s = perturbed_rings(nbox=41, seed=6, dpos=0.3)
+    else:
+        s = scenes.make_rings(41, hdx=1.5)  # 2h = 3 dx: lattice distance (3, 0)
+    ps = s.particles
+    print(f"[surrogate fused graph] {case} n={ps.n}", flush=True)
+    out = {}
+    for prec in (PRECISION_FP64, PRECISION_MIXED):
+        with Solver(precision=prec) as sv, sg.Surrogate() as su:
+            sv.add_body(ps, s.params)
+            out[prec] = su.graph(sv)
+    for a, b in zip(out[PRECISION_FP64], out[PRECISION_MIXED]):
+        assert np.array_equal(a, b)
+    orow, ocol, _, _ = om.radius_graph(ps.x, ps.y, ps.h)
+    row, col, _ = out[PRECISION_MIXED]
+    assert np.array_equal(row.astype(np.int64), orow)
+    for k in range(ps.n):
+        assert np.array_equal(np.sort(col[row[k]:row[k + 1]]), np.sort(ocol[orow[k]:orow[k + 1]]))
+
+
 def test_graph_multibody_matches_oracle():
     bodies = [perturbed_rings(nbox=nb, seed=20 + k, dpos=0.3) for k, nb in enumerate((21, 41, 31))]
     print(f"[surrogate graph] bodies n={[b.particles.n for b in bodies]}", flush=True)
