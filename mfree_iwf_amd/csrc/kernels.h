@@ -109,6 +109,7 @@ struct PairListArgs {
 void launch_pair_list(hipStream_t st, const PairListArgs& a);
 
 __global__ void k_advance_time(uint32_t nscene, SceneConst* sc);
+__global__ void k_advance_time_n(uint32_t nscene, SceneConst* sc, uint32_t nsteps);
 
 void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n);
 

@@ -683,8 +683,7 @@ void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges) {
 }
 
 void advance_time_device(mfree_ctx* c, uint32_t nsteps) {
-  for (uint32_t k = 0; k < nsteps; k++)
-    hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc);
+  hipLaunchKernelGGL(k_advance_time_n, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc, nsteps);
 }
 
 void advance_time_host(mfree_ctx* c, uint32_t nsteps) {

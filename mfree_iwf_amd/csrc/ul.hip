@@ -413,6 +413,16 @@ __global__ void k_advance_time(uint32_t nscene, SceneConst* sc) {
   if (s < nscene) sc[s].time = sc[s].time + sc[s].dt;
 }
 
+// nsteps additions of dt, one after the other (the same rounding as nsteps
+// single advances)
+__global__ void k_advance_time_n(uint32_t nscene, SceneConst* sc, uint32_t nsteps) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscene) return;
+  double t = sc[s].time;
+  for (uint32_t k = 0; k < nsteps; k++) t = t + sc[s].dt;
+  sc[s].time = t;
+}
+
 // The reference's Verlet list of one body (src/grid.h:117-177): count pass
 // (fill = 0) and fill pass (fill = 1), in idx space.
 __global__ __launch_bounds__(kBlock) void k_pair_list(PairListArgs a) {
