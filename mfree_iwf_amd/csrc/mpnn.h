@@ -56,6 +56,22 @@ __device__ __host__ inline float bf2f(bf16_t b) {
   return f;
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigm(z); }
+
+// two floats -> packed bf16 pair (low = a), round to nearest even (v_cvt_pk_bf16_f32)
+__device__ inline uint32_t pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+__device__ inline bf16_t bf16_bits(float a) { return __builtin_bit_cast(bf16_t, (__bf16)a); }
+__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) { return make_uint2(pk_bf16(a, b), pk_bf16(c, d)); }
+
 // cubic-spline shape f(q) and f'(q) (src/kernel.cpp:20-55 without the
 // 10/(7 pi h^2) factor)
 __device__ inline void spline_shape(float q, float& f, float& df) {
@@ -157,6 +173,10 @@ void mp_acts_free(MpActs& a);
 // is the BF16 copy of H); with train == true the FP32 hidden states and the
 // pre-activations the backward needs are stored too.
 void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st);
+// The inference forward after the encoder: X0 and U_0 given (a fused
+// features + encoder kernel wrote them); the three fused layers and the
+// decoder -> a.Y.
+void mp_forward_layers(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, hipStream_t st);
 // Loss + dY from targets T (n x 8, normalised) -> accumulates weight
 // gradients into net.g (zeroed by the caller).  loss_out: device float.
 void mp_backward(MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, const float* T, float* loss_out,

@@ -26,11 +26,6 @@ namespace mfree {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
-__device__ inline float silu(float z) { return z * sigm(z); }
 __device__ inline float silu_d(float z) {
   const float s = sigm(z);
   return s * (1.f + z * (1.f - s));
@@ -71,17 +66,6 @@ constexpr int gemm_blocks_max() {
   return M * (K + 8) * 2 > 32 * 1024 ? 256 : 2048;
 }
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-
-// two floats -> packed bf16 pair (low = a), round to nearest even (v_cvt_pk_bf16_f32)
-__device__ inline uint32_t pk_bf16(float a, float b) {
-  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
-}
-__device__ inline bf16_t bf16_bits(float a) { return __builtin_bit_cast(bf16_t, (__bf16)a); }
-
-__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) { return make_uint2(pk_bf16(a, b), pk_bf16(c, d)); }
 __device__ inline float4 unpack4_bf16(uint2 u) {
   return make_float4(bf2f((bf16_t)(u.x & 0xFFFF)), bf2f((bf16_t)(u.x >> 16)), bf2f((bf16_t)(u.y & 0xFFFF)),
                      bf2f((bf16_t)(u.y >> 16)));
@@ -964,6 +948,24 @@ void mp_acts_free(MpActs& a) {
   a.part = nullptr;
 }
 
+void mp_forward_layers(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, hipStream_t st) {
+  if (n == 0) return;
+  static_assert(kLayers == 3, "fused inference forward: first, middle and last layer");
+  const uint32_t grid = std::min<uint32_t>(256, xcd_grid(blocks(blocks(n, 16), kLayerThreads / 64)));
+  const bf16_t* we = net.t[net.We].w16;
+  const float* be = net.w + net.t[net.be].off;
+  auto bo = [&](int l) { return net.w + net.t[net.bo[l]].off; };
+  hipLaunchKernelGGL((k_layer<LO_U, true>), dim3(grid), dim3(kLayerThreads), 0, st, n, nullptr, g.deg, g.col, g.cap,
+                     g.npos, a.U[0], net.t[net.Wo[0]].w16, bo(0), net.t[net.Wu[1]].w16, nullptr, a.H16[1], a.U[1],
+                     nullptr, a.X0, we, be);
+  hipLaunchKernelGGL((k_layer<LO_U, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[1], g.deg, g.col,
+                     g.cap, g.npos, a.U[1], net.t[net.Wo[1]].w16, bo(1), net.t[net.Wu[2]].w16, nullptr, a.H16[2],
+                     a.U[2], nullptr, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL((k_layer<LO_DEC, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[2], g.deg, g.col,
+                     g.cap, g.npos, a.U[2], net.t[net.Wo[2]].w16, bo(2), net.t[net.Wd].w16,
+                     net.w + net.t[net.bd].off, nullptr, nullptr, a.Y, nullptr, nullptr, nullptr);
+}
+
 void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool train, hipStream_t st) {
   if (n == 0) return;
   // encoder: H0 = SiLU(X0 We^T + be)
@@ -981,20 +983,7 @@ void mp_forward(const MpNet& net, MpActs& a, const MpGraph& g, uint32_t n, bool 
     // X0, the last one ends in the decoder
     e.out16 = nullptr;
     gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);
-    static_assert(kLayers == 3, "fused inference forward: first, middle and last layer");
-    const uint32_t grid = std::min<uint32_t>(256, xcd_grid(blocks(blocks(n, 16), kLayerThreads / 64)));
-    const bf16_t* we = net.t[net.We].w16;
-    const float* be = net.w + net.t[net.be].off;
-    auto bo = [&](int l) { return net.w + net.t[net.bo[l]].off; };
-    hipLaunchKernelGGL((k_layer<LO_U, true>), dim3(grid), dim3(kLayerThreads), 0, st, n, nullptr, g.deg, g.col, g.cap,
-                       g.npos, a.U[0], net.t[net.Wo[0]].w16, bo(0), net.t[net.Wu[1]].w16, nullptr, a.H16[1], a.U[1],
-                       nullptr, a.X0, we, be);
-    hipLaunchKernelGGL((k_layer<LO_U, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[1], g.deg, g.col,
-                       g.cap, g.npos, a.U[1], net.t[net.Wo[1]].w16, bo(1), net.t[net.Wu[2]].w16, nullptr, a.H16[2],
-                       a.U[2], nullptr, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL((k_layer<LO_DEC, false>), dim3(grid), dim3(kLayerThreads), 0, st, n, a.H16[2], g.deg, g.col,
-                       g.cap, g.npos, a.U[2], net.t[net.Wo[2]].w16, bo(2), net.t[net.Wd].w16,
-                       net.w + net.t[net.bd].off, nullptr, nullptr, a.Y, nullptr, nullptr, nullptr);
+    mp_forward_layers(net, a, g, n, st);
     return;
   }
   gemm<kK0, kH>(a.X0, kK0, net.t[net.We].w16, n, e, st);

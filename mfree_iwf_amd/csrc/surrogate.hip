@@ -156,7 +156,7 @@ struct StepArgs {
   const SceneConst* sc;
   Fields8 y;                           // state, storage order
   const double* h;                     // storage order
-  const double *d1, *d2;               // derivatives by sorted position (8 per particle)
+  const double *d1, *d2;               // by sorted position: D1 (8 per particle), D2 acceleration (2)
   double* vprev;                       // by global id (2 per particle)
   const float* wsum;                   // by sorted position
   const float* norm;                   // kNorm floats (device), or null
@@ -194,6 +194,40 @@ __global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
   for (int k = 5; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
 }
 
+// The 16 raw node features of sorted position p (state field order
+// x, y, vx, vy, rho, Sxx, Sxy, Syy):
+//   tau a2 / c0 (2), tau dS1 / K (3), tau drho1 / rho0, S / K (3), rho/rho0 - 1,
+//   (v - v_prev) / c0 (2), tau (xdot1 - v) / h (2), c0 tau / h, kernel-weight sum.
+__device__ inline void node_features(const StepArgs& a, uint32_t p, float* f) {
+  const uint32_t s = a.perm[p];
+  const uint32_t g = a.gid[s];
+  const SceneConst& c = a.sc[a.scene[s]];
+  const double tau = tau_of(c);
+  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[s];
+  double st[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
+  const double* d1 = a.d1 + 8 * (size_t)p;
+  const double* d2 = a.d2 + 2 * (size_t)p;  // acceleration at the half state
+  const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
+  f[0] = (float)(tau * d2[0] / c0);
+  f[1] = (float)(tau * d2[1] / c0);
+  f[2] = (float)(tau * d1[5] / K);
+  f[3] = (float)(tau * d1[6] / K);
+  f[4] = (float)(tau * d1[7] / K);
+  f[5] = (float)(tau * d1[4] / rho0);
+  f[6] = (float)(st[5] / K);
+  f[7] = (float)(st[6] / K);
+  f[8] = (float)(st[7] / K);
+  f[9] = (float)(st[4] / rho0 - 1.0);
+  f[10] = (float)((st[2] - vpx) / c0);
+  f[11] = (float)((st[3] - vpy) / c0);
+  f[12] = (float)(tau * (d1[0] - st[2]) / h);
+  f[13] = (float)(tau * (d1[1] - st[3]) / h);
+  f[14] = (float)(c0 * tau / h);
+  f[15] = a.wsum[p];
+}
+
 // One thread per sorted position: features, base update, optional network
 // correction.
 __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
@@ -209,25 +243,9 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
   const double* d1 = a.d1 + 8 * (size_t)p;
   const double* d2 = a.d2 + 2 * (size_t)p;  // acceleration at the half state
-  const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
   if (a.F || a.X0) {
     float f[kNF];
-    f[0] = (float)(tau * d2[0] / c0);
-    f[1] = (float)(tau * d2[1] / c0);
-    f[2] = (float)(tau * d1[5] / K);
-    f[3] = (float)(tau * d1[6] / K);
-    f[4] = (float)(tau * d1[7] / K);
-    f[5] = (float)(tau * d1[4] / rho0);
-    f[6] = (float)(st[5] / K);
-    f[7] = (float)(st[6] / K);
-    f[8] = (float)(st[7] / K);
-    f[9] = (float)(st[4] / rho0 - 1.0);
-    f[10] = (float)((st[2] - vpx) / c0);
-    f[11] = (float)((st[3] - vpy) / c0);
-    f[12] = (float)(tau * (d1[0] - st[2]) / h);
-    f[13] = (float)(tau * (d1[1] - st[3]) / h);
-    f[14] = (float)(c0 * tau / h);
-    f[15] = a.wsum[p];
+    node_features(a, p, f);
     if (a.F) {
 #pragma unroll
       for (int k = 0; k < kNF; k++) a.F[(size_t)p * kNF + k] = f[k];
@@ -279,6 +297,100 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   if (a.apply) {
 #pragma unroll
     for (int k = 0; k < 8; k++) a.y.f[k][s] = nw[k];
+  }
+}
+
+// Inference: features and the encoder's messages in one pass.  A wave takes
+// 64 sorted positions (one per lane): it writes their normalised features
+// (X0, which the first fused layer reads again) and stages them in LDS, then,
+// per 16 nodes, H_0 = SiLU(X0 We^T + be) on v_mfma_f32_16x16x32_bf16
+// (D = We X0^T: lane = node lane%16, channels 16t + 4(lane/16)..+3), H_0 rows
+// staged as bf16, and U_0 = H_0 Wu_0^T, the only output of the encoder.
+constexpr int kFeThreads = 256;
+__global__ __launch_bounds__(kFeThreads) void k_features_encode(StepArgs a, const bf16_t* __restrict__ We,
+                                                                const float* __restrict__ be,
+                                                                const bf16_t* __restrict__ Wu, bf16_t* __restrict__ U0) {
+  // X0 has kNF = 16 features and 16 zero columns: the MFMA's k = 16..31 half
+  // is zero, so only the first 16 columns of X0 and We are staged
+  constexpr int KE = kNF + 8, MP = kH + 8, KU = kH + 8, T = kH / 16;
+  constexpr int kWaves = kFeThreads / 64;
+  __shared__ __attribute__((aligned(16))) bf16_t wel[kH * KE];
+  __shared__ __attribute__((aligned(16))) bf16_t wul[kC * KU];
+  __shared__ __attribute__((aligned(16))) float bel[kH];
+  __shared__ __attribute__((aligned(16))) bf16_t xt[kWaves * 64 * KE];
+  __shared__ __attribute__((aligned(16))) bf16_t stage[kWaves * 16 * MP];
+  for (int i = threadIdx.x; i < kH * 2; i += kFeThreads) {  // 2 chunks of 8 of every We row
+    const int r = i >> 1, c = (i & 1) * 8;
+    *(uint4*)&wel[r * KE + c] = *(const uint4*)&We[r * kK0 + c];
+  }
+  for (int i = threadIdx.x * 8; i < kC * kH; i += kFeThreads * 8) {
+    const int r = i / kH, c = i - r * kH;
+    *(uint4*)&wul[r * KU + c] = *(const uint4*)&Wu[i];
+  }
+  if (threadIdx.x < kH) bel[threadIdx.x] = be[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane & 15, kq = (lane >> 4) * 8, mq = (lane >> 4) * 4;
+  bf16_t* xw = xt + wave * 64 * KE;
+  bf16_t* stg = stage + wave * 16 * MP;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  for (uint32_t base = (blockIdx.x * kWaves + wave) * 64u; base < a.n; base += gridDim.x * kWaves * 64u) {
+    const uint32_t p = base + lane;
+    uint4 row[2] = {zero4, zero4};
+    if (p < a.n) {
+      float f[kNF];
+      node_features(a, p, f);
+#pragma unroll
+      for (int k = 0; k < kNF; k++) f[k] = (f[k] - a.norm[k]) / a.norm[kNF + k];
+#pragma unroll
+      for (int k = 0; k < 2; k++)
+        row[k] = make_uint4(pk_bf16(f[8 * k], f[8 * k + 1]), pk_bf16(f[8 * k + 2], f[8 * k + 3]),
+                            pk_bf16(f[8 * k + 4], f[8 * k + 5]), pk_bf16(f[8 * k + 6], f[8 * k + 7]));
+      uint4* xr = (uint4*)&a.X0[(size_t)p * kK0];
+      xr[0] = row[0];
+      xr[1] = row[1];
+      xr[2] = zero4;
+      xr[3] = zero4;
+    }
+    *(uint4*)&xw[lane * KE] = row[0];
+    *(uint4*)&xw[lane * KE + 8] = row[1];
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+    for (int sub = 0; sub < 4; sub++) {
+      // B operand: features kq..kq+7 of node sub*16 + q (zero for kq >= 16)
+      const bf16x8 xb = kq < kNF ? *(const bf16x8*)&xw[(sub * 16 + q) * KE + kq] : bf16x8{};
+#pragma unroll
+      for (int t = 0; t < T; t++) {
+        const bf16x8 wa = kq < kNF ? *(const bf16x8*)&wel[(t * 16 + q) * KE + kq] : bf16x8{};
+        const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const int m = t * 16 + mq;
+        const float4 bb = *(const float4*)&bel[m];
+        *(uint2*)&stg[q * MP + m] = pack4_bf16(silu(z[0] + bb.x), silu(z[1] + bb.y), silu(z[2] + bb.z), silu(z[3] + bb.w));
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+      f32x4 ua[kC / 16];
+#pragma unroll
+      for (int t2 = 0; t2 < kC / 16; t2++) ua[t2] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < kH / 32; kk++) {
+        const bf16x8 hb = *(const bf16x8*)&stg[q * MP + kk * 32 + kq];
+#pragma unroll
+        for (int t2 = 0; t2 < kC / 16; t2++) {
+          const bf16x8 wa = *(const bf16x8*)&wul[(t2 * 16 + q) * KU + kk * 32 + kq];
+          ua[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hb, ua[t2], 0, 0, 0);
+        }
+      }
+      const uint32_t nu = base + sub * 16 + q;
+      if (nu < a.n) {
+#pragma unroll
+        for (int t2 = 0; t2 < kC / 16; t2++)
+          *(uint2*)&U0[(size_t)nu * kC + t2 * 16 + mq] = pack4_bf16(ua[t2][0], ua[t2][1], ua[t2][2], ua[t2][3]);
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -699,11 +811,13 @@ void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
   if (use_net) {
     if (!fused) enqueue_graph(s, c);
     a.X0 = s->acts.X0;
-    // features first (they read the state), then the network, then the update
-    StepArgs f = a;
-    f.apply = 0;
-    hipLaunchKernelGGL(k_learned, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f);
-    mp_forward(s->net, s->acts, s->graph, c->N, false, c->stream);
+    // features and the encoder first (they read the state), then the fused
+    // layers, then the update
+    const MpNet& net = s->net;
+    const uint32_t fe_blocks = std::max<uint32_t>(1, std::min<uint32_t>(nblk(c->N, kFeThreads), 1024));
+    hipLaunchKernelGGL(k_features_encode, dim3(fe_blocks), dim3(kFeThreads), 0, c->stream, a,
+                       net.t[net.We].w16, net.w + net.t[net.be].off, net.t[net.Wu[0]].w16, s->acts.U[0]);
+    mp_forward_layers(s->net, s->acts, s->graph, c->N, c->stream);
     a.X0 = nullptr;
     a.Y = s->acts.Y;
   }
