@@ -179,7 +179,8 @@ __device__ inline double tau_of(const SceneConst& c) { return (double)kStepsPerL
 // positive where the Euler step rho + tau drho would not.
 __device__ inline double advance_rho(double rho, double drho, double tau) { return rho * exp(tau * drho / rho); }
 
-// rho and S advanced over tau with D1, x and v unchanged (storage order).
+// rho and S advanced over tau with D1 (storage order; fields 4..7 of half
+// only: the half state takes x and v from the state itself).
 __global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
@@ -187,8 +188,6 @@ __global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
   const SceneConst& c = a.sc[a.scene[i]];
   const double tau = tau_of(c);
   const double* d = a.d1 + 8 * (size_t)p;
-#pragma unroll
-  for (int k = 0; k < 4; k++) a.half.f[k][i] = a.y.f[k][i];
   a.half.f[4][i] = advance_rho(a.y.f[4][i], d[4], tau);
 #pragma unroll
   for (int k = 5; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
@@ -767,7 +766,8 @@ void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c, bool fused_graph = false) 
   StepArgs a = step_args(s, c);
   hipLaunchKernelGGL(k_half_state, dim3(nblk(c->N, 256)), dim3(256), 0, st, a);
   UlArgs v = ul_args(c, b);
-  v.src = c->K;
+  v.src = c->K;  // rho, S of the half state; x, v of the state
+  for (int k = 0; k < 4; k++) v.src.f[k] = c->y[b].f[k];
   launch_ul_prepass(st, v);
   v.stage = 0;
   v.d8 = s->d2;
