@@ -334,7 +334,7 @@ def main():
     ap.add_argument("--sur-scenes", type=int, default=256)
     ap.add_argument("--sur-captures", type=int, default=80)
     ap.add_argument("--sur-gap", type=int, default=0)
-    ap.add_argument("--sur-iters", type=int, default=2000)
+    ap.add_argument("--sur-iters", type=int, default=4000)
     ap.add_argument("--sur-lr", type=float, default=1e-3)
     ap.add_argument("--sur-batch", type=int, default=1 << 20)
     ap.add_argument("--sur-group", type=int, default=16)
