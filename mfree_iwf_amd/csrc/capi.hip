@@ -358,14 +358,17 @@ mfree_status autotune_tiles(mfree_ctx* c) {
   return MFREE_OK;
 }
 
-// Enqueues one RK4 step of every UL body (src/rk4.h:71-108 with the UL
+// Enqueues one RK step of every UL body (src/rk4.h:71-108 with the UL
 // compute_time_derivatives of src/body.h:125-148 per stage).
-void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
+void enqueue_ul_rk(mfree_ctx* c, const RkPlan& plan, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
   hipStream_t st = c->stream;
   const int a = c->cur, b = 1 - a;
+  const SceneConst* sc = plan.sc ? plan.sc : c->sc;
   // stage 1: k1 = y_n; bin y_n, and move storage into the new cell order
   launch_binning(st, bin_args(c, c->y[a].f[0], c->y[a].f[1], a));
+  if (plan.after_bin1) plan.after_bin1(a);
   UlArgs u = ul_args(c, a);
+  u.sc = sc;
   u.src = c->y[a];
   u.reorder = 1;
   u.dst = c->y[b];
@@ -375,21 +378,34 @@ void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>
   u.gid_dst = c->gid[b];
   launch_ul_prepass(st, u);
   UlArgs p = ul_args(c, b);
+  p.sc = sc;
   p.src = c->y[b];
   p.stage = 1;
   p.home_identity = 1;
+  p.k1s = plan.k1s;
+  if (plan.stage1) plan.stage1(p);
   pair_launch(c, p, evs);
-  for (int s = 2; s <= 4; s++) {
+  static const int rk4[] = {2, 3, 4}, rk3[] = {5, 4};
+  const int* stages = plan.order == 3 ? rk3 : rk4;
+  const int nst = plan.order == 3 ? 2 : 3;
+  for (int k = 0; k < nst; k++) {
     launch_binning(st, bin_args(c, c->K.f[0], c->K.f[1], b));
     UlArgs q = ul_args(c, b);
+    q.sc = sc;
     q.src = c->K;
     launch_ul_prepass(st, q);
-    q.stage = s;
+    q.stage = stages[k];
     q.home_identity = 0;
+    q.rate_out = plan.rate_out && stages[k] == 4;
+    if (stages[k] == 5) q.k2s = plan.k2s;
     pair_launch(c, q, evs);
   }
-  hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);
+  if (plan.advance_time) hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);
   c->cur = b;
+}
+
+void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
+  enqueue_ul_rk(c, RkPlan(), evs);
 }
 
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {

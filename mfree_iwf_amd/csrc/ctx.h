@@ -3,6 +3,7 @@
 #pragma once
 
 #include <atomic>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -105,6 +106,22 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b);
 UlArgs ul_args(mfree_ctx* c, int b);
 // nsteps RK4 steps of every body (graphs when enabled).
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps);
+
+// One explicit RK step of every UL body, enqueued on the context's stream
+// (kernels only, capturable).  Storage moves to the other buffer.
+struct RkPlan {
+  const SceneConst* sc = nullptr;  // constants of the step (dt); null: the context's
+  int order = 4;                   // 4: RK4; 3: Kutta's third-order method
+  bool rate_out = false;           // acc <- mean rate of the step
+  bool advance_time = true;        // advance the bodies' time by dt
+  Fields8 k1s{}, k2s{};            // order 3: keep the stage rates k1, k2 (storage order of the new buffer)
+  // after the stage-1 binning, before the stage-1 prepass (argument: the
+  // storage buffer of the state)
+  std::function<void(int)> after_bin1;
+  // adjusts the stage-1 pair-pass arguments
+  std::function<void(UlArgs&)> stage1;
+};
+void enqueue_ul_rk(mfree_ctx* c, const RkPlan& p, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs);
 
 // Reference-structured FP64 pipeline (baseline.hip).
 mfree_status ref_alloc(mfree_ctx* c);

@@ -61,7 +61,11 @@ struct UlArgs {
   float2* pos32;
   PairRec32* rec32;
   // outputs
-  int stage;                    // 1..4: RK stage; 0: derivative evaluation only
+  int stage;                    // 1..4: RK4 stage; 0: derivative evaluation only;
+                                // 5: second stage of Kutta's third-order method
+                                // (stages 1, 5, 4)
+  int rate_out;                 // last stage: also acc <- mean rate (y_new = y + dt acc)
+  Fields8 k1s, k2s;             // stage rates k1 (stage 1) and k2 (stage 5), storage order (null: not kept)
   int home_identity;            // storage index == sorted position
   Fields8 y;                    // y_n (storage order), updated at stage 4
   Fields8 K;                    // next stage state (storage order)
@@ -85,6 +89,47 @@ struct UlArgs {
   const uint32_t* tile_off;
   uint32_t tile_grid;           // workgroups of the tile kernel (multiple of 8)
 };
+
+// RK stage update of storage index home from its derivatives d.  RK4
+// (src/rk4.h:71-108) runs stages 1-4; Kutta's third-order method runs
+// stages 1, 5, 4: k2 at y + dt/2 k1, k3 at y - dt k1 + 2 dt k2,
+// y_new = y + dt/6 (k1 + 4 k2 + k3).  acc holds the weighted sum of the
+// rates so far.
+__device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t home,
+                                       const double* d) {
+  if (a.stage == 1) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home];
+      a.acc.f[k][home] = d[k];
+      a.K.f[k][home] = yk + d[k] * c->half_dt;
+      if (a.k1s.f[0]) a.k1s.f[k][home] = d[k];
+    }
+  } else if (a.stage == 2 || a.stage == 3) {
+    const double dts = a.stage == 2 ? c->half_dt : c->dt;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home];
+      a.acc.f[k][home] = a.acc.f[k][home] + 2 * d[k];
+      a.K.f[k][home] = yk + d[k] * dts;
+    }
+  } else if (a.stage == 5) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home], k1 = a.acc.f[k][home];
+      a.acc.f[k][home] = k1 + 4 * d[k];
+      a.K.f[k][home] = yk + c->dt * (2 * d[k] - k1);
+      if (a.k2s.f[0]) a.k2s.f[k][home] = d[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double yk = a.y.f[k][home], sum = a.acc.f[k][home] + d[k];
+      a.y.f[k][home] = yk + c->dt6 * sum;
+      if (a.rate_out) a.acc.f[k][home] = sum * (1. / 6.);
+    }
+  }
+}
 
 void launch_ul_prepass(hipStream_t st, const UlArgs& a);
 void launch_ul_pairs(hipStream_t st, const UlArgs& a);

@@ -152,130 +152,113 @@ __global__ void k_edge_export(uint32_t n, uint32_t stride, const uint32_t* __res
 
 struct StepArgs {
   uint32_t n;
-  const uint32_t *perm, *scene, *gid;  // perm: sorted -> storage
+  const uint32_t *scene, *gid;         // storage order (= graph node order after the base step)
   const SceneConst* sc;
-  Fields8 y;                           // state, storage order
+  Fields8 y;                           // state after the base step, storage order
+  Fields8 rate;                        // mean rate of the base step: y = y_start + tau rate
+  Fields8 k1, k2;                      // its first two stage rates
   const double* h;                     // storage order
-  const double *d1, *d2;               // by sorted position: D1 (8 per particle), D2 acceleration (2)
   double* vprev;                       // by global id (2 per particle)
-  const float* wsum;                   // by sorted position
+  const float* wsum;                   // by node
   const float* norm;                   // kNorm floats (device), or null
   // outputs
-  float* F;                            // raw features by sorted position (kNF), or null
-  bf16_t* X0;                          // normalised features by sorted position (kK0), or null
-  const float* Y;                      // network output by sorted position (kOutP), or null
+  float* F;                            // raw features by node (kNF), or null
+  bf16_t* X0;                          // normalised features by node (kK0), or null
+  const float* Y;                      // network output by node (kOutP), or null
   double* base;                        // base update by global id (8), or null
-  uint32_t* inv;                       // global id -> sorted position, or null
-  double *ox, *oy, *oh;                // positions / h by sorted position, or null
-  int apply;                           // write the update into y (storage order)
-  int save_vprev;                      // vprev <- v before the update
-  Fields8 half;                        // k_half_state output
+  uint32_t* inv;                       // global id -> node, or null
+  double *ox, *oy, *oh;                // start positions / h by node, or null
+  int apply;                           // add the correction to y
+  int save_vprev;                      // vprev <- v at the start of the step
 };
 
 __device__ inline double tau_of(const SceneConst& c) { return (double)kStepsPerLearned * c.dt; }
 
-// Density over tau from its rate: rho exp(tau drho/rho), the exact solution
-// of the continuity equation for a constant velocity divergence; it stays
-// positive where the Euler step rho + tau drho would not.
-__device__ inline double advance_rho(double rho, double drho, double tau) { return rho * exp(tau * drho / rho); }
-
-// rho and S advanced over tau with D1 (storage order; fields 4..7 of half
-// only: the half state takes x and v from the state itself).
-__global__ __launch_bounds__(256) void k_half_state(StepArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-  const uint32_t i = a.perm[p];
-  const SceneConst& c = a.sc[a.scene[i]];
-  const double tau = tau_of(c);
-  const double* d = a.d1 + 8 * (size_t)p;
-  a.half.f[4][i] = advance_rho(a.y.f[4][i], d[4], tau);
-#pragma unroll
-  for (int k = 5; k < 8; k++) a.half.f[k][i] = a.y.f[k][i] + tau * d[k];
+// Constants of the base step: every body's dt scaled by kStepsPerLearned.
+__global__ void k_step_consts(uint32_t nscene, const SceneConst* __restrict__ sc, SceneConst* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscene) return;
+  SceneConst c = sc[s];
+  c.dt = tau_of(c);
+  c.half_dt = 0.5 * c.dt;
+  c.dt6 = c.dt * (1. / 6.);
+  out[s] = c;
 }
 
-// The 16 raw node features of sorted position p (state field order
-// x, y, vx, vy, rho, Sxx, Sxy, Syy):
-//   tau a2 / c0 (2), tau dS1 / K (3), tau drho1 / rho0, S / K (3), rho/rho0 - 1,
-//   (v - v_prev) / c0 (2), tau (xdot1 - v) / h (2), c0 tau / h, kernel-weight sum.
+// The 16 raw node features of node p (state field order x, y, vx, vy, rho,
+// Sxx, Sxy, Syy).  The base step is Kutta's method over tau with stage
+// increments a_i = tau k_i; for a linear system y' = A y,
+// D2 = 2 (a2 - a1) = (tau A)^2 y and D3 = a3 - 2 a2 + a1 = (tau A)^3 y, and
+// the step's error against the exact flow starts with (tau A)^4 y / 24: one
+// more application of the SPH operator to D3, which the message passing
+// can learn.  Features:
+//   (v - v_prev) / c0 (2), S / K (3), rho/rho0 - 1 (state at the start),
+//   D2 v / c0 (2), D3 v / c0 (2), D3 S / K (3), D3 rho / rho0,
+//   c0 tau / h, kernel-weight sum.
 __device__ inline void node_features(const StepArgs& a, uint32_t p, float* f) {
-  const uint32_t s = a.perm[p];
-  const uint32_t g = a.gid[s];
-  const SceneConst& c = a.sc[a.scene[s]];
+  const uint32_t g = a.gid[p];
+  const SceneConst& c = a.sc[a.scene[p]];
   const double tau = tau_of(c);
-  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[s];
-  double st[8];
+  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[p];
+  double st[8], d2[8], d3[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
-  const double* d1 = a.d1 + 8 * (size_t)p;
-  const double* d2 = a.d2 + 2 * (size_t)p;  // acceleration at the half state
+  for (int k = 2; k < 8; k++) {
+    const double m = a.rate.f[k][p], k1 = a.k1.f[k][p], k2 = a.k2.f[k][p];
+    const double k3 = 6.0 * m - k1 - 4.0 * k2;
+    st[k] = a.y.f[k][p] - tau * m;
+    d2[k] = 2.0 * tau * (k2 - k1);
+    d3[k] = tau * (k3 - 2.0 * k2 + k1);
+  }
   const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
-  f[0] = (float)(tau * d2[0] / c0);
-  f[1] = (float)(tau * d2[1] / c0);
-  f[2] = (float)(tau * d1[5] / K);
-  f[3] = (float)(tau * d1[6] / K);
-  f[4] = (float)(tau * d1[7] / K);
-  f[5] = (float)(tau * d1[4] / rho0);
-  f[6] = (float)(st[5] / K);
-  f[7] = (float)(st[6] / K);
-  f[8] = (float)(st[7] / K);
-  f[9] = (float)(st[4] / rho0 - 1.0);
-  f[10] = (float)((st[2] - vpx) / c0);
-  f[11] = (float)((st[3] - vpy) / c0);
-  f[12] = (float)(tau * (d1[0] - st[2]) / h);
-  f[13] = (float)(tau * (d1[1] - st[3]) / h);
+  f[0] = (float)((st[2] - vpx) / c0);
+  f[1] = (float)((st[3] - vpy) / c0);
+  f[2] = (float)(st[5] / K);
+  f[3] = (float)(st[6] / K);
+  f[4] = (float)(st[7] / K);
+  f[5] = (float)(st[4] / rho0 - 1.0);
+  f[6] = (float)(d2[2] / c0);
+  f[7] = (float)(d2[3] / c0);
+  f[8] = (float)(d3[2] / c0);
+  f[9] = (float)(d3[3] / c0);
+  f[10] = (float)(d3[5] / K);
+  f[11] = (float)(d3[6] / K);
+  f[12] = (float)(d3[7] / K);
+  f[13] = (float)(d3[4] / rho0);
   f[14] = (float)(c0 * tau / h);
   f[15] = a.wsum[p];
 }
 
-// One thread per sorted position: features, base update, optional network
-// correction.
+// One thread per node: features (training data), base-update export,
+// network correction.
 __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
-  const uint32_t s = a.perm[p];
-  const uint32_t g = a.gid[s];
-  const SceneConst& c = a.sc[a.scene[s]];
+  const uint32_t g = a.gid[p];
+  const SceneConst& c = a.sc[a.scene[p]];
   const double tau = tau_of(c);
-  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[s];
-  double st[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) st[k] = a.y.f[k][s];
-  const double* d1 = a.d1 + 8 * (size_t)p;
-  const double* d2 = a.d2 + 2 * (size_t)p;  // acceleration at the half state
-  if (a.F || a.X0) {
+  const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[p];
+  if (a.F) {
     float f[kNF];
     node_features(a, p, f);
-    if (a.F) {
 #pragma unroll
-      for (int k = 0; k < kNF; k++) a.F[(size_t)p * kNF + k] = f[k];
-    }
-    if (a.X0) {
-      bf16_t* x = a.X0 + (size_t)p * kK0;
-#pragma unroll
-      for (int k = 0; k < kNF; k++) x[k] = f2bf((f[k] - a.norm[k]) / a.norm[kNF + k]);
-#pragma unroll
-      for (int k = kNF; k < kK0; k++) x[k] = 0;
-    }
+    for (int k = 0; k < kNF; k++) a.F[(size_t)p * kNF + k] = f[k];
   }
-  // base update
   double nw[8];
-  nw[0] = st[0] + tau * d1[0] + 0.5 * tau * tau * d2[0];
-  nw[1] = st[1] + tau * d1[1] + 0.5 * tau * tau * d2[1];
-  nw[2] = st[2] + tau * d2[0];
-  nw[3] = st[3] + tau * d2[1];
-  nw[4] = advance_rho(st[4], d1[4], tau);
-  nw[5] = st[5] + tau * d1[5];
-  nw[6] = st[6] + tau * d1[6];
-  nw[7] = st[7] + tau * d1[7];
+#pragma unroll
+  for (int k = 0; k < 8; k++) nw[k] = a.y.f[k][p];
   if (a.base) {
 #pragma unroll
     for (int k = 0; k < 8; k++) a.base[8 * (size_t)g + k] = nw[k];
   }
   if (a.inv) a.inv[g] = p;
   if (a.ox) {
-    a.ox[p] = st[0];
-    a.oy[p] = st[1];
+    a.ox[p] = nw[0] - tau * a.rate.f[0][p];
+    a.oy[p] = nw[1] - tau * a.rate.f[1][p];
     a.oh[p] = h;
+  }
+  if (a.save_vprev) {
+    a.vprev[2 * (size_t)g] = nw[2] - tau * a.rate.f[2][p];
+    a.vprev[2 * (size_t)g + 1] = nw[3] - tau * a.rate.f[3][p];
   }
   if (a.Y) {
     // correction: output k (denormalised) times the field's scale
@@ -289,18 +272,14 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
       nw[k] += r * scale[k];
     }
   }
-  if (a.save_vprev) {
-    a.vprev[2 * (size_t)g] = st[2];
-    a.vprev[2 * (size_t)g + 1] = st[3];
-  }
   if (a.apply) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) a.y.f[k][s] = nw[k];
+    for (int k = 0; k < 8; k++) a.y.f[k][p] = nw[k];
   }
 }
 
 // Inference: features and the encoder's messages in one pass.  A wave takes
-// 64 sorted positions (one per lane): it writes their normalised features
+// 64 nodes (one per lane): it writes their normalised features
 // (X0, which the first fused layer reads again) and stages them in LDS, then,
 // per 16 nodes, H_0 = SiLU(X0 We^T + be) on v_mfma_f32_16x16x32_bf16
 // (D = We X0^T: lane = node lane%16, channels 16t + 4(lane/16)..+3), H_0 rows
@@ -615,17 +594,18 @@ struct mfree_surrogate {
   uint32_t bN = 0;
   bool uniform_h = false;  // every body of the bound context has one smoothing length
   std::vector<void*> callocs;
-  double *d1 = nullptr, *d2 = nullptr, *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
+  double *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
+  SceneConst* sc4 = nullptr;  // constants of the base step (dt scaled by kStepsPerLearned)
+  Fields8 k1s{}, k2s{};       // stage rates of the base step (storage order)
   uint32_t* inv = nullptr;
   uint32_t* comp = nullptr;  // per body: largest compression (float bits)
   bool vprev_valid = false;
   // rollback snapshot (storage buffer of the group start)
   double* snap = nullptr;   // 10 fields x N: y(8), m, h
   uint32_t* snapu = nullptr;  // scene, gid
-  double* jsnap = nullptr;  // 8 x N: state before the judge step
   SceneConst* snap_sc = nullptr;
   SceneConst* jsc = nullptr;
-  hipGraphExec_t lexec[2] = {nullptr, nullptr};  // captured learned steps per storage buffer
+  hipGraphExec_t lexec[2] = {nullptr, nullptr};  // captured learned steps per starting storage buffer
 };
 
 namespace mfree {
@@ -666,8 +646,7 @@ void drop_graphs(mfree_surrogate* s) {
     }
 }
 
-// Per-context buffers (derivatives, base update, previous velocities,
-// snapshots); reallocated when the surrogate meets another context.
+// Per-context buffers (base update, previous velocities, snapshots); reallocated when the surrogate meets another context.
 mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   mfree_status st = commit(c);
   if (st != MFREE_OK) return sfail(s, st, std::string("context: ") + c->err);
@@ -679,8 +658,6 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   drop_graphs(s);
   free_list(s->callocs);
   const size_t N = c->N;
-  SCHK(s, salloc(s->callocs, &s->d1, 8 * N));
-  SCHK(s, salloc(s->callocs, &s->d2, 2 * N));
   SCHK(s, salloc(s->callocs, &s->base, 8 * N));
   SCHK(s, salloc(s->callocs, &s->vprev, 2 * N));
   SCHK(s, salloc(s->callocs, &s->cand, 2 * N));
@@ -689,7 +666,11 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   SCHK(s, salloc(s->callocs, &s->comp, c->S));
   SCHK(s, salloc(s->callocs, &s->snap, 10 * N));
   SCHK(s, salloc(s->callocs, &s->snapu, 2 * N));
-  SCHK(s, salloc(s->callocs, &s->jsnap, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->sc4, c->S));
+  for (int k = 0; k < 8; k++) {
+    SCHK(s, salloc(s->callocs, &s->k1s.f[k], N));
+    SCHK(s, salloc(s->callocs, &s->k2s.f[k], N));
+  }
   SCHK(s, salloc(s->callocs, &s->snap_sc, c->S));
   SCHK(s, salloc(s->callocs, &s->jsc, c->S));
   std::string msg;
@@ -710,23 +691,24 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   return MFREE_OK;
 }
 
+// Learned-step arguments over the current storage buffer (the one the base
+// step wrote: storage index = graph node).
 StepArgs step_args(mfree_surrogate* s, mfree_ctx* c) {
   StepArgs a;
   memset(&a, 0, sizeof(a));
   const int b = c->cur;
   a.n = c->N;
-  a.perm = c->perm;
   a.scene = c->scene[b];
   a.gid = c->gid[b];
   a.sc = c->sc;
   a.y = c->y[b];
+  a.rate = c->acc;
+  a.k1 = s->k1s;
+  a.k2 = s->k2s;
   a.h = c->h[b];
-  a.d1 = s->d1;
-  a.d2 = s->d2;
   a.vprev = s->vprev;
   a.wsum = s->graph.wsum;
   a.norm = s->dnorm;
-  a.half = c->K;
   return a;
 }
 
@@ -737,47 +719,9 @@ void record_vprev(mfree_surrogate* s, mfree_ctx* c) {
   s->vprev_valid = true;
 }
 
-// Steps 1-2 of a learned step: cell list of the current state, D1, the
-// half state, D2.
-void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges = true);
-
-// Binning, D1 at the state and D2 at the half state.  fused_graph: the node
-// records first and the edge lists from the mixed D1 pass (bodies of
-// uniform h).
-void enqueue_derivs(mfree_surrogate* s, mfree_ctx* c, bool fused_graph = false) {
-  hipStream_t st = c->stream;
-  const int b = c->cur;
-  launch_binning(st, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
-  if (fused_graph) enqueue_graph(s, c, false);
-  UlArgs u = ul_args(c, b);
-  u.src = c->y[b];
-  launch_ul_prepass(st, u);
-  u.stage = 0;
-  u.d8 = s->d1;
-  u.d8_sorted = 1;
-  if (fused_graph) {
-    u.gcol = s->graph.col;
-    u.gdeg = s->graph.deg;
-    u.gwsum = s->graph.wsum;
-    u.gdpos = s->graph.dpos;
-    u.gstride = s->graph.cap;
-  }
-  launch_ul_pairs(st, u);
-  StepArgs a = step_args(s, c);
-  hipLaunchKernelGGL(k_half_state, dim3(nblk(c->N, 256)), dim3(256), 0, st, a);
-  UlArgs v = ul_args(c, b);
-  v.src = c->K;  // rho, S of the half state; x, v of the state
-  for (int k = 0; k < 4; k++) v.src.f[k] = c->y[b].f[k];
-  launch_ul_prepass(st, v);
-  v.stage = 0;
-  v.d8 = s->d2;
-  v.d8_sorted = 1;
-  v.accel_only = 1;  // the step uses only the acceleration of D2
-  launch_ul_pairs(st, v);
-}
-
-void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges) {
-  const int b = c->cur;
+// Graph of the state in storage buffer b over the current cell list: node
+// records, and the edge lists unless a pair pass emits them.
+void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, int b, bool edges) {
   GraphArgs g;
   memset(&g, 0, sizeof(g));
   g.n = c->N;
@@ -794,8 +738,37 @@ void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, bool edges) {
   launch_graph(c->stream, g, s->graph, edges);
 }
 
-void advance_time_device(mfree_ctx* c, uint32_t nsteps) {
-  hipLaunchKernelGGL(k_advance_time_n, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc, nsteps);
+// Base step of a learned step: one step of Kutta's third-order method over
+// tau = kStepsPerLearned dt that leaves the mean rate in acc and the first
+// two stage rates in k1s, k2s.  graph: the network's graph of the start state over the
+// stage-1 cell list (nodes = stage-1 sorted positions = storage indices of
+// the new buffer); on a mixed context whose bodies each have one h, the
+// stage-1 pair pass emits the edge lists.
+void enqueue_base(mfree_surrogate* s, mfree_ctx* c, bool graph, bool advance_time) {
+  hipLaunchKernelGGL(k_step_consts, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc, s->sc4);
+  RkPlan plan;
+  plan.sc = s->sc4;
+  plan.order = 3;
+  plan.rate_out = true;
+  plan.k1s = s->k1s;
+  plan.k2s = s->k2s;
+  plan.advance_time = false;
+  if (graph) {
+    const bool fused = c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h;
+    plan.after_bin1 = [s, c, fused](int b) { enqueue_graph(s, c, b, !fused); };
+    if (fused)
+      plan.stage1 = [s](UlArgs& u) {
+        u.gcol = s->graph.col;
+        u.gdeg = s->graph.deg;
+        u.gwsum = s->graph.wsum;
+        u.gdpos = s->graph.dpos;
+        u.gstride = s->graph.cap;
+      };
+  }
+  enqueue_ul_rk(c, plan, nullptr);
+  if (advance_time)
+    hipLaunchKernelGGL(k_advance_time_n, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc,
+                       kStepsPerLearned);
 }
 
 void advance_time_host(mfree_ctx* c, uint32_t nsteps) {
@@ -803,16 +776,15 @@ void advance_time_host(mfree_ctx* c, uint32_t nsteps) {
     for (uint32_t k = 0; k < nsteps; k++) c->time[b] = c->time[b] + c->prm[b].dt;
 }
 
-// Device work of one learned step of every body (capturable: kernels only).
+// Device work of one learned step of every body (capturable: kernels only);
+// storage moves to the other buffer.
 void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
-  const bool fused = use_net && c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h;
-  enqueue_derivs(s, c, fused);
+  enqueue_base(s, c, use_net, true);
   StepArgs a = step_args(s, c);
   if (use_net) {
-    if (!fused) enqueue_graph(s, c);
     a.X0 = s->acts.X0;
     // features and the encoder first (they read the state), then the fused
-    // layers, then the update
+    // layers, then the correction
     const MpNet& net = s->net;
     const uint32_t fe_blocks = std::max<uint32_t>(1, std::min<uint32_t>(nblk(c->N, kFeThreads), 1024));
     hipLaunchKernelGGL(k_features_encode, dim3(fe_blocks), dim3(kFeThreads), 0, c->stream, a,
@@ -821,31 +793,32 @@ void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
     a.X0 = nullptr;
     a.Y = s->acts.Y;
   }
-  a.apply = 1;
+  a.apply = use_net ? 1 : 0;
   a.save_vprev = 1;
   hipLaunchKernelGGL(k_learned, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, a);
-  advance_time_device(c, kStepsPerLearned);
 }
 
 // One learned step of every body (no synchronisation).  With graphs enabled
 // on the context, the network step is replayed from a graph captured once
-// per storage buffer (the step never reorders storage).
+// per starting storage buffer.
 mfree_status enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
-  const int b = c->cur;
+  const int a = c->cur;
   if (use_net && c->opt.use_graph) {
-    if (!s->lexec[b]) {
+    if (!s->lexec[a]) {
       hipGraph_t g;
       SCHK(s, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       enqueue_learned_device(s, c, true);
+      c->cur = a;  // capture did not execute anything
       SCHK(s, hipStreamEndCapture(c->stream, &g));
-      hipError_t e = hipGraphInstantiate(&s->lexec[b], g, nullptr, nullptr, 0);
+      hipError_t e = hipGraphInstantiate(&s->lexec[a], g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) {
-        s->lexec[b] = nullptr;
+        s->lexec[a] = nullptr;
         return sfail(s, MFREE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       }
     }
-    SCHK(s, hipGraphLaunch(s->lexec[b], c->stream));
+    SCHK(s, hipGraphLaunch(s->lexec[a], c->stream));
+    c->cur = 1 - a;
   } else {
     enqueue_learned_device(s, c, use_net);
   }
@@ -1271,8 +1244,10 @@ mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     launch_fill_u32(c->stream, s->comp, 0u, c->S);
     hipLaunchKernelGGL(k_body_compression, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N,
                        c->y[c->cur].f[4], c->scene[c->cur], c->sc, s->comp);
-    enqueue_derivs(s, c);
-    enqueue_graph(s, c);  // for the kernel-sum feature
+    // the base step writes the other storage buffer and leaves the capture
+    // state where it is
+    const int a0 = c->cur;
+    enqueue_base(s, c, true, false);
     StepArgs a = step_args(s, c);
     a.F = s->dsF + row0 * kNF;
     a.base = s->base;
@@ -1281,6 +1256,7 @@ mfree_status mfree_surrogate_collect(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     a.oy = s->dsY + row0;
     a.oh = s->dsH + row0;
     hipLaunchKernelGGL(k_learned, dim3(nblk(N, 256)), dim3(256), 0, c->stream, a);
+    c->cur = a0;
     st = sync_ctx(s, c);
     if (st != MFREE_OK) return st;
     record_vprev(s, c);
@@ -1404,15 +1380,17 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
     js.ms_learned += ms_since(t0);
     // judge: the last learned step and 4 plain steps from the same state
     auto t1 = std::chrono::steady_clock::now();
-    const int b = c->cur;
-    for (int k = 0; k < 8; k++) copy_async(s->jsnap + k * N, c->y[b].f[k], N * 8, c->stream);
+    // (the learned step writes the other storage buffer: the judged state
+    // stays in this one)
+    const int a0 = c->cur;
     copy_async(s->jsc, c->sc, c->S * sizeof(SceneConst), c->stream);
     const std::vector<double> jtimes = c->time;
     st = enqueue_learned(s, c, true);  // also records the velocities of this state
     if (st != MFREE_OK) return st;
+    const int b = c->cur;
     hipLaunchKernelGGL(k_pos_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b].f[0],
                        c->y[b].f[1], c->gid[b], s->cand);
-    for (int k = 0; k < 8; k++) copy_async(c->y[b].f[k], s->jsnap + k * N, N * 8, c->stream);
+    c->cur = a0;
     copy_async(c->sc, s->jsc, c->S * sizeof(SceneConst), c->stream);
     c->time = jtimes;
     st = sync_ctx(s, c);
@@ -1463,21 +1441,19 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
   SCHK(s, hipSetDevice(s->opt.device));
   mfree_status st = bind_ctx(s, c);
   if (st != MFREE_OK) return st;
+  // as in a learned step (the base step builds the graph; the state stays in
+  // its storage buffer)
+  const int a0 = c->cur;
+  enqueue_base(s, c, true, false);
   const int b = c->cur;
-  if (c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h) {
-    enqueue_derivs(s, c, true);  // as in a learned step: the D1 pass emits the edges
-  } else {
-    launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
-    enqueue_graph(s, c);
-  }
+  c->cur = a0;
   st = sync_ctx(s, c);
   if (st != MFREE_OK) return st;
   const size_t N = c->N;
   const size_t stride = s->graph.cap;  // >= N
-  std::vector<uint32_t> deg(N), perm(N), gid(N), colv(stride * kMaxDeg);
+  std::vector<uint32_t> deg(N), gid(N), colv(stride * kMaxDeg);
   SCHK(s, hipMemcpy(deg.data(), s->graph.deg, N * 4, hipMemcpyDeviceToHost));
-  SCHK(s, hipMemcpy(perm.data(), c->perm, N * 4, hipMemcpyDeviceToHost));
-  SCHK(s, hipMemcpy(gid.data(), c->gid[b], N * 4, hipMemcpyDeviceToHost));
+  SCHK(s, hipMemcpy(gid.data(), c->gid[b], N * 4, hipMemcpyDeviceToHost));  // node -> global id
   SCHK(s, hipMemcpy(colv.data(), s->graph.col, stride * kMaxDeg * 4, hipMemcpyDeviceToHost));
   std::vector<uint32_t> rp(N + 1, 0);
   for (size_t p = 0; p < N; p++) rp[p + 1] = rp[p] + deg[p];
@@ -1504,16 +1480,16 @@ mfree_status mfree_surrogate_graph(mfree_surrogate* s, mfree_ctx* c, uint32_t* r
     (void)hipFree(drp);
     if (e != hipSuccess) return sfail(s, MFREE_ERR_HIP, hipGetErrorString(e));
   }
-  // rows by global id, neighbours as global ids in sorted-position order
+  // rows by global id, neighbours as global ids in node order
   std::vector<uint32_t> row_of(N);
-  for (size_t p = 0; p < N; p++) row_of[gid[perm[p]]] = (uint32_t)p;
+  for (size_t p = 0; p < N; p++) row_of[gid[p]] = (uint32_t)p;
   row_ptr[0] = 0;
   for (size_t g = 0; g < N; g++) row_ptr[g + 1] = row_ptr[g] + deg[row_of[g]];
   for (size_t g = 0; g < N; g++) {
     const size_t p = row_of[g];
     for (uint32_t k = 0; k < deg[p]; k++) {
       const size_t o = row_ptr[g] + k, e = rp[p] + k;
-      if (col) col[o] = gid[perm[colv[(size_t)k * stride + p]]];
+      if (col) col[o] = gid[colv[(size_t)k * stride + p]];
       if (edge3) {
         edge3[3 * o] = ew[3 * e];
         edge3[3 * o + 1] = ew[3 * e + 1];

@@ -215,28 +215,7 @@ __device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__
     }
     return;
   }
-  if (a.stage == 1) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home];
-      a.acc.f[k][home] = d[k];
-      a.K.f[k][home] = yk + d[k] * c->half_dt;
-    }
-  } else if (a.stage == 2 || a.stage == 3) {
-    const double dts = a.stage == 2 ? c->half_dt : c->dt;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home];
-      a.acc.f[k][home] = a.acc.f[k][home] + 2 * d[k];
-      a.K.f[k][home] = yk + d[k] * dts;
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home];
-      a.y.f[k][home] = yk + c->dt6 * (a.acc.f[k][home] + d[k]);
-    }
-  }
+  rk_stage_update(a, c, home, d);
 }
 
 // Neighbour list entries buffered per thread in LDS (strided by the block

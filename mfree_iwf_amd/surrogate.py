@@ -1,4 +1,5 @@
-"""Learned stepper handle (message-passing surrogate of 4 RK4 steps).
+"""Learned stepper handle (message-passing correction of one large RK step
+that stands in for 4 RK4 steps).
 
 Thin wrapper over the mfree_surrogate_* / mfree_mpnn_* entry points of
 include/mfree_iwf_amd.h; training, inference and the judged rollout all run

@@ -12,18 +12,23 @@ restates the project's own definitions, in float64 numpy:
   backward       exact gradients of the mean-squared loss (the transposed
                  aggregation is a scatter over the edges, it does not assume
                  a symmetric graph)
-  features       the 16 node inputs of a learned step
-  base_update    the learned step without the network
+  kutta3         the base step, the learned step without the network: one
+                 step of Kutta's third-order method over tau of a derivative
+                 function, with its stage rates
+  features       the 16 node inputs of a learned step (start state, previous
+                 velocity, the base step's stage differences)
 
 Pins (tests/test_mpnn_oracle.py): finite differences of the loss against
 backward(); torch.autograd in float64 against forward() and backward(); a
 per-edge Python loop against the vectorised aggregation and the adjoint
 identity of its transpose; radius_graph's weights against the
 reference-pinned cubic spline of the CPU oracle; the graph's symmetry;
-base_update / half_state / advance_rho against the exact solutions for
-constant rates (uniformly accelerated motion, exponential compression);
-features against their invariance under a change of units; targets and
-correction as inverse maps.
+kutta3 against the degree-3 Taylor polynomial of exp(tau A) for a linear
+system y' = A y (the stability polynomial, which pins every coefficient of
+the tableau) and against the exact solution of uniformly accelerated
+motion; features against their invariance under a change of units and the
+stage differences against (tau A)^2 y and (tau A)^3 y of a linear system;
+targets and correction as inverse maps.
 """
 import numpy as np
 
@@ -210,50 +215,45 @@ def loss_and_grad(row_ptr, col, edge3, X, T, w):
 OUT_OF_FIELD = [0, 1, 2, 3, 7, 4, 5, 6]  # state field (x,y,vx,vy,rho,Sxx,Sxy,Syy) -> output index
 
 
-def features(state8, d1, d2, vprev, h, wsum, tau, c0, K, rho0):
-    """16 raw node features (same order as the library's k_learned)."""
-    s, a, b = state8, d1, d2
+def kutta3(state8, deriv, tau):
+    """One step of Kutta's third-order method over tau of y' = deriv(y):
+    k2 at y + tau/2 k1, k3 at y - tau k1 + 2 tau k2,
+    y + tau/6 (k1 + 4 k2 + k3).  Returns (new state, k1, k2, k3)."""
+    y = state8
+    k1 = deriv(y)
+    k2 = deriv(y + 0.5 * tau * k1)
+    k3 = deriv(y - tau * k1 + 2.0 * tau * k2)
+    return y + tau / 6.0 * (k1 + 4.0 * k2 + k3), k1, k2, k3
+
+
+def stage_differences(k1, k2, k3, tau):
+    """D2 = 2 (a2 - a1), D3 = a3 - 2 a2 + a1 with a_i = tau k_i: (tau A)^2 y
+    and (tau A)^3 y for a linear system y' = A y."""
+    return 2.0 * tau * (k2 - k1), tau * (k3 - 2.0 * k2 + k1)
+
+
+def features(start8, k1, k2, k3, vprev, h, wsum, tau, c0, K, rho0):
+    """16 raw node features (same order as the library's node_features)."""
+    s = start8
+    d2, d3 = stage_differences(k1, k2, k3, tau)
     f = np.zeros((s.shape[0], NF))
-    f[:, 0] = tau * b[:, 2] / c0
-    f[:, 1] = tau * b[:, 3] / c0
-    f[:, 2] = tau * a[:, 5] / K
-    f[:, 3] = tau * a[:, 6] / K
-    f[:, 4] = tau * a[:, 7] / K
-    f[:, 5] = tau * a[:, 4] / rho0
-    f[:, 6] = s[:, 5] / K
-    f[:, 7] = s[:, 6] / K
-    f[:, 8] = s[:, 7] / K
-    f[:, 9] = s[:, 4] / rho0 - 1.0
-    f[:, 10] = (s[:, 2] - vprev[:, 0]) / c0
-    f[:, 11] = (s[:, 3] - vprev[:, 1]) / c0
-    f[:, 12] = tau * (a[:, 0] - s[:, 2]) / h
-    f[:, 13] = tau * (a[:, 1] - s[:, 3]) / h
+    f[:, 0] = (s[:, 2] - vprev[:, 0]) / c0
+    f[:, 1] = (s[:, 3] - vprev[:, 1]) / c0
+    f[:, 2] = s[:, 5] / K
+    f[:, 3] = s[:, 6] / K
+    f[:, 4] = s[:, 7] / K
+    f[:, 5] = s[:, 4] / rho0 - 1.0
+    f[:, 6] = d2[:, 2] / c0
+    f[:, 7] = d2[:, 3] / c0
+    f[:, 8] = d3[:, 2] / c0
+    f[:, 9] = d3[:, 3] / c0
+    f[:, 10] = d3[:, 5] / K
+    f[:, 11] = d3[:, 6] / K
+    f[:, 12] = d3[:, 7] / K
+    f[:, 13] = d3[:, 4] / rho0
     f[:, 14] = c0 * tau / h
     f[:, 15] = wsum
     return f
-
-
-def advance_rho(rho, drho, tau):
-    """rho exp(tau drho / rho): continuity over tau at constant divergence."""
-    return rho * np.exp(tau * drho / rho)
-
-
-def half_state(state8, d1, tau):
-    """rho and S advanced over tau with D1; x, v unchanged."""
-    s = state8.copy()
-    s[:, 4] = advance_rho(state8[:, 4], d1[:, 4], tau)
-    s[:, 5:8] += tau * d1[:, 5:8]
-    return s
-
-
-def base_update(state8, d1, d2, tau):
-    s = state8
-    n = s.copy()
-    n[:, 0:2] = s[:, 0:2] + tau * d1[:, 0:2] + 0.5 * tau * tau * d2[:, 2:4]
-    n[:, 2:4] = s[:, 2:4] + tau * d2[:, 2:4]
-    n[:, 4] = advance_rho(s[:, 4], d1[:, 4], tau)
-    n[:, 5:8] = s[:, 5:8] + tau * d1[:, 5:8]
-    return n
 
 
 def _field_scales(h, c0, K, rho0):

@@ -136,29 +136,49 @@ def _random_state(n, rng):
     return s8
 
 
-def test_base_update_is_exact_for_constant_rates():
-    """Uniformly accelerated motion, constant stress rate, constant velocity
-    divergence: the base update is the exact solution over tau."""
-    rng = np.random.default_rng(21)
+def test_kutta3_is_the_taylor_polynomial_for_linear_systems():
+    """For y' = A y one step is p(tau A) y with p the degree-3 Taylor
+    polynomial of exp (every weight and node of the tableau enters one of
+    its coefficients), and the stage differences are (tau A)^2 y and
+    (tau A)^3 y."""
+    rng = np.random.default_rng(20)
+    A = rng.standard_normal((8, 8))
+    y = rng.standard_normal((5, 8))
+    tau = 0.3
+    got, k1, k2, k3 = om.kutta3(y, lambda s: s @ A.T, tau)
+    M = tau * A
+    P = np.eye(8) + M + M @ M / 2 + M @ M @ M / 6
+    np.testing.assert_allclose(got, y @ P.T, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(k1, y @ A.T, rtol=1e-13, atol=1e-14)
+    d2, d3 = om.stage_differences(k1, k2, k3, tau)
+    np.testing.assert_allclose(d2, y @ (M @ M).T, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(d3, y @ (M @ M @ M).T, rtol=1e-12, atol=1e-14)
+    # and not the degree-4 polynomial
+    assert np.abs(got - y @ (P + M @ M @ M @ M / 24).T).max() > 1e-6
+
+
+def test_kutta3_is_exact_for_uniform_acceleration():
+    """x' = v, v' = a, S' = const: the exact solution is a polynomial of
+    degree 2 in tau, which the method integrates exactly."""
+    rng = np.random.default_rng(31)
     n, tau = 50, 3e-3
     s8 = _random_state(n, rng)
     a = rng.standard_normal((n, 2))
     Sdot = 1e6 * rng.standard_normal((n, 3))
-    div = 5.0 * rng.standard_normal(n)  # drho/dt = -rho div v
-    d1 = np.zeros((n, 8))
-    d1[:, 0:2] = s8[:, 2:4]
-    d1[:, 2:4] = a
-    d1[:, 4] = -s8[:, 4] * div
-    d1[:, 5:8] = Sdot
-    d2 = d1.copy()  # constant rates: the same at the half state
-    new = om.base_update(s8, d1, d2, tau)
-    np.testing.assert_allclose(new[:, 0:2], s8[:, 0:2] + tau * s8[:, 2:4] + 0.5 * tau * tau * a, rtol=1e-14, atol=1e-15)
-    np.testing.assert_allclose(new[:, 2:4], s8[:, 2:4] + tau * a, rtol=1e-14, atol=1e-15)
-    np.testing.assert_allclose(new[:, 4], s8[:, 4] * np.exp(-div * tau), rtol=1e-14)
-    np.testing.assert_allclose(new[:, 5:8], s8[:, 5:8] + tau * Sdot, rtol=1e-14)
-    half = om.half_state(s8, d1, tau)
-    np.testing.assert_array_equal(half[:, 0:4], s8[:, 0:4])
-    np.testing.assert_array_equal(half[:, 4:8], new[:, 4:8])
+
+    def deriv(s):
+        d = np.zeros_like(s)
+        d[:, 0:2] = s[:, 2:4]
+        d[:, 2:4] = a
+        d[:, 5:8] = Sdot
+        return d
+
+    new, _, _, _ = om.kutta3(s8, deriv, tau)
+    np.testing.assert_allclose(new[:, 0:2], s8[:, 0:2] + tau * s8[:, 2:4] + 0.5 * tau * tau * a, rtol=1e-13,
+                               atol=1e-15)
+    np.testing.assert_allclose(new[:, 2:4], s8[:, 2:4] + tau * a, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(new[:, 5:8], s8[:, 5:8] + tau * Sdot, rtol=1e-13)
+    np.testing.assert_array_equal(new[:, 4], s8[:, 4])
 
 
 def test_features_are_unit_invariant():
@@ -168,18 +188,22 @@ def test_features_are_unit_invariant():
     rng = np.random.default_rng(22)
     n = 40
     s8 = _random_state(n, rng)
-    d1 = rng.standard_normal((n, 8)) * np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
-    d2 = rng.standard_normal((n, 8)) * np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
+    rates = np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
+    k1, k2, k3 = (rng.standard_normal((n, 8)) * rates for _ in range(3))
     vprev = s8[:, 2:4] + 0.01 * rng.standard_normal((n, 2))
     h = 1e-3 * (1.0 + 0.1 * rng.random(n))
     wsum = rng.random(n)
     tau, c0, K, rho0 = 4e-7, 128.0, 1.6e7, 1010.0
-    f = om.features(s8, d1, d2, vprev, h, wsum, tau, c0, K, rho0)
+    f = om.features(s8, k1, k2, k3, vprev, h, wsum, tau, c0, K, rho0)
     L, T, R, P = 3.7, 0.29, 2.3, 11.0  # length, time, density, stress units
     sc = np.array([L, L, L / T, L / T, R, P, P, P])   # state fields
     rt = sc / T                                       # their rates
-    g = om.features(s8 * sc, d1 * rt, d2 * rt, vprev * L / T, h * L, wsum, tau * T, c0 * L / T, K * P, rho0 * R)
-    np.testing.assert_allclose(g, f, rtol=1e-12, atol=1e-15)
+    g = om.features(s8 * sc, k1 * rt, k2 * rt, k3 * rt, vprev * L / T, h * L, wsum, tau * T, c0 * L / T, K * P,
+                    rho0 * R)
+    np.testing.assert_allclose(g, f, rtol=1e-10, atol=1e-15)
+    # equal stage rates (a constant rate): no stage differences
+    z = om.features(s8, k1, k1, k1, vprev, h, wsum, tau, c0, K, rho0)
+    np.testing.assert_array_equal(z[:, 6:14], 0.0)
 
 
 def test_targets_and_correction_are_inverse():
