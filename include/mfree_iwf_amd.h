@@ -94,6 +94,9 @@ typedef struct mfree_options {
   double tl_h_search;       /* TL: h passed to find_neighbors; <= 0: per-particle h */
   int32_t tile_cells;       /* mixed UL: cells per LDS pair tile; 0 = autotune, < 0 = no tiles */
   int32_t pipeline;         /* MFREE_PIPELINE_* (REFERENCE: UL + FP64 only) */
+  uint32_t xcd_chunk;       /* UL pair passes: particle tiles per XCD chunk (consecutive tiles that
+                               one XCD takes in turn); 0 = library default (64); a value of at
+                               least tiles/8 gives each XCD one contiguous range */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y

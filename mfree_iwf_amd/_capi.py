@@ -43,7 +43,8 @@ class Params(C.Structure):
 class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
-                ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32)]
+                ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32),
+                ("xcd_chunk", C.c_uint32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")

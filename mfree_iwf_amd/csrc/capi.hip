@@ -136,6 +136,7 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, dalloc(c, &c->key, N));
   HIPCHK(c, dalloc(c, &c->rank, N));
   HIPCHK(c, dalloc(c, &c->count, N));
+  HIPCHK(c, hipMemset(c->count, 0, (size_t)N * sizeof(uint32_t)));  // the binning keeps it zero between calls
   HIPCHK(c, dalloc(c, &c->cell_start, N + 1));
   HIPCHK(c, dalloc(c, &c->perm, N));
   HIPCHK(c, dalloc(c, &c->tsum, (N + kScanTile - 1) / kScanTile + 1));
@@ -247,6 +248,7 @@ UlArgs ul_args(mfree_ctx* c, int b) {
   u.tile_cells = c->tile_cells;
   u.tile_off = c->tile_cells ? c->tile_off : nullptr;
   u.tile_grid = c->tile_grid;
+  u.xcd_chunk = c->opt.xcd_chunk ? c->opt.xcd_chunk : kXcdChunkDefault;
   return u;
 }
 

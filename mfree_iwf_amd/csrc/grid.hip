@@ -244,25 +244,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* __restric
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
 }
 
-// One workgroup: exclusive scan of the tile sums in place.
-__global__ __launch_bounds__(kBlock) void k_scan_tilesums(uint32_t* __restrict__ tsum, uint32_t nt) {
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < nt; b += kBlock) {
-    uint32_t i = b + threadIdx.x;
-    uint32_t v = i < nt ? tsum[i] : 0;
-    uint32_t ex;
-    uint32_t tot = block_excl_scan(v, &ex);
-    if (i < nt) tsum[i] = carry + ex;
-    carry += tot;
-    __syncthreads();
-  }
-}
-
 // out[i] = tile base + exclusive scan within the tile; out[n] = total.
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+// Each workgroup sums the tile sums before its own (at most a few thousand
+// values).  clear: zero in[] after reading it (the binning's cell counts,
+// which then start every binning at zero without a fill pass), or null.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ in, uint32_t n,
                                                        const uint32_t* __restrict__ tsum,
-                                                       uint32_t* __restrict__ out, uint32_t total) {
+                                                       uint32_t* __restrict__ out, uint32_t total, int clear) {
   constexpr uint32_t per = kScanTile / kBlock;
+  uint32_t pre = 0;
+  for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) pre += tsum[t];
+  uint32_t ex;
+  const uint32_t tile_base = block_excl_scan(pre, &ex);
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * per;
   uint32_t vals[per];
   uint32_t v = 0;
@@ -272,9 +265,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
     vals[k] = i < n ? in[i] : 0;
     v += vals[k];
   }
-  uint32_t ex;
+  if (clear) {
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++)
+      if (base + k < n) in[base + k] = 0;
+  }
   block_excl_scan(v, &ex);
-  uint32_t run = tsum[blockIdx.x] + ex;
+  uint32_t run = tile_base + ex;
 #pragma unroll
   for (uint32_t k = 0; k < per; k++) {
     uint32_t i = base + k;
@@ -364,8 +361,7 @@ void launch_excl_scan(hipStream_t st, const uint32_t* in, uint32_t n, uint32_t* 
   }
   const uint32_t nt = nblk(n, kScanTile);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, in, n, tsum);
-  hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(kBlock), 0, st, tsum, nt);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, in, n, tsum, out, 0u);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, const_cast<uint32_t*>(in), n, tsum, out, 0u, 0);
   hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, in, n, out);
 }
 
@@ -375,14 +371,13 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
                      a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err);
   if (a.tile_cells && a.tile_off)
     hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kBlock), 0, st, a.nscene, a.geom, a.tile_off);
-  launch_fill_u32(st, a.count, 0u, a.ncell_cap);
+  // a.count is zero here: zeroed at allocation and cleared again by the scan
   hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
                      a.geom, a.key, a.rank, a.count, a.err);
   uint32_t nt = nblk(a.ncell_cap, kScanTile);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum);
-  hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(kBlock), 0, st, a.tsum, nt);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum, a.cell_start,
-                     a.n);
+                     a.n, 1);
   hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
                      a.cell_start, a.perm);
   if (a.deterministic)

@@ -22,6 +22,24 @@ __device__ inline uint32_t xcd_tile() {
   const uint32_t per = gridDim.x / kXcds;
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
+// Default XCD chunk of the per-particle pair kernels: 64 tiles of 256
+// particles, about 16 cell columns of a 1M-particle ring scene (neighbour
+// reads stay mostly within one XCD's chunk) while a dense stretch of tiles
+// (a contact zone) is shared by several XCDs.
+constexpr uint32_t kXcdChunkDefault = 64;
+// The same with the tiles dealt to the XCDs in chunks of `chunk`
+// consecutive tiles (XCD x takes chunks x, x + 8, ...; the remainder is
+// split evenly), so that a costly stretch of tiles is shared by several
+// XCDs; chunk = 0: one contiguous range per XCD (xcd_tile).  A bijection
+// of [0, gridDim.x) when gridDim.x is a multiple of kXcds.
+__device__ inline uint32_t xcd_tile_chunked(uint32_t chunk) {
+  const uint32_t per = gridDim.x / kXcds;
+  const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  if (chunk == 0 || chunk >= per) return x * per + j;
+  const uint32_t full = per / chunk, c = j / chunk, o = j - c * chunk;
+  if (c < full) return (c * kXcds + x) * chunk + o;
+  return full * kXcds * chunk + x * (per - full * chunk) + o;
+}
 
 // error bits raised by kernels (mapped to mfree_status on the host)
 enum : uint32_t {

@@ -88,6 +88,7 @@ struct UlArgs {
   uint32_t tile_cells;          // 0: per-particle kernel without tiles
   const uint32_t* tile_off;
   uint32_t tile_grid;           // workgroups of the tile kernel (multiple of 8)
+  uint32_t xcd_chunk;           // per-particle pair kernels: tiles per XCD chunk (xcd_tile_chunked)
 };
 
 // RK stage update of storage index home from its derivatives d.  RK4

@@ -157,7 +157,7 @@ struct StepArgs {
   Fields8 y;                           // state after the base step, storage order
   Fields8 rate;                        // mean rate of the base step: y = y_start + tau rate
   Fields8 k1, k2;                      // its first two stage rates
-  const double* h;                     // storage order
+  const double *h, *m;                 // storage order
   double* vprev;                       // by global id (2 per particle)
   const float* wsum;                   // by node
   const float* norm;                   // kNorm floats (device), or null
@@ -168,6 +168,7 @@ struct StepArgs {
   double* base;                        // base update by global id (8), or null
   uint32_t* inv;                       // global id -> node, or null
   double *ox, *oy, *oh;                // start positions / h by node, or null
+  const double* cmean;                 // per body: mass-weighted mean of the x, y, vx, vy corrections
   int apply;                           // add the correction to y
   int save_vprev;                      // vprev <- v at the start of the step
 };
@@ -228,6 +229,64 @@ __device__ inline void node_features(const StepArgs& a, uint32_t p, float* f) {
   f[15] = a.wsum[p];
 }
 
+// Network correction of field k (state order) at node p: output (denormalised)
+// times the field's scale (h, h, c0, c0, rho0, K, K, K).
+__device__ inline double correction(const StepArgs& a, uint32_t p, int k, double h, double c0, double K,
+                                    double rho0) {
+  const int out_of_field[8] = {0, 1, 2, 3, 7, 4, 5, 6};  // field -> output index
+  const double scale[8] = {h, h, c0, c0, rho0, K, K, K};
+  const int o = out_of_field[k];
+  const double r = (double)a.Y[(size_t)p * kOutP + o] * (double)a.norm[2 * kNF + kOut + o] + (double)a.norm[2 * kNF + o];
+  return r * scale[k];
+}
+
+// Conservation of the correction.  Kutta's step and the RK4 steps it stands
+// in for both conserve each body's linear momentum exactly (antisymmetric
+// pair forces) and move its centre of mass by tau times the momentum over
+// the mass (the XSPH term sums to zero), so their difference has a zero
+// mass-weighted mean in x, y, vx, vy.  The correction is projected onto
+// that subspace: per chunk of a body, sums of m dx, m dy, m dvx, m dvy, m
+// (fixed-order block reduction), then per body the means.
+constexpr int kCorrThreads = 256;
+__global__ __launch_bounds__(kCorrThreads) void k_corr_partial(StepArgs a, const Chunk* __restrict__ chunks,
+                                                               double* __restrict__ part) {
+  const Chunk ch = chunks[blockIdx.x];
+  const SceneConst& c = a.sc[ch.scene];
+  const double c0 = sqrt(c.c0sq);
+  double acc[5] = {0., 0., 0., 0., 0.};
+  for (uint32_t p = ch.begin + threadIdx.x; p < ch.end; p += kCorrThreads) {
+    const double m = a.m[p], h = a.h[p];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += m * correction(a, p, k, h, c0, c.K, c.rho0);
+    acc[4] += m;
+  }
+  __shared__ double red[5][kCorrThreads];
+#pragma unroll
+  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = kCorrThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[5 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_corr_final(uint32_t nscene, const uint32_t* __restrict__ chunk_first,
+                             const double* __restrict__ part, double* __restrict__ cmean) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscene) return;
+  double t[5] = {0., 0., 0., 0., 0.};
+  for (uint32_t q = chunk_first[s]; q < chunk_first[s + 1]; q++) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) t[k] += part[5 * (size_t)q + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) cmean[4 * (size_t)s + k] = t[4] > 0. ? t[k] / t[4] : 0.;
+}
+
 // One thread per node: features (training data), base-update export,
 // network correction.
 __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
@@ -261,16 +320,9 @@ __global__ __launch_bounds__(256) void k_learned(StepArgs a) {
     a.vprev[2 * (size_t)g + 1] = nw[3] - tau * a.rate.f[3][p];
   }
   if (a.Y) {
-    // correction: output k (denormalised) times the field's scale
-    const double scale[8] = {h, h, c0, c0, rho0, K, K, K};
-    const int out_of_field[8] = {0, 1, 2, 3, 7, 4, 5, 6};  // field -> output index
-    const float* yv = a.Y + (size_t)p * kOutP;
+    const double* mean = a.cmean + 4 * (size_t)a.scene[p];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int o = out_of_field[k];
-      const double r = (double)yv[o] * (double)a.norm[2 * kNF + kOut + o] + (double)a.norm[2 * kNF + o];
-      nw[k] += r * scale[k];
-    }
+    for (int k = 0; k < 8; k++) nw[k] += correction(a, p, k, h, c0, K, rho0) - (k < 4 ? mean[k] : 0.);
   }
   if (a.apply) {
 #pragma unroll
@@ -597,6 +649,7 @@ struct mfree_surrogate {
   double *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
   SceneConst* sc4 = nullptr;  // constants of the base step (dt scaled by kStepsPerLearned)
   Fields8 k1s{}, k2s{};       // stage rates of the base step (storage order)
+  double *cpart = nullptr, *cmean = nullptr;  // correction sums per chunk (5), means per body (4)
   uint32_t* inv = nullptr;
   uint32_t* comp = nullptr;  // per body: largest compression (float bits)
   bool vprev_valid = false;
@@ -667,6 +720,8 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   SCHK(s, salloc(s->callocs, &s->snap, 10 * N));
   SCHK(s, salloc(s->callocs, &s->snapu, 2 * N));
   SCHK(s, salloc(s->callocs, &s->sc4, c->S));
+  SCHK(s, salloc(s->callocs, &s->cpart, 5 * (size_t)c->nchunks));
+  SCHK(s, salloc(s->callocs, &s->cmean, 4 * (size_t)c->S));
   for (int k = 0; k < 8; k++) {
     SCHK(s, salloc(s->callocs, &s->k1s.f[k], N));
     SCHK(s, salloc(s->callocs, &s->k2s.f[k], N));
@@ -706,6 +761,7 @@ StepArgs step_args(mfree_surrogate* s, mfree_ctx* c) {
   a.k1 = s->k1s;
   a.k2 = s->k2s;
   a.h = c->h[b];
+  a.m = c->m[b];
   a.vprev = s->vprev;
   a.wsum = s->graph.wsum;
   a.norm = s->dnorm;
@@ -792,6 +848,10 @@ void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
     mp_forward_layers(s->net, s->acts, s->graph, c->N, c->stream);
     a.X0 = nullptr;
     a.Y = s->acts.Y;
+    a.cmean = s->cmean;
+    hipLaunchKernelGGL(k_corr_partial, dim3(c->nchunks), dim3(kCorrThreads), 0, c->stream, a, c->chunks, s->cpart);
+    hipLaunchKernelGGL(k_corr_final, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->chunk_first, s->cpart,
+                       s->cmean);
   }
   a.apply = use_net ? 1 : 0;
   a.save_vprev = 1;
@@ -960,6 +1020,7 @@ mfree_status build_windows(mfree_surrogate* s) {
     SCHK(s, salloc(s->tallocs, &s->tkey, n));
     SCHK(s, salloc(s->tallocs, &s->trank, n));
     SCHK(s, salloc(s->tallocs, &s->tcount, n));
+    SCHK(s, hipMemset(s->tcount, 0, n * sizeof(uint32_t)));  // the binning keeps it zero between calls
     SCHK(s, salloc(s->tallocs, &s->tcell, n + 1));
     SCHK(s, salloc(s->tallocs, &s->tperm, n));
     SCHK(s, salloc(s->tallocs, &s->ttsum, (n + kScanTile - 1) / kScanTile + 1));

@@ -225,7 +225,7 @@ constexpr int kNbCap = 32;
 __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
   const uint32_t s = mt.y;
@@ -327,10 +327,16 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
     nb = 0;
   };
 
-  // scan: the 3x3 block ni-major, nj-minor, particles of a cell in idx order
+  // scan: the 3x3 block ni-major, nj-minor, particles of a cell in idx
+  // order.  Flushes are collective: before each column the lanes flush
+  // together if any lane's buffer is likely to overflow in it (half the
+  // column's candidates; a flush taken by each lane on its own inside the
+  // scan runs once per lane, the others idle, in dense regions); the
+  // in-scan flush guards the rare lane that still fills its buffer.
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
     const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
+    if (__any(nb + ((jb - ja + 1) >> 1) > (uint32_t)kNbCap)) flush();
     for (uint32_t j = ja; j < jb; j++) {
       const double2 pj = a.pos64[j];
       const double xij = pi.x - pj.x;

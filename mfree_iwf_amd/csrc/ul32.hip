@@ -290,7 +290,7 @@ template <bool ACC, bool GRAPH = false>
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -340,10 +340,18 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
     }
     nb = 0;
   };
+  // Buffer flushes are collective: before each column of the 3x3 block,
+  // the lanes flush together if any lane's buffer is likely to overflow in
+  // that column (half its candidates: about a third of a column lies
+  // inside the cutoff).  A flush taken by each lane on its own inside the
+  // scan would run once per lane, with the other lanes idle, wherever dense
+  // regions fill the buffers at different times; the in-scan flush below
+  // stays as the guard for the rare lane that still fills its buffer.
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
     const float offx = (float)((int)gi - (int)ni) * cellf;
     uint32_t jb = a.cell_start[rb + lo_j];
+    if (__any(nb + ((a.cell_start[rb + hi_j] - jb + 1) >> 1) > (uint32_t)kNbCap)) flush();
     for (uint32_t nj = lo_j; nj < hi_j; nj++) {
       const uint32_t ja = jb;
       jb = a.cell_start[rb + nj + 1];

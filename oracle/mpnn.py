@@ -17,6 +17,8 @@ restates the project's own definitions, in float64 numpy:
                  function, with its stage rates
   features       the 16 node inputs of a learned step (start state, previous
                  velocity, the base step's stage differences)
+  conserve       the correction's projection onto zero mass-weighted mean
+                 of x, y, vx, vy (linear momentum and centre of mass)
 
 Pins (tests/test_mpnn_oracle.py): finite differences of the loss against
 backward(); torch.autograd in float64 against forward() and backward(); a
@@ -28,7 +30,9 @@ system y' = A y (the stability polynomial, which pins every coefficient of
 the tableau) and against the exact solution of uniformly accelerated
 motion; features against their invariance under a change of units and the
 stage differences against (tau A)^2 y and (tau A)^3 y of a linear system;
-targets and correction as inverse maps.
+targets and correction as inverse maps; conserve as the orthogonal
+projection of the mass-weighted inner product (idempotent, zero sums,
+closest point of the subspace).
 """
 import numpy as np
 
@@ -276,3 +280,13 @@ def correction(Y, norm, h, c0, K, rho0):
     ts = norm[2 * NF + NOUT:]
     r = np.asarray(Y, dtype=np.float64) * ts + tm
     return r[:, OUT_OF_FIELD] * _field_scales(h, c0, K, rho0)
+
+
+def conserve(delta, m):
+    """Per-body projection of the increments of x, y, vx, vy onto zero
+    mass-weighted mean (one body; the library does it per body)."""
+    d = np.array(delta, dtype=np.float64, copy=True)
+    M = m.sum()
+    for k in range(4):
+        d[:, k] -= (m * d[:, k]).sum() / M
+    return d

@@ -224,7 +224,8 @@ def test_base_step_mixed_close_to_fp64():
 
 def test_learned_step_matches_oracle():
     """Network step on random weights: the correction added to the base
-    update agrees with the FP64 oracle's features -> forward -> correction."""
+    update agrees with the FP64 oracle's features -> forward -> correction
+    -> conservation projection."""
     s = perturbed_rings(nbox=41, seed=6, dpos=0.2)
     ps = s.particles
     tau, c0, K, rho0 = _scales(s)
@@ -237,7 +238,7 @@ def test_learned_step_matches_oracle():
     w = om.init_params(seed=9).astype(np.float32)
     X = (feats - norm[:om.NF]) / norm[om.NF:2 * om.NF]
     Y, _ = om.forward(row, col, e3, X, w.astype(np.float64))
-    want = om.correction(Y, norm.astype(np.float64), ps.h, c0, K, rho0)
+    want = om.conserve(om.correction(Y, norm.astype(np.float64), ps.h, c0, K, rho0), ps.m)
     with _ctx(s) as sv, sg.Surrogate() as su:
         su.set_params(w, norm)
         su.record_velocity(sv)

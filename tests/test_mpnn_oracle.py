@@ -219,3 +219,20 @@ def test_targets_and_correction_are_inverse():
     # the position increments are in units of h, the stresses in units of K
     assert t[:, 0] == pytest.approx(delta[:, 0] / h)
     assert t[:, om.OUT_OF_FIELD[5]] == pytest.approx(delta[:, 5] / K)
+
+
+def test_conserve_is_the_mass_weighted_orthogonal_projection():
+    rng = np.random.default_rng(24)
+    n = 60
+    m = rng.uniform(0.5, 2.0, n)
+    y = rng.standard_normal((n, 8))
+    p = om.conserve(y, m)
+    # zero mass-weighted sums of x, y, vx, vy; rho and S untouched
+    np.testing.assert_allclose((m[:, None] * p[:, 0:4]).sum(0), 0.0, atol=1e-12)
+    np.testing.assert_array_equal(p[:, 4:8], y[:, 4:8])
+    # idempotent
+    np.testing.assert_allclose(om.conserve(p, m), p, rtol=0, atol=1e-14)
+    # the closest point of the subspace in the m-weighted norm: p - y is
+    # m-orthogonal to every member t of the subspace
+    t = om.conserve(rng.standard_normal((n, 8)), m)
+    np.testing.assert_allclose((m[:, None] * (p - y)[:, 0:4] * t[:, 0:4]).sum(), 0.0, atol=1e-10)
