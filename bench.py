@@ -113,23 +113,25 @@ def bench_surrogate(scene, args, ref_state, log):
     """Learned stepper: training data from the batched workload's generator
     (subsonic scenes, seed 0), training on the GPU, then the judged rollout
     of the full workload over the accuracy horizon, timed end to end."""
-    from mfree_iwf_amd import PRECISION_MIXED, Solver, Surrogate, scenes
+    from mfree_iwf_amd import PRECISION_FP64, PRECISION_MIXED, Solver, Surrogate, scenes
     t0 = time.perf_counter()
     train = scenes.scenes_from_specs(scenes.subsonic_batch_specs(args.sur_scenes, seed=0))
     ntrain = sum(s.particles.n for s in train)
     log(f"surrogate: {len(train)} training bodies, {ntrain} particles (setup {time.perf_counter() - t0:.1f} s)")
     res = {}
-    with Surrogate(seed=0, max_batch_nodes=args.sur_batch) as su:
+    res["base_order"] = args.sur_order
+    with Surrogate(seed=0, max_batch_nodes=args.sur_batch, base_order=args.sur_order) as su:
         if args.sur_iters == 0:
             # kernel timing only: the seeded network, unit feature normalisation
             su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.full(8, 1e-6)]))
             return _rollout(su, scene, args, None, res, log)
-        with Solver(precision=PRECISION_MIXED) as gen:
+        with Solver(precision=PRECISION_FP64 if args.sur_collect == "fp64" else PRECISION_MIXED) as gen:
             for s in train:
                 gen.add_body(s.particles, s.params)
             t0 = time.perf_counter()
             su.collect(gen, ncapture=args.sur_captures, gap=args.sur_gap)
             res["collect_s"] = time.perf_counter() - t0
+        res["collect_precision"] = args.sur_collect
         res["snapshots"] = su.snapshots
         res["dataset_nodes"] = su.dataset_nodes
         log(f"  collected {su.snapshots} usable snapshots ({su.dataset_nodes} nodes) in {res['collect_s']:.1f} s")
@@ -334,7 +336,11 @@ def main():
     ap.add_argument("--sur-scenes", type=int, default=256)
     ap.add_argument("--sur-captures", type=int, default=80)
     ap.add_argument("--sur-gap", type=int, default=0)
-    ap.add_argument("--sur-iters", type=int, default=4000)
+    ap.add_argument("--sur-iters", type=int, default=8000)
+    ap.add_argument("--sur-order", type=int, default=4, choices=[3, 4],
+                    help="base step of a learned step: RK4 (4) or Kutta's third-order method (3) over 4 dt")
+    ap.add_argument("--sur-collect", default="fp64", choices=["fp64", "mixed"],
+                    help="precision of the batched solver that generates the training rollouts")
     ap.add_argument("--sur-lr", type=float, default=1e-3)
     ap.add_argument("--sur-batch", type=int, default=1 << 20)
     ap.add_argument("--sur-group", type=int, default=16)

@@ -275,6 +275,7 @@ typedef struct mfree_surrogate_options {
   uint64_t seed;             /* weight initialisation and minibatch order */
   uint32_t max_batch_nodes;  /* training minibatch cap (whole bodies of one snapshot) */
   float max_compression;     /* snapshots of bodies with max |rho/rho0 - 1| above this are not trained on */
+  int32_t base_order;        /* base step of a learned step: 4 (RK4 over 4 dt, default) or 3 (Kutta) */
 } mfree_surrogate_options;
 
 void mfree_surrogate_default_options(mfree_surrogate_options* o);

@@ -68,7 +68,7 @@ class PlateSpec(C.Structure):
 # name -> (restype, argtypes); every symbol include/mfree_iwf_amd.h declares.
 class SurrogateOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("seed", C.c_uint64), ("max_batch_nodes", C.c_uint32),
-                ("max_compression", C.c_float)]
+                ("max_compression", C.c_float), ("base_order", C.c_int32)]
 
 
 class JudgeStats(C.Structure):

@@ -399,7 +399,8 @@ void enqueue_ul_rk(mfree_ctx* c, const RkPlan& plan, std::vector<std::pair<hipEv
     q.stage = stages[k];
     q.home_identity = 0;
     q.rate_out = plan.rate_out && stages[k] == 4;
-    if (stages[k] == 5) q.k2s = plan.k2s;
+    if (stages[k] == 5 || stages[k] == 2) q.k2s = plan.k2s;
+    if (stages[k] == 3) q.k3s = plan.k3s;
     pair_launch(c, q, evs);
   }
   if (plan.advance_time) hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);

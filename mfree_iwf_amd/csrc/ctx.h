@@ -114,7 +114,7 @@ struct RkPlan {
   int order = 4;                   // 4: RK4; 3: Kutta's third-order method
   bool rate_out = false;           // acc <- mean rate of the step
   bool advance_time = true;        // advance the bodies' time by dt
-  Fields8 k1s{}, k2s{};            // order 3: keep the stage rates k1, k2 (storage order of the new buffer)
+  Fields8 k1s{}, k2s{}, k3s{};     // keep the stage rates k1, k2 (and k3 of RK4), storage order of the new buffer
   // after the stage-1 binning, before the stage-1 prepass (argument: the
   // storage buffer of the state)
   std::function<void(int)> after_bin1;

@@ -65,7 +65,8 @@ struct UlArgs {
                                 // 5: second stage of Kutta's third-order method
                                 // (stages 1, 5, 4)
   int rate_out;                 // last stage: also acc <- mean rate (y_new = y + dt acc)
-  Fields8 k1s, k2s;             // stage rates k1 (stage 1) and k2 (stage 5), storage order (null: not kept)
+  Fields8 k1s, k2s, k3s;        // stage rates k1 (stage 1), k2 (stage 2 or 5), k3 (stage 3), storage
+                                // order; fields with a null pointer are not kept
   int home_identity;            // storage index == sorted position
   Fields8 y;                    // y_n (storage order), updated at stage 4
   Fields8 K;                    // next stage state (storage order)
@@ -104,15 +105,17 @@ __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __rest
       const double yk = a.y.f[k][home];
       a.acc.f[k][home] = d[k];
       a.K.f[k][home] = yk + d[k] * c->half_dt;
-      if (a.k1s.f[0]) a.k1s.f[k][home] = d[k];
+      if (a.k1s.f[k]) a.k1s.f[k][home] = d[k];
     }
   } else if (a.stage == 2 || a.stage == 3) {
     const double dts = a.stage == 2 ? c->half_dt : c->dt;
+    const Fields8& ks = a.stage == 2 ? a.k2s : a.k3s;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const double yk = a.y.f[k][home];
       a.acc.f[k][home] = a.acc.f[k][home] + 2 * d[k];
       a.K.f[k][home] = yk + d[k] * dts;
+      if (ks.f[k]) ks.f[k][home] = d[k];
     }
   } else if (a.stage == 5) {
 #pragma unroll
@@ -120,7 +123,7 @@ __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __rest
       const double yk = a.y.f[k][home], k1 = a.acc.f[k][home];
       a.acc.f[k][home] = k1 + 4 * d[k];
       a.K.f[k][home] = yk + c->dt * (2 * d[k] - k1);
-      if (a.k2s.f[0]) a.k2s.f[k][home] = d[k];
+      if (a.k2s.f[k]) a.k2s.f[k][home] = d[k];
     }
   } else {
 #pragma unroll

@@ -156,7 +156,8 @@ struct StepArgs {
   const SceneConst* sc;
   Fields8 y;                           // state after the base step, storage order
   Fields8 rate;                        // mean rate of the base step: y = y_start + tau rate
-  Fields8 k1, k2;                      // its first two stage rates
+  Fields8 k1, k2, k3;                  // its stage rates: RK4 k2, k3 (fields 2..7); Kutta k1 (2, 3), k2 (2..7)
+  int order;                           // base step: 3 (Kutta) or 4 (RK4)
   const double *h, *m;                 // storage order
   double* vprev;                       // by global id (2 per particle)
   const float* wsum;                   // by node
@@ -187,28 +188,40 @@ __global__ void k_step_consts(uint32_t nscene, const SceneConst* __restrict__ sc
 }
 
 // The 16 raw node features of node p (state field order x, y, vx, vy, rho,
-// Sxx, Sxy, Syy).  The base step is Kutta's method over tau with stage
-// increments a_i = tau k_i; for a linear system y' = A y,
-// D2 = 2 (a2 - a1) = (tau A)^2 y and D3 = a3 - 2 a2 + a1 = (tau A)^3 y, and
-// the step's error against the exact flow starts with (tau A)^4 y / 24: one
-// more application of the SPH operator to D3, which the message passing
-// can learn.  Features:
+// Sxx, Sxy, Syy).  The base step is an explicit RK step over tau with stage
+// increments a_i = tau k_i.  For a linear system y' = A y the stage
+// differences give powers of tau A applied to y:
+//   Kutta:  D2 = 2 (a2 - a1) = (tau A)^2 y,  D3 = a3 - 2 a2 + a1 = (tau A)^3 y;
+//   RK4:    D2 = 2 (a2 - a1),  D3 = 4 (a3 - a2) = (tau A)^3 y,
+//           D4 = 4 (a4 - a1 - D2 - D3/2) = 4 (a4 - 2 a3 + a1) = (tau A)^4 y;
+// with the mean rate m (k4 = 6 m - k1 - 2 k2 - 2 k3, k3 = 6 m - k1 - 4 k2)
+// these need RK4's k2, k3 (D4 = 4 tau (6 m - 2 k2 - 4 k3)) and Kutta's k1 (for
+// the velocity's D2) and k2 (D3 = 6 tau (m - k2)).
+// the step's error against the exact flow starts with (tau A)^(p+1) y/(p+1)!
+// (order p): one more application of the SPH operator to the highest
+// difference, which the message passing can learn.  Features:
 //   (v - v_prev) / c0 (2), S / K (3), rho/rho0 - 1 (state at the start),
-//   D2 v / c0 (2), D3 v / c0 (2), D3 S / K (3), D3 rho / rho0,
-//   c0 tau / h, kernel-weight sum.
+//   Dlo v / c0 (2), Dhi v / c0 (2), Dhi S / K (3), Dhi rho / rho0,
+//   c0 tau / h, kernel-weight sum,
+// with (Dlo, Dhi) = (D2, D3) for Kutta and (D3, D4) for RK4.
 __device__ inline void node_features(const StepArgs& a, uint32_t p, float* f) {
   const uint32_t g = a.gid[p];
   const SceneConst& c = a.sc[a.scene[p]];
   const double tau = tau_of(c);
   const double c0 = sqrt(c.c0sq), K = c.K, rho0 = c.rho0, h = a.h[p];
-  double st[8], d2[8], d3[8];
+  double st[8], dlo[8], dhi[8];
 #pragma unroll
   for (int k = 2; k < 8; k++) {
-    const double m = a.rate.f[k][p], k1 = a.k1.f[k][p], k2 = a.k2.f[k][p];
-    const double k3 = 6.0 * m - k1 - 4.0 * k2;
+    const double m = a.rate.f[k][p], k2 = a.k2.f[k][p];
     st[k] = a.y.f[k][p] - tau * m;
-    d2[k] = 2.0 * tau * (k2 - k1);
-    d3[k] = tau * (k3 - 2.0 * k2 + k1);
+    if (a.order == 4) {
+      const double k3 = a.k3.f[k][p];
+      dlo[k] = k < 4 ? 4.0 * tau * (k3 - k2) : 0.0;
+      dhi[k] = 4.0 * tau * (6.0 * m - 2.0 * k2 - 4.0 * k3);
+    } else {
+      dlo[k] = k < 4 ? 2.0 * tau * (k2 - a.k1.f[k][p]) : 0.0;
+      dhi[k] = 6.0 * tau * (m - k2);
+    }
   }
   const double vpx = a.vprev[2 * (size_t)g], vpy = a.vprev[2 * (size_t)g + 1];
   f[0] = (float)((st[2] - vpx) / c0);
@@ -217,14 +230,14 @@ __device__ inline void node_features(const StepArgs& a, uint32_t p, float* f) {
   f[3] = (float)(st[6] / K);
   f[4] = (float)(st[7] / K);
   f[5] = (float)(st[4] / rho0 - 1.0);
-  f[6] = (float)(d2[2] / c0);
-  f[7] = (float)(d2[3] / c0);
-  f[8] = (float)(d3[2] / c0);
-  f[9] = (float)(d3[3] / c0);
-  f[10] = (float)(d3[5] / K);
-  f[11] = (float)(d3[6] / K);
-  f[12] = (float)(d3[7] / K);
-  f[13] = (float)(d3[4] / rho0);
+  f[6] = (float)(dlo[2] / c0);
+  f[7] = (float)(dlo[3] / c0);
+  f[8] = (float)(dhi[2] / c0);
+  f[9] = (float)(dhi[3] / c0);
+  f[10] = (float)(dhi[5] / K);
+  f[11] = (float)(dhi[6] / K);
+  f[12] = (float)(dhi[7] / K);
+  f[13] = (float)(dhi[4] / rho0);
   f[14] = (float)(c0 * tau / h);
   f[15] = a.wsum[p];
 }
@@ -648,7 +661,7 @@ struct mfree_surrogate {
   std::vector<void*> callocs;
   double *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
   SceneConst* sc4 = nullptr;  // constants of the base step (dt scaled by kStepsPerLearned)
-  Fields8 k1s{}, k2s{};       // stage rates of the base step (storage order)
+  Fields8 k1s{}, k2s{}, k3s{};  // stage rates of the base step (storage order; fields 2..7)
   double *cpart = nullptr, *cmean = nullptr;  // correction sums per chunk (5), means per body (4)
   uint32_t* inv = nullptr;
   uint32_t* comp = nullptr;  // per body: largest compression (float bits)
@@ -722,9 +735,13 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   SCHK(s, salloc(s->callocs, &s->sc4, c->S));
   SCHK(s, salloc(s->callocs, &s->cpart, 5 * (size_t)c->nchunks));
   SCHK(s, salloc(s->callocs, &s->cmean, 4 * (size_t)c->S));
-  for (int k = 0; k < 8; k++) {
-    SCHK(s, salloc(s->callocs, &s->k1s.f[k], N));
+  // the stage rates the features use (node_features): RK4 k2, k3 of v, rho,
+  // S; Kutta k1 of v and k2 of v, rho, S
+  s->k1s = s->k2s = s->k3s = Fields8{};
+  for (int k = 2; k < 8; k++) {
     SCHK(s, salloc(s->callocs, &s->k2s.f[k], N));
+    if (s->opt.base_order == 4) SCHK(s, salloc(s->callocs, &s->k3s.f[k], N));
+    else if (k < 4) SCHK(s, salloc(s->callocs, &s->k1s.f[k], N));
   }
   SCHK(s, salloc(s->callocs, &s->snap_sc, c->S));
   SCHK(s, salloc(s->callocs, &s->jsc, c->S));
@@ -760,6 +777,8 @@ StepArgs step_args(mfree_surrogate* s, mfree_ctx* c) {
   a.rate = c->acc;
   a.k1 = s->k1s;
   a.k2 = s->k2s;
+  a.k3 = s->k3s;
+  a.order = s->opt.base_order;
   a.h = c->h[b];
   a.m = c->m[b];
   a.vprev = s->vprev;
@@ -794,20 +813,22 @@ void enqueue_graph(mfree_surrogate* s, mfree_ctx* c, int b, bool edges) {
   launch_graph(c->stream, g, s->graph, edges);
 }
 
-// Base step of a learned step: one step of Kutta's third-order method over
-// tau = kStepsPerLearned dt that leaves the mean rate in acc and the first
-// two stage rates in k1s, k2s.  graph: the network's graph of the start state over the
-// stage-1 cell list (nodes = stage-1 sorted positions = storage indices of
+// Base step of a learned step: one RK step (RK4, or Kutta's third-order
+// method with base_order 3) over tau = kStepsPerLearned dt that leaves the
+// mean rate in acc and the stage rates k1, k2 (and k3 of RK4) in k1s, k2s,
+// k3s.  graph: the network's graph of the start state over the stage-1 cell
+// list (nodes = stage-1 sorted positions = storage indices of
 // the new buffer); on a mixed context whose bodies each have one h, the
 // stage-1 pair pass emits the edge lists.
 void enqueue_base(mfree_surrogate* s, mfree_ctx* c, bool graph, bool advance_time) {
   hipLaunchKernelGGL(k_step_consts, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc, s->sc4);
   RkPlan plan;
   plan.sc = s->sc4;
-  plan.order = 3;
+  plan.order = s->opt.base_order;
   plan.rate_out = true;
   plan.k1s = s->k1s;
   plan.k2s = s->k2s;
+  plan.k3s = s->k3s;
   plan.advance_time = false;
   if (graph) {
     const bool fused = c->opt.precision == MFREE_PRECISION_MIXED && s->uniform_h;
@@ -1167,6 +1188,7 @@ void mfree_surrogate_default_options(mfree_surrogate_options* o) {
   o->seed = 1;
   o->max_batch_nodes = 1u << 20;
   o->max_compression = 1.0f;
+  o->base_order = 4;
 }
 
 mfree_status mfree_surrogate_create(const mfree_surrogate_options* o, mfree_surrogate** out) {
@@ -1177,6 +1199,10 @@ mfree_status mfree_surrogate_create(const mfree_surrogate_options* o, mfree_surr
     s->opt = *o;
   else
     mfree_surrogate_default_options(&s->opt);
+  if (s->opt.base_order != 3 && s->opt.base_order != 4) {
+    delete s;
+    return MFREE_ERR_ARG;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= s->opt.device) {
     delete s;

@@ -65,13 +65,14 @@ def forward_backward(row_ptr, col, pos3, x, t, w, device=0, train=True):
 
 
 class Surrogate:
-    def __init__(self, device=0, seed=1, max_batch_nodes=1 << 20):
+    def __init__(self, device=0, seed=1, max_batch_nodes=1 << 20, base_order=4):
         L = _capi.lib()
         o = _capi.SurrogateOptions()
         L.mfree_surrogate_default_options(C.byref(o))
         o.device = device
         o.seed = seed
         o.max_batch_nodes = max_batch_nodes
+        o.base_order = base_order
         h = C.c_void_p()
         _capi.check(L.mfree_surrogate_create(C.byref(o), C.byref(h)), what="mfree_surrogate_create")
         self._h = h

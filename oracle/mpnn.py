@@ -12,9 +12,10 @@ restates the project's own definitions, in float64 numpy:
   backward       exact gradients of the mean-squared loss (the transposed
                  aggregation is a scatter over the edges, it does not assume
                  a symmetric graph)
-  kutta3         the base step, the learned step without the network: one
-                 step of Kutta's third-order method over tau of a derivative
-                 function, with its stage rates
+  rk_stages      the base step, the learned step without the network: one
+                 step of the classical RK4 (src/rk4.h:71-108) or of Kutta's
+                 third-order method over tau of a derivative function, with
+                 its stage rates
   features       the 16 node inputs of a learned step (start state, previous
                  velocity, the base step's stage differences)
   conserve       the correction's projection onto zero mass-weighted mean
@@ -25,11 +26,11 @@ backward(); torch.autograd in float64 against forward() and backward(); a
 per-edge Python loop against the vectorised aggregation and the adjoint
 identity of its transpose; radius_graph's weights against the
 reference-pinned cubic spline of the CPU oracle; the graph's symmetry;
-kutta3 against the degree-3 Taylor polynomial of exp(tau A) for a linear
-system y' = A y (the stability polynomial, which pins every coefficient of
-the tableau) and against the exact solution of uniformly accelerated
-motion; features against their invariance under a change of units and the
-stage differences against (tau A)^2 y and (tau A)^3 y of a linear system;
+rk_stages against the degree-3 / degree-4 Taylor polynomial of exp(tau A)
+for a linear system y' = A y (the stability polynomial, which pins every
+coefficient of the tableau) and against the exact solution of uniformly
+accelerated motion; features against their invariance under a change of
+units and the stage differences against (tau A)^p y of a linear system;
 targets and correction as inverse maps; conserve as the orthogonal
 projection of the mass-weighted inner product (idempotent, zero sums,
 closest point of the subspace).
@@ -219,27 +220,39 @@ def loss_and_grad(row_ptr, col, edge3, X, T, w):
 OUT_OF_FIELD = [0, 1, 2, 3, 7, 4, 5, 6]  # state field (x,y,vx,vy,rho,Sxx,Sxy,Syy) -> output index
 
 
-def kutta3(state8, deriv, tau):
-    """One step of Kutta's third-order method over tau of y' = deriv(y):
-    k2 at y + tau/2 k1, k3 at y - tau k1 + 2 tau k2,
-    y + tau/6 (k1 + 4 k2 + k3).  Returns (new state, k1, k2, k3)."""
+def rk_stages(state8, deriv, tau, order=4):
+    """One explicit RK step over tau of y' = deriv(y); returns (new state,
+    [k1, ..., k_order]).  order 4: the classical RK4; order 3: Kutta's
+    method (k2 at y + tau/2 k1, k3 at y - tau k1 + 2 tau k2,
+    y + tau/6 (k1 + 4 k2 + k3))."""
     y = state8
     k1 = deriv(y)
     k2 = deriv(y + 0.5 * tau * k1)
-    k3 = deriv(y - tau * k1 + 2.0 * tau * k2)
-    return y + tau / 6.0 * (k1 + 4.0 * k2 + k3), k1, k2, k3
+    if order == 3:
+        k3 = deriv(y - tau * k1 + 2.0 * tau * k2)
+        return y + tau / 6.0 * (k1 + 4.0 * k2 + k3), [k1, k2, k3]
+    k3 = deriv(y + 0.5 * tau * k2)
+    k4 = deriv(y + tau * k3)
+    return y + tau / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4), [k1, k2, k3, k4]
 
 
-def stage_differences(k1, k2, k3, tau):
-    """D2 = 2 (a2 - a1), D3 = a3 - 2 a2 + a1 with a_i = tau k_i: (tau A)^2 y
-    and (tau A)^3 y for a linear system y' = A y."""
-    return 2.0 * tau * (k2 - k1), tau * (k3 - 2.0 * k2 + k1)
+def stage_differences(ks, tau):
+    """(Dlo, Dhi) from the stage rates, a_i = tau k_i.  Kutta (3 rates):
+    D2 = 2 (a2 - a1), D3 = a3 - 2 a2 + a1; RK4 (4 rates): D3 = 4 (a3 - a2),
+    D4 = 4 (a4 - a1 - D2 - D3/2).  For y' = A y, Dp = (tau A)^p y."""
+    a = [tau * k for k in ks]
+    d2 = 2.0 * (a[1] - a[0])
+    if len(ks) == 3:
+        return d2, a[2] - 2.0 * a[1] + a[0]
+    d3 = 4.0 * (a[2] - a[1])
+    return d3, 4.0 * (a[3] - a[0] - d2 - 0.5 * d3)
 
 
-def features(start8, k1, k2, k3, vprev, h, wsum, tau, c0, K, rho0):
-    """16 raw node features (same order as the library's node_features)."""
+def features(start8, ks, vprev, h, wsum, tau, c0, K, rho0):
+    """16 raw node features (same order as the library's node_features);
+    ks = the base step's stage rates."""
     s = start8
-    d2, d3 = stage_differences(k1, k2, k3, tau)
+    dlo, dhi = stage_differences(ks, tau)
     f = np.zeros((s.shape[0], NF))
     f[:, 0] = (s[:, 2] - vprev[:, 0]) / c0
     f[:, 1] = (s[:, 3] - vprev[:, 1]) / c0
@@ -247,14 +260,14 @@ def features(start8, k1, k2, k3, vprev, h, wsum, tau, c0, K, rho0):
     f[:, 3] = s[:, 6] / K
     f[:, 4] = s[:, 7] / K
     f[:, 5] = s[:, 4] / rho0 - 1.0
-    f[:, 6] = d2[:, 2] / c0
-    f[:, 7] = d2[:, 3] / c0
-    f[:, 8] = d3[:, 2] / c0
-    f[:, 9] = d3[:, 3] / c0
-    f[:, 10] = d3[:, 5] / K
-    f[:, 11] = d3[:, 6] / K
-    f[:, 12] = d3[:, 7] / K
-    f[:, 13] = d3[:, 4] / rho0
+    f[:, 6] = dlo[:, 2] / c0
+    f[:, 7] = dlo[:, 3] / c0
+    f[:, 8] = dhi[:, 2] / c0
+    f[:, 9] = dhi[:, 3] / c0
+    f[:, 10] = dhi[:, 5] / K
+    f[:, 11] = dhi[:, 6] / K
+    f[:, 12] = dhi[:, 7] / K
+    f[:, 13] = dhi[:, 4] / rho0
     f[:, 14] = c0 * tau / h
     f[:, 15] = wsum
     return f

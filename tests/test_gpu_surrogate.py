@@ -173,10 +173,10 @@ def _scales(scene):
     return 4 * p.dt, c0, K, p.rho0
 
 
-def _oracle_base(scene, ps):
-    """The base step of state ps: one step of Kutta's third-order method
-    over tau = 4 dt of the oracle's UL derivatives (stable order).  Returns
-    (start state, new state, k1, k2, k3)."""
+def _oracle_base(scene, ps, order=4):
+    """The base step of state ps: one RK step (order 4: RK4, 3: Kutta) over
+    tau = 4 dt of the oracle's UL derivatives (stable order).  Returns
+    (start state, new state, stage rates)."""
     tau, _, _, _ = _scales(scene)
 
     def deriv(s8):
@@ -188,17 +188,19 @@ def _oracle_base(scene, ps):
         return d8
 
     s8 = ps.state8()
-    return (s8,) + om.kutta3(s8, deriv, tau)
+    new, ks = om.rk_stages(s8, deriv, tau, order)
+    return s8, new, ks
 
 
-def test_base_step_matches_oracle():
+@pytest.mark.parametrize("order", [3, 4])
+def test_base_step_matches_oracle(order):
     s = perturbed_rings(nbox=41, seed=5, dpos=0.2)
-    with _ctx(s) as sv, sg.Surrogate() as su:
+    with _ctx(s) as sv, sg.Surrogate(base_order=order) as su:
         su.record_velocity(sv)
         su.step(sv, use_net=False)
         got = sv.state(0).state8()
         t = sv.time(0)
-    s8, base, _, _, _ = _oracle_base(s, s.particles)
+    s8, base, _ = _oracle_base(s, s.particles, order)
     for k in range(8):
         # the increments, which carry the physics (the state itself is
         # dominated by the positions and the rest density)
@@ -222,16 +224,17 @@ def test_base_step_mixed_close_to_fp64():
         assert O.rel_l2(out[1][:, k] - base[:, k], out[0][:, k] - base[:, k]) < 1e-4, scenes.STATE_FIELDS[k]
 
 
-def test_learned_step_matches_oracle():
+@pytest.mark.parametrize("order", [3, 4])
+def test_learned_step_matches_oracle(order):
     """Network step on random weights: the correction added to the base
     update agrees with the FP64 oracle's features -> forward -> correction
     -> conservation projection."""
     s = perturbed_rings(nbox=41, seed=6, dpos=0.2)
     ps = s.particles
     tau, c0, K, rho0 = _scales(s)
-    s8, base, k1, k2, k3 = _oracle_base(s, ps)
+    s8, base, ks = _oracle_base(s, ps, order)
     row, col, e3, wsum = om.radius_graph(ps.x, ps.y, ps.h)
-    feats = om.features(s8, k1, k2, k3, s8[:, 2:4], ps.h, wsum, tau, c0, K, rho0)
+    feats = om.features(s8, ks, s8[:, 2:4], ps.h, wsum, tau, c0, K, rho0)
     fm, fs = feats.mean(0), feats.std(0)
     fs[fs < 1e-6 * np.abs(fm) + 1e-30] = 1.0  # constant features (the Courant number)
     norm = np.concatenate([fm, fs, np.zeros(om.NOUT), np.full(om.NOUT, 1e-3)]).astype(np.float32)
@@ -239,7 +242,7 @@ def test_learned_step_matches_oracle():
     X = (feats - norm[:om.NF]) / norm[om.NF:2 * om.NF]
     Y, _ = om.forward(row, col, e3, X, w.astype(np.float64))
     want = om.conserve(om.correction(Y, norm.astype(np.float64), ps.h, c0, K, rho0), ps.m)
-    with _ctx(s) as sv, sg.Surrogate() as su:
+    with _ctx(s) as sv, sg.Surrogate(base_order=order) as su:
         su.set_params(w, norm)
         su.record_velocity(sv)
         su.step(sv, use_net=True)
@@ -300,18 +303,27 @@ def _twins(scene, k):
     return [_ctx(scene, PRECISION_MIXED) for _ in range(k)]
 
 
-def test_training_and_ablation():
-    """Trains on rollouts of a small batch, then compares one learned step
-    with and without the network against 4 RK4 steps on held-out scenes."""
+# median gain over the velocity and stress errors the network must reach on
+# held-out scenes: Kutta's step leaves a larger, (tau A)^4-dominated error
+# for the network to remove than RK4 over the same 4 dt does
+ABLATION_GAIN = {3: 1.5, 4: 1.2}
+
+
+@pytest.mark.parametrize("order", [4, 3])
+def test_training_and_ablation(order):
+    """Trains on FP64 rollouts of a small batch, then compares one learned
+    step with and without the network against 4 RK4 steps on held-out
+    scenes (mixed precision, as in a rollout)."""
     train = _batch(24, seed=1, nmin=2000, nmax=6000)
-    print(f"[surrogate train] bodies={len(train)} n={sum(t.particles.n for t in train)}", flush=True)
-    with Solver(precision=PRECISION_MIXED) as sv, sg.Surrogate(seed=3, max_batch_nodes=40000) as su:
+    print(f"[surrogate train] order {order} bodies={len(train)} n={sum(t.particles.n for t in train)}", flush=True)
+    with Solver(precision=PRECISION_FP64) as sv, \
+            sg.Surrogate(seed=3, max_batch_nodes=40000, base_order=order) as su:
         for sc in train:
             sv.add_body(sc.particles, sc.params)
         su.collect(sv, ncapture=24, gap=4)
         print(f"[surrogate train] usable snapshots {su.snapshots} of {24 * 24}", flush=True)
         assert 0.5 * 24 * 24 < su.snapshots <= 24 * 24
-        loss = su.train(600, lr=2e-3)
+        loss = su.train(1000, lr=2e-3)
         print(f"[surrogate train] loss first {loss[:10].mean():.4f} last {loss[-50:].mean():.4f}", flush=True)
         assert np.all(np.isfinite(loss))
         assert loss[-50:].mean() < 0.7 * loss[:10].mean()
@@ -336,4 +348,4 @@ def test_training_and_ablation():
                   + " ".join(f"{g:.2f}" for g in gains[-1]), flush=True)
         gains = np.array(gains)
         # the network reduces the error of the velocity and stress update
-        assert np.median(gains[:, [0, 1, 3, 4, 5]]) > 1.5
+        assert np.median(gains[:, [0, 1, 3, 4, 5]]) > ABLATION_GAIN[order]

@@ -2,6 +2,8 @@
 differences, torch.autograd, per-edge loops, the reference-pinned cubic
 spline, graph symmetry and the adjoint identity of the aggregation.  CPU
 only."""
+import math
+
 import numpy as np
 import pytest
 
@@ -136,31 +138,35 @@ def _random_state(n, rng):
     return s8
 
 
-def test_kutta3_is_the_taylor_polynomial_for_linear_systems():
-    """For y' = A y one step is p(tau A) y with p the degree-3 Taylor
+@pytest.mark.parametrize("order", [3, 4])
+def test_rk_stages_are_the_taylor_polynomial_for_linear_systems(order):
+    """For y' = A y one step is p(tau A) y with p the degree-`order` Taylor
     polynomial of exp (every weight and node of the tableau enters one of
-    its coefficients), and the stage differences are (tau A)^2 y and
-    (tau A)^3 y."""
-    rng = np.random.default_rng(20)
+    its coefficients), and the stage differences are powers of tau A."""
+    rng = np.random.default_rng(20 + order)
     A = rng.standard_normal((8, 8))
     y = rng.standard_normal((5, 8))
     tau = 0.3
-    got, k1, k2, k3 = om.kutta3(y, lambda s: s @ A.T, tau)
+    got, ks = om.rk_stages(y, lambda s: s @ A.T, tau, order)
     M = tau * A
-    P = np.eye(8) + M + M @ M / 2 + M @ M @ M / 6
+    powers = [np.eye(8)]
+    for _ in range(5):
+        powers.append(powers[-1] @ M)
+    P = sum(powers[p] / math.factorial(p) for p in range(order + 1))
     np.testing.assert_allclose(got, y @ P.T, rtol=1e-13, atol=1e-14)
-    np.testing.assert_allclose(k1, y @ A.T, rtol=1e-13, atol=1e-14)
-    d2, d3 = om.stage_differences(k1, k2, k3, tau)
-    np.testing.assert_allclose(d2, y @ (M @ M).T, rtol=1e-12, atol=1e-14)
-    np.testing.assert_allclose(d3, y @ (M @ M @ M).T, rtol=1e-12, atol=1e-14)
-    # and not the degree-4 polynomial
-    assert np.abs(got - y @ (P + M @ M @ M @ M / 24).T).max() > 1e-6
+    np.testing.assert_allclose(ks[0], y @ A.T, rtol=1e-13, atol=1e-14)
+    dlo, dhi = om.stage_differences(ks, tau)
+    np.testing.assert_allclose(dlo, y @ powers[order - 1].T, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(dhi, y @ powers[order].T, rtol=1e-11, atol=1e-13)
+    # and not the next degree
+    assert np.abs(got - y @ (P + powers[order + 1] / math.factorial(order + 1)).T).max() > 1e-6
 
 
-def test_kutta3_is_exact_for_uniform_acceleration():
+@pytest.mark.parametrize("order", [3, 4])
+def test_rk_stages_are_exact_for_uniform_acceleration(order):
     """x' = v, v' = a, S' = const: the exact solution is a polynomial of
-    degree 2 in tau, which the method integrates exactly."""
-    rng = np.random.default_rng(31)
+    degree 2 in tau, which both methods integrate exactly."""
+    rng = np.random.default_rng(30 + order)
     n, tau = 50, 3e-3
     s8 = _random_state(n, rng)
     a = rng.standard_normal((n, 2))
@@ -173,7 +179,7 @@ def test_kutta3_is_exact_for_uniform_acceleration():
         d[:, 5:8] = Sdot
         return d
 
-    new, _, _, _ = om.kutta3(s8, deriv, tau)
+    new, _ = om.rk_stages(s8, deriv, tau, order)
     np.testing.assert_allclose(new[:, 0:2], s8[:, 0:2] + tau * s8[:, 2:4] + 0.5 * tau * tau * a, rtol=1e-13,
                                atol=1e-15)
     np.testing.assert_allclose(new[:, 2:4], s8[:, 2:4] + tau * a, rtol=1e-13, atol=1e-15)
@@ -181,7 +187,8 @@ def test_kutta3_is_exact_for_uniform_acceleration():
     np.testing.assert_array_equal(new[:, 4], s8[:, 4])
 
 
-def test_features_are_unit_invariant():
+@pytest.mark.parametrize("order", [3, 4])
+def test_features_are_unit_invariant(order):
     """Every feature is dimensionless: rescaling length, time, mass-density
     and stress units leaves the 16 features unchanged (a feature normalised
     by the wrong quantity would not be)."""
@@ -189,20 +196,19 @@ def test_features_are_unit_invariant():
     n = 40
     s8 = _random_state(n, rng)
     rates = np.array([1, 1, 1e3, 1e3, 1e4, 1e8, 1e8, 1e8])
-    k1, k2, k3 = (rng.standard_normal((n, 8)) * rates for _ in range(3))
+    ks = [rng.standard_normal((n, 8)) * rates for _ in range(order)]
     vprev = s8[:, 2:4] + 0.01 * rng.standard_normal((n, 2))
     h = 1e-3 * (1.0 + 0.1 * rng.random(n))
     wsum = rng.random(n)
     tau, c0, K, rho0 = 4e-7, 128.0, 1.6e7, 1010.0
-    f = om.features(s8, k1, k2, k3, vprev, h, wsum, tau, c0, K, rho0)
+    f = om.features(s8, ks, vprev, h, wsum, tau, c0, K, rho0)
     L, T, R, P = 3.7, 0.29, 2.3, 11.0  # length, time, density, stress units
     sc = np.array([L, L, L / T, L / T, R, P, P, P])   # state fields
     rt = sc / T                                       # their rates
-    g = om.features(s8 * sc, k1 * rt, k2 * rt, k3 * rt, vprev * L / T, h * L, wsum, tau * T, c0 * L / T, K * P,
-                    rho0 * R)
+    g = om.features(s8 * sc, [k * rt for k in ks], vprev * L / T, h * L, wsum, tau * T, c0 * L / T, K * P, rho0 * R)
     np.testing.assert_allclose(g, f, rtol=1e-10, atol=1e-15)
     # equal stage rates (a constant rate): no stage differences
-    z = om.features(s8, k1, k1, k1, vprev, h, wsum, tau, c0, K, rho0)
+    z = om.features(s8, [ks[0]] * order, vprev, h, wsum, tau, c0, K, rho0)
     np.testing.assert_array_equal(z[:, 6:14], 0.0)
 
 
