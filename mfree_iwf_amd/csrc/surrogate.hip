@@ -287,17 +287,32 @@ __global__ __launch_bounds__(kCorrThreads) void k_corr_partial(StepArgs a, const
   if (threadIdx.x < 5) part[5 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_corr_final(uint32_t nscene, const uint32_t* __restrict__ chunk_first,
-                             const double* __restrict__ part, double* __restrict__ cmean) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nscene) return;
+// One workgroup per body: strided sums over its chunks, then the same
+// fixed-order tree as k_corr_partial.
+__global__ __launch_bounds__(kCorrThreads) void k_corr_final(const uint32_t* __restrict__ chunk_first,
+                                                             const double* __restrict__ part,
+                                                             double* __restrict__ cmean) {
+  const uint32_t s = blockIdx.x;
   double t[5] = {0., 0., 0., 0., 0.};
-  for (uint32_t q = chunk_first[s]; q < chunk_first[s + 1]; q++) {
+  for (uint32_t q = chunk_first[s] + threadIdx.x; q < chunk_first[s + 1]; q += kCorrThreads) {
 #pragma unroll
     for (int k = 0; k < 5; k++) t[k] += part[5 * (size_t)q + k];
   }
+  __shared__ double red[5][kCorrThreads];
 #pragma unroll
-  for (int k = 0; k < 4; k++) cmean[4 * (size_t)s + k] = t[4] > 0. ? t[k] / t[4] : 0.;
+  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = t[k];
+  __syncthreads();
+  for (int w = kCorrThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) {
+    const double M = red[4][0];
+    cmean[4 * (size_t)s + threadIdx.x] = M > 0. ? red[threadIdx.x][0] / M : 0.;
+  }
 }
 
 // One thread per node: features (training data), base-update export,
@@ -871,7 +886,7 @@ void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
     a.Y = s->acts.Y;
     a.cmean = s->cmean;
     hipLaunchKernelGGL(k_corr_partial, dim3(c->nchunks), dim3(kCorrThreads), 0, c->stream, a, c->chunks, s->cpart);
-    hipLaunchKernelGGL(k_corr_final, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->chunk_first, s->cpart,
+    hipLaunchKernelGGL(k_corr_final, dim3(c->S), dim3(kCorrThreads), 0, c->stream, c->chunk_first, s->cpart,
                        s->cmean);
   }
   a.apply = use_net ? 1 : 0;
