@@ -71,14 +71,13 @@ struct UlArgs {
   Fields8 y;                    // y_n (storage order), updated at stage 4
   Fields8 K;                    // next stage state (storage order)
   Fields8 acc;                  // RK accumulator (storage order)
-  double* d8;                   // stage 0: derivatives by global id (by sorted position if d8_sorted)
-  int d8_sorted;
-  int accel_only;               // stage 0: only the acceleration, as 2 doubles per particle in d8
+  double* d8;                   // stage 0: derivatives by global id
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;
-  // learned-step graph emitted by the mixed per-particle pass (stage 0, not
-  // accel_only, bodies of uniform h): edges r < 2h over sorted positions
+  // learned-step graph emitted by the mixed per-particle pass (stage 1 of
+  // the learned step's base step, bodies of uniform h): edges r < 2h over
+  // sorted positions
   uint32_t* gcol;               // slot-major edge list (null: no graph)
   uint32_t* gdeg;
   float* gwsum;

@@ -199,13 +199,7 @@ __device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__
   if (!finite8(d)) atomicOr(a.err, EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t g = a.gid[home];
-    if (a.accel_only) {  // the acceleration only: 2 doubles per particle
-      double* o = a.d8 + 2 * (size_t)(a.d8_sorted ? i : g);
-      o[0] = d[2];
-      o[1] = d[3];
-      return;
-    }
-    double* o = a.d8 + 8 * (size_t)(a.d8_sorted ? i : g);
+    double* o = a.d8 + 8 * (size_t)g;
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {

@@ -110,13 +110,10 @@ __device__ inline Nb nb_from_rec(const PairRec32& r) {
 }
 
 // All four pair sums of one neighbour at separation (xij, yij), r2 = |x_ij|^2.
-// ACC: only the sums the acceleration needs (stress divergence, artificial
-// viscosity); the velocity gradient and XSPH sums stay zero.
 // Square roots and reciprocals are the hardware's single instructions
 // (v_sqrt_f32, v_rcp_f32: about 1 ulp) rather than the correctly rounded
 // sequences; each term then carries about one rounding more, well inside the
 // compensated sums' error.
-template <bool ACC>
 __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
   const float rij = __builtin_amdgcn_sqrtf(r2);
   const float q = rij * me.h1;
@@ -146,12 +143,10 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
   acc_add(S.Sxy_x, mj * (me.sxy + nj.sxy + Rxy) * wx);
   acc_add(S.Syy_y, mj * (me.syy + nj.syy + Ryy) * wy);
   const float dvx = nj.vx - me.vx, dvy = nj.vy - me.vy;
-  if constexpr (!ACC) {
-    acc_add(S.vx_x, dvx * wx * nj.qw);
-    acc_add(S.vx_y, dvx * wy * nj.qw);
-    acc_add(S.vy_x, dvy * wx * nj.qw);
-    acc_add(S.vy_y, dvy * wy * nj.qw);
-  }
+  acc_add(S.vx_x, dvx * wx * nj.qw);
+  acc_add(S.vx_y, dvx * wy * nj.qw);
+  acc_add(S.vy_x, dvy * wx * nj.qw);
+  acc_add(S.vy_y, dvy * wy * nj.qw);
   const float vp = -(dvx * xij + dvy * yij);
   const float irhoij = __builtin_amdgcn_rcpf(0.5f * (me.rho + nj.rho));
   if (!(vp >= 0.f)) {
@@ -162,11 +157,9 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
     acc_add(S.ax, -mj * piij * wx);
     acc_add(S.ay, -mj * piij * wy);
   }
-  if constexpr (!ACC) {
-    const float fx = -me.xeps_mi * w * irhoij;
-    acc_add(S.xs, -fx * dvx);
-    acc_add(S.ys, -fx * dvy);
-  }
+  const float fx = -me.xeps_mi * w * irhoij;
+  acc_add(S.xs, -fx * dvx);
+  acc_add(S.ys, -fx * dvy);
 }
 
 __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
@@ -205,13 +198,7 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
   if (!ok) atomicOr(a.err, EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t gid = a.gid[home];
-    if (a.accel_only) {  // the acceleration only: 2 doubles per particle
-      double* o = a.d8 + 2 * (size_t)(a.d8_sorted ? i : gid);
-      o[0] = d[2];
-      o[1] = d[3];
-      return;
-    }
-    double* o = a.d8 + 8 * (size_t)(a.d8_sorted ? i : gid);
+    double* o = a.d8 + 8 * (size_t)gid;
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {
@@ -241,7 +228,6 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
 
 // Sorted particle i straight from global memory, no buffering (the
 // tile kernel's path for tiles whose candidates do not fit in LDS).
-template <bool ACC>
 __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -269,7 +255,7 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
         const float yij = (pi.y - pj.y) + offy;
         const float r2 = xij * xij + yij * yij;
         if (r2 > me.cut2) continue;
-        pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
+        pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
       }
     }
   }
@@ -286,7 +272,7 @@ constexpr float kSplineNormF = 0.45472840883398667f;      // 10/(7 pi): 2D cubic
 // inside the cutoff in LDS (sorted position + cell offsets packed in 32
 // bits), then evaluate the pair terms on the compact list with full SIMD
 // occupancy.
-template <bool ACC, bool GRAPH = false>
+template <bool GRAPH>
 __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
@@ -309,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   // from the cell-relative positions (about 1e-6 relative on r^2), with the
   // FP64 predicate of the graph builder within 4e-6 of the cutoff, so the
   // edge set and order are those of k_graph
-  constexpr bool graph = GRAPH && !ACC;
+  constexpr bool graph = GRAPH;
   uint32_t gd = 0;
   float gws = kSplineNormF;
   const float gcut = 4.f * me.h * me.h;
@@ -336,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
           gd++;
         }
       }
-      pair_terms<ACC>(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
+      pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
     }
     nb = 0;
   };
@@ -386,7 +372,6 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
 constexpr int kTileCap = 768;  // staged candidates per tile (10-bit LDS index)
 constexpr int kTileNb = 32;    // compacted neighbours buffered per thread
 
-template <bool ACC>
 __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
   __shared__ float s_px[kTileCap], s_py[kTileCap];
   __shared__ float s_vx[kTileCap], s_vy[kTileCap], s_rho[kTileCap], s_qw[kTileCap], s_m[kTileCap];
@@ -471,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
     const float cellf = (float)g.cell;
     for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
       if (!staged) {
-        particle_global<ACC>(a, i);
+        particle_global(a, i);
         continue;
       }
       const uint32_t li = s_off[1] + (i - s_rs[1]);
@@ -515,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
           nj.Rxx = s_Rxx[lj];
           nj.Rxy = s_Rxy[lj];
           nj.Ryy = s_Ryy[lj];
-          pair_terms<ACC>(me, nj, xij, yij, xij * xij + yij * yij, S);
+          pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
         }
         nb = 0;
       };
@@ -553,13 +538,8 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {
   const dim3 grid(xcd_grid((a.n + kBlock - 1) / kBlock));
   if (a.tile_cells && a.tile_off && !a.gcol) {
-    if (a.accel_only)
-      hipLaunchKernelGGL(k_ul_tiles32<true>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
-    else
-      hipLaunchKernelGGL(k_ul_tiles32<false>, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
-  } else if (a.gcol && !a.accel_only) {
-    hipLaunchKernelGGL((k_ul_pairs32<false, true>), grid, dim3(kBlock), 0, st, a);
-  } else if (a.accel_only) {
+    hipLaunchKernelGGL(k_ul_tiles32, dim3(a.tile_grid), dim3(kBlock), 0, st, a);
+  } else if (a.gcol) {
     hipLaunchKernelGGL(k_ul_pairs32<true>, grid, dim3(kBlock), 0, st, a);
   } else {
     hipLaunchKernelGGL(k_ul_pairs32<false>, grid, dim3(kBlock), 0, st, a);
