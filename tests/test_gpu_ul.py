@@ -72,6 +72,30 @@ def test_derivatives_fp64_vs_oracle(seed):
     assert np.array_equal(aux[:, 13], oaux[:, 13])  # reference neighbour counts
 
 
+@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_MIXED])
+def test_dense_derivatives_vs_oracle_and_xcd_chunks(precision):
+    """h = 2.2 dx: about 60 neighbours inside the cutoff, so every lane fills
+    its 32-entry neighbour buffer at least once and the collective and guard
+    flushes of the pair pass both run; the result must not depend on how
+    tiles are dealt to the XCDs either (one tile per chunk, the default
+    64-tile chunks, one contiguous range per XCD)."""
+    s = perturbed_rings(nbox=61, seed=21, dpos=0.3, dv=5.0, dS=5e5, hdx=2.2, shuffle=True)
+    print(f"[dense] precision={precision} n={s.particles.n}", flush=True)
+    out = []
+    for chunk in (1, 0, 1 << 20):
+        with solver_for(s, precision=precision, xcd_chunk=chunk) as sv:
+            d8, _ = sv.derivatives()
+            sv.step(2)
+            out.append((d8, sv.state(0).state8()))
+    for d8, st in out[1:]:
+        assert np.array_equal(d8, out[0][0]) and np.array_equal(st, out[0][1])
+    rc, od8, _ = O.ul_derivatives(s.particles, s.params, order=O.ORDER_STABLE)
+    assert rc == O.OK
+    tol = 1e-12 if precision == PRECISION_FP64 else 2e-4
+    for k in range(8):
+        assert O.rel_l2(out[0][0][:, k], od8[:, k]) < tol, k
+
+
 def test_fp64_100_steps_config1():
     s = scenes.rings_config1()
     print(f"[fp64 100 steps] n={s.particles.n}", flush=True)
