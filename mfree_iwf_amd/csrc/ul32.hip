@@ -511,6 +511,9 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
         const float offx = (float)(1 - (int)r) * cellf;
         const uint32_t base = s_off[r] - s_rs[r];
         uint32_t jb = a.cell_start[rb + lo_j];
+        // collective flush before a row that may overflow a lane's buffer
+        // (as in k_ul_pairs32)
+        if (__any(nb + ((a.cell_start[rb + hi_j] - jb + 1) >> 1) > (uint32_t)kTileNb)) flush();
         for (uint32_t nj = lo_j; nj < hi_j; nj++) {
           const uint32_t ja = jb;
           jb = a.cell_start[rb + nj + 1];

@@ -229,11 +229,14 @@ def test_mixed_tiles_match_untiled(tc):
         assert O.rel_l2(d8[:, k], od8[:, k]) < 2e-4, k
 
 
-def test_mixed_tiles_lds_overflow_fallback():
-    # ~19 particles per cell: the candidates of a 32-cell tile exceed the LDS
-    # capacity and the tile is evaluated from global memory
+@pytest.mark.parametrize("tc", [4, 32])
+def test_mixed_tiles_dense(tc):
+    # ~19 particles per cell, ~60 neighbours inside the cutoff: with 4-cell
+    # tiles the candidates are staged in LDS and every lane's neighbour buffer
+    # fills (collective and guard flushes); with 32-cell tiles they exceed the
+    # LDS capacity and the tile is evaluated from global memory
     s = perturbed_rings(nbox=41, seed=12, dpos=0.3, hdx=2.2)
-    with solver_for(s, precision=PRECISION_MIXED, tile_cells=32) as sv:
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=tc) as sv:
         d8, _ = sv.derivatives(with_aux=False)
     rc, od8, _ = O.ul_derivatives(s.particles, s.params)
     for k in range(8):
