@@ -357,8 +357,8 @@ def main():
             print(msg, file=sys.stderr, flush=True)
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the design for sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
