@@ -143,7 +143,33 @@ def bench_surrogate(scene, args, ref_state, log):
         res["loss_last"] = float(np.mean(loss[-100:]))
         log(f"  trained {args.sur_iters} iterations in {res['train_s']:.1f} s, loss {res['loss_first']:.4f} -> "
             f"{res['loss_last']:.4f}")
-        return _rollout(su, scene, args, ref_state, res, log)
+        res = _rollout(su, scene, args, ref_state, res, log)
+        if ref_state is not None and "error_vs_fp64" in res:
+            res["base_step_only"] = _ablation(su, scene, args, ref_state, log)
+        return res
+
+
+def _ablation(su, scene, args, ref_state, log):
+    """The same judged rollout with the network's output switched off (decoder
+    weights, bias and output mean zero): what the base step alone reaches.
+    Not timed into any reported rate."""
+    from mfree_iwf_amd import PRECISION_MIXED, Solver, surrogate
+    w, norm = su.params()
+    z, zn = w.copy(), norm.copy()
+    shapes = surrogate.layout()
+    off = sum(r * c for r, c in shapes[:-2])
+    z[off:] = 0.0                        # decoder weights and bias
+    zn[2 * surrogate.NF:2 * surrogate.NF + surrogate.NOUT] = 0.0  # output mean
+    su.set_params(z, zn)
+    try:
+        with Solver(precision=PRECISION_MIXED) as sv:
+            sv.add_body(scene.particles, scene.params)
+            su.rollout(sv, args.accuracy_steps, group=args.sur_group, threshold=args.sur_threshold)
+            err = errors(sv.state(0), ref_state)
+    finally:
+        su.set_params(w, norm)
+    log(f"  base step alone vs fp64 after {args.accuracy_steps} steps: {err}")
+    return dict(error_vs_fp64=err)
 
 
 def _rollout(su, scene, args, ref_state, res, log):
