@@ -97,6 +97,8 @@ typedef struct mfree_options {
   uint32_t xcd_chunk;       /* UL pair passes: particle tiles per XCD chunk (consecutive tiles that
                                one XCD takes in turn); 0 = library default (64); a value of at
                                least tiles/8 gives each XCD one contiguous range */
+  double cell_scale;        /* mixed UL: cell side in units of 2 h_max (>= 1); 0 = autotuned with
+                               tile_cells (tile_cells = 0) or 1; FP64 contexts always use 1 */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
@@ -203,6 +205,11 @@ mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps,
 /* Tile size (cells per LDS tile) the mixed UL pair pass uses, after the
  * autotuner ran (first compute call); 0 when tiles are not used. */
 int32_t mfree_tile_cells(const mfree_ctx* ctx);
+
+/* Cell side of the mixed UL pair pass in units of the reference's 2 h_max
+ * (1 unless the autotuner chose larger cells); no counterpart in the
+ * reference, whose grid is fixed at 2 h_max (src/grid.h:83-115). */
+double mfree_cell_scale(const mfree_ctx* ctx);
 
 /* Total particle count and per-body counts. */
 uint64_t mfree_num_particles(const mfree_ctx* ctx);

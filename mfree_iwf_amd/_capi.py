@@ -44,7 +44,7 @@ class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
                 ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32),
-                ("xcd_chunk", C.c_uint32)]
+                ("xcd_chunk", C.c_uint32), ("cell_scale", C.c_double)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")
@@ -104,6 +104,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_uint64)]),
     "mfree_calibrate_copy": (C.c_int32, [C.c_int32, C.c_uint64, C.c_uint32, dptr]),
     "mfree_tile_cells": (C.c_int32, [C.c_void_p]),
+    "mfree_cell_scale": (C.c_double, [C.c_void_p]),
     "mfree_num_particles": (C.c_uint64, [C.c_void_p]),
     "mfree_num_bodies": (C.c_uint32, [C.c_void_p]),
     "mfree_scene_rings_count": (C.c_uint32, [C.POINTER(RingsSpec)]),

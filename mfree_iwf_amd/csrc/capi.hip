@@ -179,6 +179,7 @@ mfree_status commit(mfree_ctx* c) {
   }
   c->committed = true;
   if (!tl && c->opt.precision == MFREE_PRECISION_MIXED) {
+    c->cell_scale = c->opt.cell_scale >= 1.0 ? c->opt.cell_scale : 1.0;
     if (c->opt.tile_cells > 0) {
       c->tile_cells = (uint32_t)c->opt.tile_cells;
     } else if (c->opt.tile_cells == 0) {
@@ -196,7 +197,9 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.nchunks = c->nchunks;
   a.ncell_cap = c->N;
   a.deterministic = c->opt.deterministic;
-  a.kernel_width = 2.0;  // src/body.h:135,138
+  // src/body.h:135,138: side 2 h_max; the mixed pair pass may use larger
+  // cells (the 3x3 block still holds every pair r < 2h, with more candidates)
+  a.kernel_width = 2.0 * c->cell_scale;
   a.cell_min = 0.0;
   a.chunks = c->chunks;
   a.chunk_first = c->chunk_first;
@@ -308,14 +311,22 @@ void pair_launch(mfree_ctx* c, const UlArgs& u, std::vector<std::pair<hipEvent_t
 
 // Project autotuner for the mixed UL pair pass: times one complete stage
 // (binning, prepass, pair pass in derivative-evaluation mode) of the actual
-// bodies for each candidate tile size and keeps the fastest.  Results are
-// cached per (particle-count bucket, body count) for the process.
+// bodies for each candidate cell size (side 2 h_max times 1 or 1.25) and
+// tile size, and keeps the fastest.  Results are cached per
+// (particle-count bucket, body count) for the process.
 mfree_status autotune_tiles(mfree_ctx* c) {
-  static std::vector<std::pair<uint64_t, uint32_t>> cache;
+  struct Choice {
+    uint64_t key;
+    uint32_t tile_cells;
+    double cell_scale;
+  };
+  static std::vector<Choice> cache;
+  const bool forced = c->opt.cell_scale >= 1.0;  // the cell size is given: tiles only, not cached
   const uint64_t key = ((uint64_t)(64 - __builtin_clzll((uint64_t)c->N | 1)) << 32) | c->S;
   for (auto& e : cache)
-    if (e.first == key) {
-      c->tile_cells = e.second;
+    if (!forced && e.key == key) {
+      c->tile_cells = e.tile_cells;
+      c->cell_scale = e.cell_scale;
       return MFREE_OK;
     }
   double* d8 = nullptr;
@@ -324,37 +335,46 @@ mfree_status autotune_tiles(mfree_ctx* c) {
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   const uint32_t cand[] = {0, 8, 12, 16, 24, 32};
+  const double scales[] = {1.0, 1.25};
   float best = 1e30f;
   uint32_t best_tc = 0;
+  double best_scale = c->cell_scale;
   const int b = c->cur;
-  for (uint32_t tc : cand) {
-    c->tile_cells = tc;
-    float tmin = 1e30f;
-    for (int rep = 0; rep < 4; rep++) {
-      hipEventRecord(e0, c->stream);
-      launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
-      UlArgs u = ul_args(c, b);
-      u.src = c->y[b];
-      launch_ul_prepass(c->stream, u);
-      u.stage = 0;
-      u.d8 = d8;
-      launch_ul_pairs(c->stream, u);
-      hipEventRecord(e1, c->stream);
-      HIPCHK(c, hipEventSynchronize(e1));
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      if (rep > 0) tmin = std::min(tmin, ms);  // first run warms up
+  for (double sc : scales) {
+    if (forced) sc = c->opt.cell_scale;
+    for (uint32_t tc : cand) {
+      c->tile_cells = tc;
+      c->cell_scale = sc;
+      float tmin = 1e30f;
+      for (int rep = 0; rep < 4; rep++) {
+        hipEventRecord(e0, c->stream);
+        launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+        UlArgs u = ul_args(c, b);
+        u.src = c->y[b];
+        launch_ul_prepass(c->stream, u);
+        u.stage = 0;
+        u.d8 = d8;
+        launch_ul_pairs(c->stream, u);
+        hipEventRecord(e1, c->stream);
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) tmin = std::min(tmin, ms);  // first run warms up
+      }
+      if (tmin < best) {
+        best = tmin;
+        best_tc = tc;
+        best_scale = sc;
+      }
     }
-    if (tmin < best) {
-      best = tmin;
-      best_tc = tc;
-    }
+    if (forced) break;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(d8);
   c->tile_cells = best_tc;
-  cache.push_back({key, best_tc});
+  c->cell_scale = best_scale;
+  if (!forced) cache.push_back({key, best_tc, best_scale});
   uint32_t zero = 0;
   HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
   return MFREE_OK;
@@ -547,6 +567,8 @@ void mfree_default_options(mfree_options* o) {
   o->tl_h_search = 0.;
   o->tile_cells = 0;
   o->pipeline = MFREE_PIPELINE_FUSED;
+  o->xcd_chunk = 0;
+  o->cell_scale = 0.;
 }
 
 mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
@@ -853,7 +875,11 @@ mfree_status mfree_get_pairs(mfree_ctx* c, uint32_t body, uint32_t* row_ptr, uin
     return st;
   }
   const int b = c->cur;
-  launch_binning(c->stream, bin_args(c, c->y[b].f[0], c->y[b].f[1], b));
+  BinArgs ba = bin_args(c, c->y[b].f[0], c->y[b].f[1], b);
+  ba.kernel_width = 2.0;  // the reference's cells: the lists are its Verlet lists
+  ba.tile_cells = 0;
+  ba.tile_off = nullptr;
+  launch_binning(c->stream, ba);
   uint32_t *dcnt = nullptr, *drow = nullptr, *dcol = nullptr;
   HIPCHK(c, hipMalloc(&dcnt, (size_t)n * sizeof(uint32_t)));
   PairListArgs pa;
@@ -964,6 +990,8 @@ mfree_status mfree_calibrate_copy(int32_t device, uint64_t bytes, uint32_t reps,
 }
 
 int32_t mfree_tile_cells(const mfree_ctx* c) { return c ? (int32_t)c->tile_cells : 0; }
+
+double mfree_cell_scale(const mfree_ctx* c) { return c ? c->cell_scale : 0.0; }
 
 uint64_t mfree_num_particles(const mfree_ctx* c) { return c ? c->off.back() : 0; }
 uint32_t mfree_num_bodies(const mfree_ctx* c) { return c ? (uint32_t)c->prm.size() : 0; }

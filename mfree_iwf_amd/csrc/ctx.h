@@ -61,6 +61,7 @@ struct mfree_ctx {
   std::vector<SceneConst> h_sc;
   uint32_t* tile_off = nullptr;  // nscene+1
   uint32_t tile_cells = 0;       // 0: per-particle pair kernel
+  double cell_scale = 1.0;       // cell side = 2 h_max * cell_scale (mixed UL: autotuned; else 1)
   uint32_t tile_grid = 0;
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)

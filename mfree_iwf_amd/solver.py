@@ -14,7 +14,7 @@ from .scenes import ParticleSet
 class Solver:
     def __init__(self, method=_capi.METHOD_UL_MONAGHAN, precision=_capi.PRECISION_FP64, use_graph=True,
                  deterministic=True, max_nbh=200, device=0, tl_h_search=0.0, tile_cells=0,
-                 pipeline=_capi.PIPELINE_FUSED, xcd_chunk=0):
+                 pipeline=_capi.PIPELINE_FUSED, xcd_chunk=0, cell_scale=0.0):
         L = _capi.lib()
         opt = _capi.Options()
         L.mfree_default_options(C.byref(opt))
@@ -28,6 +28,7 @@ class Solver:
         opt.tile_cells = tile_cells
         opt.pipeline = pipeline
         opt.xcd_chunk = xcd_chunk
+        opt.cell_scale = cell_scale
         h = C.c_void_p()
         _capi.check(L.mfree_create(C.byref(opt), C.byref(h)), what="mfree_create")
         self._h = h
@@ -131,6 +132,11 @@ class Solver:
     def tile_cells(self):
         """Cells per LDS tile of the mixed pair pass (0: no tiles)."""
         return int(self._L.mfree_tile_cells(self._h))
+
+    @property
+    def cell_scale(self):
+        """Cell side of the mixed pair pass in units of 2 h_max."""
+        return float(self._L.mfree_cell_scale(self._h))
 
     # -- profiling
     def profile(self, on=True):

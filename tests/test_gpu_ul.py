@@ -243,6 +243,35 @@ def test_mixed_tiles_dense(tc):
         assert O.rel_l2(d8[:, k], od8[:, k]) < 2e-4, k
 
 
+@pytest.mark.parametrize("tc", [-1, 16])
+def test_mixed_larger_cells(tc):
+    """Cells of 1.25 x 2 h_max (an autotuner candidate): the 3x3 block still
+    holds every pair inside the cutoff, only with more candidates, so the
+    derivatives agree with the oracle and with the reference-sized cells, and
+    the Verlet-list helper still returns the reference's lists bit for bit."""
+    s = perturbed_rings(nbox=61, seed=23, dv=5.0, dS=5e5, shuffle=True)
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=tc, cell_scale=1.25) as sv:
+        d8, _ = sv.derivatives(with_aux=False)  # the first compute call fixes the cell size
+        assert sv.cell_scale == 1.25
+        row, col = sv.pairs(0)
+        sv.step(3)
+        a = sv.state(0)
+    with solver_for(s, precision=PRECISION_MIXED, tile_cells=-1) as sv:
+        e8, _ = sv.derivatives(with_aux=False)
+        assert sv.cell_scale == 1.0
+        sv.step(3)
+        b = sv.state(0)
+    rc, od8, _ = O.ul_derivatives(s.particles, s.params)
+    for k in range(8):
+        assert O.rel_l2(d8[:, k], e8[:, k]) < 1e-6, k
+        assert O.rel_l2(d8[:, k], od8[:, k]) < 2e-4, k
+    ps = s.particles
+    rc, orow, ocol = O.verlet_pairs(ps.x, ps.y, ps.h, order=O.ORDER_STABLE)
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+    ep, es = pos_stress_err(a, b)
+    assert ep < 1e-9 and es < 1e-5
+
+
 def test_mixed_autotuned_steps_vs_oracle():
     s = perturbed_rings(nbox=81, seed=15, dv=2.0, dS=1e5, shuffle=True)
     with solver_for(s, precision=PRECISION_MIXED) as sv:
