@@ -500,7 +500,17 @@ bool valid_particles(const mfree_particles* p, bool tl) {
   return true;
 }
 
+// Storage order -> idx order of one body: dst[gid - o] = src[s] over the
+// body's storage range [o, o + n).
+__global__ __launch_bounds__(256) void k_to_idx_order(uint32_t n, uint32_t o, const uint32_t* __restrict__ gid,
+                                                      const double* __restrict__ src, double* __restrict__ dst) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  dst[gid[o + s]] = src[o + s];
+}
+
 }  // namespace mfree
+
 
 extern "C" {
 
@@ -709,25 +719,30 @@ mfree_status mfree_get_state(mfree_ctx* c, uint32_t body, mfree_particles* host)
   if (!c || body >= c->prm.size() || !host) return MFREE_ERR_ARG;
   mfree_status st = commit(c);
   if (st) return st;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   const uint32_t o = c->off[body], n = c->off[body + 1] - o;
   const int b = c->cur;
-  std::vector<uint32_t> gid(n);
-  HIPCHK(c, hipMemcpy(gid.data(), c->gid[b] + o, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  std::vector<double> tmp(n);
-  double* dst[8] = {host->x, host->y, host->vx, host->vy, host->rho, host->Sxx, host->Sxy, host->Syy};
+  // back to idx order on the device (into the stage buffers K and acc, free
+  // between steps), then one copy per field
+  double* dst[10] = {host->x, host->y, host->vx, host->vy, host->rho, host->Sxx, host->Sxy, host->Syy,
+                     host->m, host->h};
+  const double* src[10];
+  double* tmp[10];
   for (int k = 0; k < 8; k++) {
-    if (!dst[k]) continue;
-    HIPCHK(c, hipMemcpy(tmp.data(), c->y[b].f[k] + o, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n; i++) dst[k][gid[i] - o] = tmp[i];
+    src[k] = c->y[b].f[k];
+    tmp[k] = c->K.f[k];
   }
-  double* cst[2] = {host->m, host->h};
-  double* dsrc[2] = {c->m[b], c->h[b]};
-  for (int k = 0; k < 2; k++) {
-    if (!cst[k]) continue;
-    HIPCHK(c, hipMemcpy(tmp.data(), dsrc[k] + o, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n; i++) cst[k][gid[i] - o] = tmp[i];
-  }
+  src[8] = c->m[b];
+  src[9] = c->h[b];
+  tmp[8] = c->acc.f[0];
+  tmp[9] = c->acc.f[1];
+  for (int k = 0; k < 10; k++)
+    if (dst[k] && n)
+      hipLaunchKernelGGL(k_to_idx_order, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, o, c->gid[b], src[k],
+                         tmp[k]);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 10; k++)
+    if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], tmp[k] + o, n * sizeof(double), hipMemcpyDeviceToHost));
   if (host->X)
     for (uint32_t i = 0; i < n; i++) host->X[i] = c->hX[o + i];
   if (host->Y)
