@@ -311,8 +311,11 @@ mfree_status mfree_surrogate_train(mfree_surrogate* s, uint32_t iters, float lr,
  * step's previous-velocity feature). */
 mfree_status mfree_surrogate_record_velocity(mfree_surrogate* s, mfree_ctx* ctx);
 
-/* One learned step of every body of ctx.  use_net = 0: base update only
- * (the method without the network). */
+/* One learned step of every body of ctx: the base step (one RK step over
+ * 4 dt, options.base_order) plus the network's correction, projected per body
+ * onto zero mass-weighted mean of the x, y, vx, vy increments.  use_net = 0:
+ * the base step only (the method without the network).  Like a plain step it
+ * moves ctx's storage to its other buffer. */
 mfree_status mfree_surrogate_step(mfree_surrogate* s, mfree_ctx* ctx, int32_t use_net);
 
 typedef struct mfree_judge_stats {
