@@ -327,19 +327,28 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   // column's candidates; a flush taken by each lane on its own inside the
   // scan runs once per lane, the others idle, in dense regions); the
   // in-scan flush guards the rare lane that still fills its buffer.
+  // Candidates in batches of 4 positions per lane (several gathers in
+  // flight; the kernel is bound by their latency).
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
     const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
-    if (__any(nb + ((jb - ja + 1) >> 1) > (uint32_t)kNbCap)) flush();
-    for (uint32_t j = ja; j < jb; j++) {
-      const double2 pj = a.pos64[j];
-      const double xij = pi.x - pj.x;
-      const double yij = pi.y - pj.y;
-      const double r2 = xij * xij + yij * yij;
-      cnt += (r2 <= radius2 + 1e-6) ? 1u : 0u;
-      if (j == i || r2 > cut2) continue;
-      my[nb * kBlock] = j;
-      if (++nb == kNbCap) flush();
+    if (__any(nb + ((jb - ja + 1) >> 1) + 4 > (uint32_t)kNbCap)) flush();
+    for (uint32_t j0 = ja; j0 < jb; j0 += 4) {
+      if (nb > kNbCap - 4) flush();  // guard: room for the batch
+      double2 pj[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) pj[e] = a.pos64[min(j0 + e, jb - 1)];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const uint32_t j = j0 + e;
+        if (j >= jb) continue;
+        const double xij = pi.x - pj[e].x;
+        const double yij = pi.y - pj[e].y;
+        const double r2 = xij * xij + yij * yij;
+        cnt += (r2 <= radius2 + 1e-6) ? 1u : 0u;
+        if (j == i || r2 > cut2) continue;
+        my[nb++ * kBlock] = j;
+      }
     }
   }
   flush();

@@ -265,6 +265,7 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
 }  // namespace
 
 constexpr int kNbCap = 32;  // compacted neighbours buffered per thread
+constexpr int kScanBatch = 4;  // candidate positions loaded together per lane
 constexpr uint32_t kMaxGraphDeg = 64;                     // learned-step graph: edge slots per node
 constexpr float kSplineNormF = 0.45472840883398667f;      // 10/(7 pi): 2D cubic spline factor times h^2
 
@@ -326,31 +327,40 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
     }
     nb = 0;
   };
-  // Buffer flushes are collective: before each column of the 3x3 block,
-  // the lanes flush together if any lane's buffer is likely to overflow in
-  // that column (half its candidates: about a third of a column lies
-  // inside the cutoff).  A flush taken by each lane on its own inside the
-  // scan would run once per lane, with the other lanes idle, wherever dense
-  // regions fill the buffers at different times; the in-scan flush below
-  // stays as the guard for the rare lane that still fills its buffer.
+  // Each column of the 3x3 block is one range of consecutive candidates (its
+  // 1-3 cells are consecutive in the cell order): its cell boundaries are
+  // loaded together and the candidates in batches of kScanBatch positions,
+  // so that several gathers are in flight per lane (the kernel is bound by
+  // their latency).  Buffer flushes are collective: before each column the
+  // lanes flush together if any lane's buffer is likely to overflow in it
+  // (half its candidates, about a third of a column lying inside the
+  // cutoff, plus one batch).  A flush taken by each lane on its own inside
+  // the scan would run once per lane, with the other lanes idle, wherever
+  // dense regions fill the buffers at different times; the flush before a
+  // batch stays as the guard for the rare lane that still fills its buffer.
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
-    const uint32_t rb = c->cell_offset + ni * g.ny;
+    const uint32_t rb = c->cell_offset + ni * g.ny + lo_j;
+    const uint32_t nc = hi_j - lo_j;  // 1..3 cells
+    const uint32_t b0 = a.cell_start[rb], b3 = a.cell_start[rb + nc];
+    const uint32_t b1 = nc > 1 ? a.cell_start[rb + 1] : 0xFFFFFFFFu;
+    const uint32_t b2 = nc > 2 ? a.cell_start[rb + 2] : 0xFFFFFFFFu;
+    if (__any(nb + ((b3 - b0 + 1) >> 1) + kScanBatch > (uint32_t)kNbCap)) flush();
     const float offx = (float)((int)gi - (int)ni) * cellf;
-    uint32_t jb = a.cell_start[rb + lo_j];
-    if (__any(nb + ((a.cell_start[rb + hi_j] - jb + 1) >> 1) > (uint32_t)kNbCap)) flush();
-    for (uint32_t nj = lo_j; nj < hi_j; nj++) {
-      const uint32_t ja = jb;
-      jb = a.cell_start[rb + nj + 1];
-      const float offy = (float)((int)gj - (int)nj) * cellf;
-      const uint32_t tag = ((gi - ni + 1) << 28) | ((gj - nj + 1) << 30);
-      for (uint32_t j = ja; j < jb; j++) {
-        if (j == i) continue;
-        const float2 pj = a.pos32[j];
-        const float xij = (pi.x - pj.x) + offx;
-        const float yij = (pi.y - pj.y) + offy;
-        if (xij * xij + yij * yij > me.cut2) continue;
-        my[nb * kBlock] = j | tag;
-        if (++nb == kNbCap) flush();
+    const uint32_t tx = (gi - ni + 1) << 28;
+    for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
+      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      float2 pj[kScanBatch];
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) pj[e] = a.pos32[min(j0 + e, b3 - 1)];
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) {
+        const uint32_t j = j0 + e;
+        if (j < b3 && j != i) {
+          const uint32_t nj = lo_j + (j >= b1 ? 1u : 0u) + (j >= b2 ? 1u : 0u);
+          const float xij = (pi.x - pj[e].x) + offx;
+          const float yij = (pi.y - pj[e].y) + (float)((int)gj - (int)nj) * cellf;
+          if (xij * xij + yij * yij <= me.cut2) my[nb++ * kBlock] = j | tx | ((gj - nj + 1) << 30);
+        }
       }
     }
   }
