@@ -300,11 +300,21 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   uint32_t gd = 0;
   float gws = kSplineNormF;
   const float gcut = 4.f * me.h * me.h;
+  // the buffered neighbours in order; the next one's position and record
+  // are loaded while the current one is evaluated (the last load repeats the
+  // last entry, in bounds)
   auto flush = [&]() {
+    uint32_t en = nb > 0 ? my[0] : 0u;
+    float2 pn = a.pos32[en & 0x0FFFFFFFu];
+    PairRec32 rn = a.rec32[en & 0x0FFFFFFFu];
     for (int k = 0; k < nb; k++) {
-      const uint32_t e = my[k * kBlock];
+      const uint32_t e = en;
       const uint32_t j = e & 0x0FFFFFFFu;
-      const float2 pj = a.pos32[j];
+      const float2 pj = pn;
+      const PairRec32 rj = rn;
+      en = my[min(k + 1, nb - 1) * kBlock];
+      pn = a.pos32[en & 0x0FFFFFFFu];
+      rn = a.rec32[en & 0x0FFFFFFFu];
       const float xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
       const float yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
       const float r2 = xij * xij + yij * yij;
@@ -323,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
           gd++;
         }
       }
-      pair_terms(me, nb_from_rec(a.rec32[j]), xij, yij, r2, S);
+      pair_terms(me, nb_from_rec(rj), xij, yij, r2, S);
     }
     nb = 0;
   };
