@@ -252,8 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   int nb = 0;
 
   // pair terms of one buffered neighbour, in the reference's operation order
-  auto pair = [&](uint32_t j) {
-    const double2 pj = a.pos64[j];
+  auto pair = [&](uint32_t j, const double2 pj, const PairRec64& rj) {
     const double xij = pi.x - pj.x;
     const double yij = pi.y - pj.y;
     const double r2 = xij * xij + yij * yij;
@@ -277,7 +276,6 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
         wy = yij * der * fac;
       }
     }
-    const PairRec64 rj = a.rec64[j];
     // derive_stress_monaghan
     double Rxx = 0., Rxy = 0., Ryy = 0.;
     if (use_as) {
@@ -316,8 +314,22 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
     xs += fx * vijx;
     ys += fx * vijy;
   };
+  // the buffered neighbours in order; the next one's position and record
+  // are loaded while the current one is evaluated (the last load repeats the
+  // last entry, in bounds)
   auto flush = [&]() {
-    for (int k = 0; k < nb; k++) pair(my[k * kBlock]);
+    uint32_t jn = nb > 0 ? my[0] : i;
+    double2 pn = a.pos64[jn];
+    PairRec64 rn = a.rec64[jn];
+    for (int k = 0; k < nb; k++) {
+      const uint32_t j = jn;
+      const double2 pj = pn;
+      const PairRec64 rj = rn;
+      jn = my[min(k + 1, nb - 1) * kBlock];
+      pn = a.pos64[jn];
+      rn = a.rec64[jn];
+      pair(j, pj, rj);
+    }
     nb = 0;
   };
 
