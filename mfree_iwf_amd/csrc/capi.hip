@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -320,15 +321,20 @@ mfree_status autotune_tiles(mfree_ctx* c) {
     uint32_t tile_cells;
     double cell_scale;
   };
+  // shared by the process's contexts, which may commit on different threads
   static std::vector<Choice> cache;
+  static std::mutex cache_mu;
   const bool forced = c->opt.cell_scale >= 1.0;  // the cell size is given: tiles only, not cached
   const uint64_t key = ((uint64_t)(64 - __builtin_clzll((uint64_t)c->N | 1)) << 32) | c->S;
-  for (auto& e : cache)
-    if (!forced && e.key == key) {
-      c->tile_cells = e.tile_cells;
-      c->cell_scale = e.cell_scale;
-      return MFREE_OK;
-    }
+  if (!forced) {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    for (auto& e : cache)
+      if (e.key == key) {
+        c->tile_cells = e.tile_cells;
+        c->cell_scale = e.cell_scale;
+        return MFREE_OK;
+      }
+  }
   double* d8 = nullptr;
   HIPCHK(c, hipMalloc(&d8, 8 * (size_t)c->N * sizeof(double)));
   hipEvent_t e0, e1;
@@ -374,7 +380,10 @@ mfree_status autotune_tiles(mfree_ctx* c) {
   hipFree(d8);
   c->tile_cells = best_tc;
   c->cell_scale = best_scale;
-  if (!forced) cache.push_back({key, best_tc, best_scale});
+  if (!forced) {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    cache.push_back({key, best_tc, best_scale});
+  }
   uint32_t zero = 0;
   HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
   return MFREE_OK;
