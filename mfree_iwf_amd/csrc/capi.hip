@@ -335,11 +335,21 @@ mfree_status autotune_tiles(mfree_ctx* c) {
         return MFREE_OK;
       }
   }
-  double* d8 = nullptr;
-  HIPCHK(c, hipMalloc(&d8, 8 * (size_t)c->N * sizeof(double)));
-  hipEvent_t e0, e1;
-  HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
+  // trial buffers, released on every return path
+  struct Scratch {
+    double* d8 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scratch() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (d8) (void)hipFree(d8);
+    }
+  } sr;
+  HIPCHK(c, hipMalloc(&sr.d8, 8 * (size_t)c->N * sizeof(double)));
+  HIPCHK(c, hipEventCreate(&sr.e0));
+  HIPCHK(c, hipEventCreate(&sr.e1));
+  double* const d8 = sr.d8;
+  const hipEvent_t e0 = sr.e0, e1 = sr.e1;
   const uint32_t cand[] = {0, 8, 12, 16, 24, 32};
   const double scales[] = {1.0, 1.25};
   float best = 1e30f;
@@ -375,9 +385,6 @@ mfree_status autotune_tiles(mfree_ctx* c) {
     }
     if (forced) break;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(d8);
   c->tile_cells = best_tc;
   c->cell_scale = best_scale;
   if (!forced) {
