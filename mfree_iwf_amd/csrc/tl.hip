@@ -125,6 +125,9 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   tl_xv(a, i, t, xi, yi, vxi, vyi);
   const double uxi = xi - a.X[i], uyi = yi - a.Y[i];
   A Hxx, Hxy, Hyx, Hyy, Fxx, Fxy, Fyx, Fyy;
+  // unrolled: the next neighbours' loads are issued with the current one's
+  // (the sums keep their order)
+#pragma unroll 4
   for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
     const uint32_t j = a.col[q];
     T w[3];
@@ -217,6 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   const T ci = (T)a.cs[i];
   const T alpha = (T)c->alpha, beta = (T)c->beta, eta = (T)c->eta;
   A Pxx_x, Pyx_y, Pxy_x, Pyy_y, ax, ay;
+#pragma unroll 4
   for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
     const uint32_t j = a.col[q];
     T w[3];
