@@ -44,10 +44,9 @@ struct TlState {
   float* w32 = nullptr;               // FP32 copy for the mixed path
   uint64_t npairs = 0;
   double* qw = nullptr;               // quadrature weight m/rho at precompute
-  double* P4 = nullptr;               // nominal stress per particle (4N)
-  double* cs = nullptr;               // sound speed sqrt(K/rho) per particle
+  double* rec = nullptr;              // pass-1 records for pass 2 (96 B per particle)
   double* part8 = nullptr;            // pass-1 derivative parts (4N: rho_t, S_t)
-  double* diag = nullptr;             // H, Fdot per particle (8N), for diagnostics
+  double* diag = nullptr;             // H, Fdot, P per particle (12N), for diagnostics
   Fields8 Kb{};                       // second stage buffer: pass 2 reads the
                                       // neighbours' stage state while writing the next
   std::vector<void*> allocs;

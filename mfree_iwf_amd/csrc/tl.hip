@@ -41,7 +41,8 @@ struct TlArgs {
   const double* qw;
   const double *m, *h;
   Fields8 src;
-  double *P4, *cs, *part8, *diag;
+  void* rec;  // TlRec64 / TlRec32 per particle, written by pass 1
+  double *part8, *diag;
   int stage;
   Fields8 y, K, acc;
   double* d8;
@@ -89,15 +90,32 @@ struct Kahan32 {
   __device__ double val() const { return (double)s - (double)c; }
 };
 
+// What pass 2 gathers of a neighbour, written by pass 1 into one record per
+// particle: the stage position and velocity with the strip BCs applied, the
+// nominal stress P and the viscosity's per-particle factors.  The mixed
+// record keeps everything but x and v in FP32 -- pass 2 converts those to
+// FP32 at use anyway, so the stored values are the ones it would compute --
+// and fits one 64-byte line.
+struct __attribute__((aligned(16))) TlRec64 {
+  double x, y, vx, vy, Pxx, Pxy, Pyx, Pyy, m, c, h, rho;
+};
+struct __attribute__((aligned(16))) TlRec32 {
+  double x, y, vx, vy;
+  float Pxx, Pxy, Pyx, Pyy, m, c, h, rho;
+};
+static_assert(sizeof(TlRec64) == 96 && sizeof(TlRec32) == 64, "TL record layout");
+
 template <bool MIXED>
 struct TlTypes {
   using T = double;
   using A = Sum64;
+  using R = TlRec64;
 };
 template <>
 struct TlTypes<true> {
   using T = float;
   using A = Kahan32;
+  using R = TlRec32;
 };
 
 template <bool MIXED>
@@ -177,19 +195,31 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   const double J = m2_det(F2);
   const M2 sigma = m2(Sxx - p, Sxy, Sxy, Syy - p);
   const M2 P = m2_mul(m2_scale(J, m2_inv(F2)), sigma);
-  double* P4 = a.P4 + 4 * (size_t)i;
-  P4[0] = P.m[0][0];  // Pxx
-  P4[1] = P.m[1][0];  // Pxy
-  P4[2] = P.m[0][1];  // Pyx
-  P4[3] = P.m[1][1];  // Pyy
-  a.cs[i] = sqrt(c->K / rho);
+  typename TlTypes<MIXED>::R r;
+  r.x = xi;
+  r.y = yi;
+  r.vx = vxi;
+  r.vy = vyi;
+  r.Pxx = (T)P.m[0][0];
+  r.Pxy = (T)P.m[1][0];
+  r.Pyx = (T)P.m[0][1];
+  r.Pyy = (T)P.m[1][1];
+  r.m = (T)a.m[i];
+  r.c = (T)sqrt(c->K / rho);
+  r.h = (T)a.h[i];
+  r.rho = (T)rho;
+  ((typename TlTypes<MIXED>::R*)a.rec)[i] = r;
   double* pp = a.part8 + 4 * (size_t)i;
   pp[0] = rho_t;
   pp[1] = 0. + t00;
   pp[2] = 0. + t01;
   pp[3] = 0. + t11;
   if (a.stage == 0 && a.aux) {
-    double* dg = a.diag + 8 * (size_t)i;
+    double* dg = a.diag + 12 * (size_t)i;
+    dg[8] = P.m[0][0];
+    dg[9] = P.m[1][0];
+    dg[10] = P.m[0][1];
+    dg[11] = P.m[1][1];
     dg[0] = hxx;
     dg[1] = hxy;
     dg[2] = hyx;
@@ -207,17 +237,17 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   using A = typename TlTypes<MIXED>::A;
   const uint32_t i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  using R = typename TlTypes<MIXED>::R;
   const SceneConst* __restrict__ c = a.sc + a.scene[i];
   const double t = c->time;
   const double rho0 = c->rho0;
-  const double* Pi = a.P4 + 4 * (size_t)i;
-  const T Pxxi = (T)Pi[0], Pxyi = (T)Pi[1], Pyxi = (T)Pi[2], Pyyi = (T)Pi[3];
+  const R* __restrict__ rec = (const R*)a.rec;
+  const R ri = rec[i];
+  const T Pxxi = ri.Pxx, Pxyi = ri.Pxy, Pyxi = ri.Pyx, Pyyi = ri.Pyy;
   const T r21 = (T)(1. / (rho0 * rho0));
   const T rho0T = (T)rho0;
-  double xi, yi, vxi, vyi;
-  tl_xv(a, i, t, xi, yi, vxi, vyi);
-  const double hi = a.h[i], rhoi = a.src.f[4][i];
-  const T ci = (T)a.cs[i];
+  const double xi = ri.x, yi = ri.y, vxi = ri.vx, vyi = ri.vy;
+  const T hi = ri.h, rhoi = ri.rho, ci = ri.c;
   const T alpha = (T)c->alpha, beta = (T)c->beta, eta = (T)c->eta;
   A Pxx_x, Pyx_y, Pxy_x, Pyy_y, ax, ay;
 #pragma unroll 4
@@ -225,23 +255,21 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
     const uint32_t j = a.col[q];
     T w[3];
     load_w<MIXED>(a, q, w);
-    const T mj = (T)a.m[j];
-    const double* Pj = a.P4 + 4 * (size_t)j;
+    const R rj = rec[j];
+    const T mj = rj.m;
     // derive_stress_tl
-    Pxx_x.add(mj * (Pxxi * r21 + (T)Pj[0] * r21) * w[1] * rho0T);
-    Pyx_y.add(mj * (Pyxi * r21 + (T)Pj[2] * r21) * w[2] * rho0T);
-    Pxy_x.add(mj * (Pxyi * r21 + (T)Pj[1] * r21) * w[1] * rho0T);
-    Pyy_y.add(mj * (Pyyi * r21 + (T)Pj[3] * r21) * w[2] * rho0T);
+    Pxx_x.add(mj * (Pxxi * r21 + rj.Pxx * r21) * w[1] * rho0T);
+    Pyx_y.add(mj * (Pyxi * r21 + rj.Pyx * r21) * w[2] * rho0T);
+    Pxy_x.add(mj * (Pxyi * r21 + rj.Pxy * r21) * w[1] * rho0T);
+    Pyy_y.add(mj * (Pyyi * r21 + rj.Pyy * r21) * w[2] * rho0T);
     // correctors_mghn_artificial_viscosity
-    double xj, yj, vxj, vyj;
-    tl_xv(a, j, t, xj, yj, vxj, vyj);
-    const T vijx = (T)(vxi - vxj), vijy = (T)(vyi - vyj);
-    const T xij = (T)(xi - xj), yij = (T)(yi - yj);
+    const T vijx = (T)(vxi - rj.vx), vijy = (T)(vyi - rj.vy);
+    const T xij = (T)(xi - rj.x), yij = (T)(yi - rj.y);
     const T vp = vijx * xij + vijy * yij;
     if (!(vp >= (T)0)) {
-      const T cij = (T)0.5 * (ci + (T)a.cs[j]);
-      const T hij = (T)0.5 * ((T)hi + (T)a.h[j]);
-      const T rhoij = (T)0.5 * ((T)rhoi + (T)a.src.f[4][j]);
+      const T cij = (T)0.5 * (ci + rj.c);
+      const T hij = (T)0.5 * (hi + rj.h);
+      const T rhoij = (T)0.5 * (rhoi + rj.rho);
       const T r2 = xij * xij + yij * yij;
       const T muij = (hij * vp) / (r2 + eta * eta * hij * hij);
       const T piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
@@ -270,13 +298,9 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {
       double* x = a.aux + 16 * (size_t)i;
-      const double* dg = a.diag + 8 * (size_t)i;
+      const double* dg = a.diag + 12 * (size_t)i;
 #pragma unroll
-      for (int k = 0; k < 8; k++) x[k] = dg[k];
-      x[8] = Pi[0];
-      x[9] = Pi[1];
-      x[10] = Pi[2];
-      x[11] = Pi[3];
+      for (int k = 0; k < 12; k++) x[k] = dg[k];
       x[12] = pxx_x;
       x[13] = pxy_x;
       x[14] = pyx_y;
@@ -490,8 +514,7 @@ TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stag
   a.m = s.m;
   a.h = s.h;
   a.src = src;
-  a.P4 = t.P4;
-  a.cs = t.cs;
+  a.rec = t.rec;
   a.part8 = t.part8;
   a.diag = t.diag;
   a.stage = stage;
@@ -636,8 +659,8 @@ mfree_status tl_alloc(TlState& t, uint32_t N, const double* X, const double* Y,
     for (uint32_t k : right[b]) t.hbc[off[b] + k] = 2;
   }
   bool bad = talloc(t, &t.X, N) || talloc(t, &t.Y, N) || talloc(t, &t.bc, N) || talloc(t, &t.qw, N) ||
-             talloc(t, &t.P4, 4 * (size_t)N) || talloc(t, &t.cs, N) || talloc(t, &t.part8, 4 * (size_t)N) ||
-             talloc(t, &t.diag, 8 * (size_t)N);
+             talloc(t, &t.rec, 12 * (size_t)N) || talloc(t, &t.part8, 4 * (size_t)N) ||
+             talloc(t, &t.diag, 12 * (size_t)N);
   for (int k = 0; k < 8 && !bad; k++) bad = talloc(t, &t.Kb.f[k], N) != hipSuccess;
   if (bad) {
     t.msg = "hipMalloc failed";
