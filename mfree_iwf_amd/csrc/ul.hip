@@ -215,6 +215,9 @@ __device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__
 // Neighbour list entries buffered per thread in LDS (strided by the block
 // size so that lanes hit distinct banks).
 constexpr int kNbCap = 32;
+// Candidate positions loaded together per lane in the FP64 scan (8: 1.5 %
+// faster than 4 and 3.5 % faster than 2 at 1M particles).
+constexpr int kScanBatch64 = 8;
 
 __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
@@ -339,19 +342,19 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   // column's candidates; a flush taken by each lane on its own inside the
   // scan runs once per lane, the others idle, in dense regions); the
   // in-scan flush guards the rare lane that still fills its buffer.
-  // Candidates in batches of 4 positions per lane (several gathers in
-  // flight; the kernel is bound by their latency).
+  // Candidates in batches of kScanBatch64 positions per lane (several
+  // gathers in flight; the kernel is bound by their latency).
   for (uint32_t ni = lo_i; ni < hi_i; ni++) {
     const uint32_t rb = c->cell_offset + ni * g.ny;
     const uint32_t ja = a.cell_start[rb + lo_j], jb = a.cell_start[rb + hi_j];
-    if (__any(nb + ((jb - ja + 1) >> 1) + 4 > (uint32_t)kNbCap)) flush();
-    for (uint32_t j0 = ja; j0 < jb; j0 += 4) {
-      if (nb > kNbCap - 4) flush();  // guard: room for the batch
-      double2 pj[4];
+    if (__any(nb + ((jb - ja + 1) >> 1) + kScanBatch64 > (uint32_t)kNbCap)) flush();
+    for (uint32_t j0 = ja; j0 < jb; j0 += kScanBatch64) {
+      if (nb > kNbCap - kScanBatch64) flush();  // guard: room for the batch
+      double2 pj[kScanBatch64];
 #pragma unroll
-      for (int e = 0; e < 4; e++) pj[e] = a.pos64[min(j0 + e, jb - 1)];
+      for (int e = 0; e < kScanBatch64; e++) pj[e] = a.pos64[min(j0 + e, jb - 1)];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < kScanBatch64; e++) {
         const uint32_t j = j0 + e;
         if (j >= jb) continue;
         const double xij = pi.x - pj[e].x;
