@@ -251,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   const T hi = ri.h, rhoi = ri.rho, ci = ri.c;
   const T alpha = (T)c->alpha, beta = (T)c->beta, eta = (T)c->eta;
   A Pxx_x, Pyx_y, Pxy_x, Pyy_y, ax, ay;
-#pragma unroll(MIXED ? 16 : 8)
+  constexpr int kUnroll = MIXED ? 16 : 8;
+#pragma unroll kUnroll
   for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
     const uint32_t j = a.col[q];
     T w[3];
