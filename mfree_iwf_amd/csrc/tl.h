@@ -47,8 +47,8 @@ struct TlState {
   double* rec = nullptr;              // pass-1 records for pass 2 (96 B per particle)
   double* part8 = nullptr;            // pass-1 derivative parts (4N: rho_t, S_t)
   double* diag = nullptr;             // H, Fdot, P per particle (12N), for diagnostics
-  Fields8 Kb{};                       // second stage buffer: pass 2 reads the
-                                      // neighbours' stage state while writing the next
+  Fields8 Kb{};                       // second stage buffer: pass 1 reads the
+                                      // stage state, pass 2 writes the next one
   std::vector<void*> allocs;
   std::vector<double> hX, hY;
   std::vector<uint8_t> hbc;

@@ -497,8 +497,8 @@ hipError_t talloc(TlState& t, T** p, size_t count) {
   return e;
 }
 
-// src: stage state read by both passes; dst: next stage state written by
-// pass 2 (never the buffer other threads are reading).
+// src: stage state, read by pass 1 (pass 2 reads pass 1's records); dst:
+// next stage state written by pass 2.
 TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stage) {
   TlArgs a;
   memset(&a, 0, sizeof(a));
