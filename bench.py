@@ -198,9 +198,26 @@ def _rollout(su, scene, args, ref_state, res, log):
     return res
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical core count."""
+    name = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
 def cpu_baseline(scene, steps, log):
-    """The CPU oracle (reference-faithful FP64 restatement: per-stage sort,
-    Verlet lists, one loop per action) on one host core, bounded sample."""
+    """The CPU oracle on one host thread, bounded sample: the reference's
+    algorithm and action order in FP64 (per-stage std::sort in the reference's
+    order, Verlet lists, one loop per action), restated over SoA arrays and
+    built with g++ -O2 -- not the reference's own AoS build, so it runs faster
+    than the reference would."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     t0 = time.perf_counter()
@@ -209,10 +226,11 @@ def cpu_baseline(scene, steps, log):
     if rc != O.OK:
         raise RuntimeError(f"oracle failed with status {rc}")
     log(f"  cpu baseline: {steps} steps in {dt:.1f} s")
+    model, ncpu = cpu_model()
     return dict(value=scene.particles.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{scene.particles.n} particles (the full workload), {steps} RK4 steps, "
-                       f"oracle in reference order on 1 thread of {platform.processor() or platform.machine()}",
-                seconds=dt)
+                sample=f"{scene.particles.n} particles (the full workload), {steps} RK4 steps; SoA oracle "
+                       f"(g++ -O2) in the reference's sort order on 1 thread",
+                cpu_model=model, host_logical_cpus=ncpu, seconds=dt)
 
 
 def validate_fp64(scene, nsteps, log):

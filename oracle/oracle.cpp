@@ -193,7 +193,7 @@ int geometry(uint32_t n, const double* x, const double* y, const double* h, doub
 // Builds s.perm (sorted position -> idx), s.cells, and the Verlet lists in
 // sorted-position space.  x, y, h are in idx order.
 int build_lists(uint32_t n, const double* x, const double* y, const double* h, int order,
-                uint32_t max_nbh, UlStage& s, orc_geom* gout) {
+                uint32_t max_nbh, UlStage& s, orc_geom* gout, int nt = 1) {
   orc_geom g;
   int st = geometry(n, x, y, h, 2., &g);
   if (gout) *gout = g;
@@ -206,12 +206,17 @@ int build_lists(uint32_t n, const double* x, const double* y, const double* h, i
     if (s.hash[i] >= g.num_cell) return ORC_ERR_CELLS;
   }
   // src/body.h:137: std::sort of the idx-ordered array by hash.
-  for (uint32_t i = 0; i < n; i++) s.perm[i] = i;
   const uint32_t* hs = s.hash.data();
-  if (order == ORC_ORDER_REFERENCE)
+  if (order == ORC_ORDER_REFERENCE) {
+    for (uint32_t i = 0; i < n; i++) s.perm[i] = i;
     std::sort(s.perm.begin(), s.perm.end(), [hs](uint32_t a, uint32_t b) { return hs[a] < hs[b]; });
-  else
-    std::stable_sort(s.perm.begin(), s.perm.end(), [hs](uint32_t a, uint32_t b) { return hs[a] < hs[b]; });
+  } else {
+    // a stable sort of the idx-ordered array by hash is a counting sort
+    std::vector<uint32_t> start((size_t)g.num_cell + 1, 0);
+    for (uint32_t i = 0; i < n; i++) start[hs[i] + 1]++;
+    for (uint32_t b = 0; b < g.num_cell; b++) start[b + 1] += start[b];
+    for (uint32_t i = 0; i < n; i++) s.perm[start[hs[i]]++] = i;
+  }
 
   // src/grid.h:48-81 (get_cells)
   s.cells.assign((size_t)g.num_cell + 1, -1);
@@ -228,53 +233,74 @@ int build_lists(uint32_t n, const double* x, const double* y, const double* h, i
   }
 
   // src/grid.h:117-177 (construct_verlet_lists), lists in sorted positions.
-  s.nbh.clear();
-  s.nbh.reserve((size_t)n * 32);
-  for (unsigned int b = 0; b < g.num_cell; b++) {
-    unsigned int gi = b / g.ny;
-    unsigned int gj = b - gi * g.ny;
-    int low_i = (int)gi - 1 < 0 ? 0 : (int)gi - 1;
-    int low_j = (int)gj - 1 < 0 ? 0 : (int)gj - 1;
-    int high_i = gi + 2 > g.nx ? g.nx : gi + 2;
-    int high_j = gj + 2 > g.ny ? g.ny : gj + 2;
-    for (int i = s.cells[b]; i < s.cells[b + 1]; i++) {
-      uint32_t pi = s.perm[i];
-      double hi = h[pi];
-      double xi = x[pi];
-      double yi = y[pi];
-      double radius2 = hi * hi * 2 * 2;
-      uint64_t start = s.nbh.size();
-      s.rowp[i] = start;
-      for (int ni = low_i; ni < high_i; ni++) {
-        for (int nj = low_j; nj < high_j; nj++) {
-          int c = ni * (int)g.ny + nj;
-          for (int j = s.cells[c]; j < s.cells[c + 1]; j++) {
-            uint32_t pj = s.perm[j];
-            double xij = xi - x[pj];
-            double yij = yi - y[pj];
-            double r2 = xij * xij + yij * yij;
-            if (r2 <= radius2 + 1e-6) s.nbh.push_back((uint32_t)j);
+  // Rows are emitted cell by cell, i.e. in sorted-position order; the rows
+  // are built in two passes (count, then fill at the counted offsets) so
+  // that threads can share the work without changing any row.
+  // positions and h in sorted order first: the scan then reads neighbours
+  // from consecutive memory instead of through the permutation
+  std::vector<double> sx(n), sy(n), sh(n);
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const uint32_t k = s.perm[i];
+    sx[i] = x[k];
+    sy[i] = y[k];
+    sh[i] = h[k];
+  }
+  auto scan_row = [&](uint32_t i, uint32_t* out) -> uint64_t {
+    const unsigned int b = hs[s.perm[i]];
+    const unsigned int gi = b / g.ny;
+    const unsigned int gj = b - gi * g.ny;
+    const int low_i = (int)gi - 1 < 0 ? 0 : (int)gi - 1;
+    const int low_j = (int)gj - 1 < 0 ? 0 : (int)gj - 1;
+    const int high_i = gi + 2 > g.nx ? g.nx : gi + 2;
+    const int high_j = gj + 2 > g.ny ? g.ny : gj + 2;
+    const double hi = sh[i];
+    const double xi = sx[i];
+    const double yi = sy[i];
+    const double radius2 = hi * hi * 2 * 2;
+    uint64_t cnt = 0;
+    for (int ni = low_i; ni < high_i; ni++) {
+      for (int nj = low_j; nj < high_j; nj++) {
+        int c = ni * (int)g.ny + nj;
+        for (int j = s.cells[c]; j < s.cells[c + 1]; j++) {
+          double xij = xi - sx[j];
+          double yij = yi - sy[j];
+          double r2 = xij * xij + yij * yij;
+          if (r2 <= radius2 + 1e-6) {
+            if (out) out[cnt] = (uint32_t)j;
+            cnt++;
           }
         }
       }
-      uint64_t cnt = s.nbh.size() - start;
-      if (cnt >= max_nbh) return ORC_ERR_NBH_OVERFLOW;
-      if (cnt == 0) return ORC_ERR_NO_NEIGHBOR;
     }
+    return cnt;
+  };
+  int bad = ORC_OK;
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) s.rowp[i + 1] = scan_row((uint32_t)i, nullptr);
+  s.rowp[0] = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint64_t cnt = s.rowp[i + 1];
+    // the first failing row in emission order decides the status, as in the
+    // reference's sequential loop
+    if (bad == ORC_OK && cnt >= max_nbh) bad = ORC_ERR_NBH_OVERFLOW;
+    if (bad == ORC_OK && cnt == 0) bad = ORC_ERR_NO_NEIGHBOR;
+    s.rowp[i + 1] += s.rowp[i];
   }
-  // rows were emitted cell by cell == sorted position order
-  s.rowp[n] = s.nbh.size();
+  if (bad) return bad;
+  s.nbh.resize(s.rowp[n]);
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) scan_row((uint32_t)i, s.nbh.data() + s.rowp[i]);
   return ORC_OK;
 }
 
 // One UL Monaghan derivative evaluation on a stage state (idx order).
 int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint32_t max_nbh,
              UlStage& s, int threads, double* d8, double* aux) {
-  int rc = build_lists(n, st->x, st->y, st->h, order, max_nbh, s, nullptr);
+  int nt = set_threads(threads);
+  int rc = build_lists(n, st->x, st->y, st->h, order, max_nbh, s, nullptr, nt);
   if (rc) return rc;
   const uint32_t* P = s.perm.data();
-  int nt = set_threads(threads);
-  (void)nt;
   // gather into sorted order (the reference's actions run on the sorted array)
 #pragma omp parallel for schedule(static) num_threads(nt)
   for (int64_t i = 0; i < (int64_t)n; i++) {
@@ -343,6 +369,13 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
     s.Ryy[i] = sn2 * rot_rxx + ct2 * rot_ryy;
   }
 
+  // The stored pairs include every candidate with r^2 <= 4h^2 + 1e-6; those
+  // with q >= 2 have W = 0 and grad W = 0, so each of their terms below is
+  // an exact +-0 (for finite states).  In stable order (the validation mode)
+  // they are skipped: the sums keep every value of the reference's loops (up
+  // to the sign of a zero sum).  In the reference's order every stored pair
+  // is evaluated, as the reference does (the CPU baseline times this mode).
+  const bool skip0 = order == ORC_ORDER_STABLE;
   const double wdeltap = c->wdeltap, corr_exp = c->corr_exp;
   const double alpha = c->alpha, beta = c->beta, eta = c->eta, xeps = c->xsph_eps;
 #pragma omp parallel for schedule(static) num_threads(nt)
@@ -359,6 +392,7 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
       for (uint64_t q = R[i]; q < R[i + 1]; q++) {
         uint32_t j = NB[q];
         const double* w = W + 3 * q;
+        if (skip0 && w[0] == 0.) continue;  // q >= 2: W = grad W = 0, every term adds an exact +-0
         double Sxxj = s.Sxx[j] - s.p[j];
         double Sxyj = s.Sxy[j];
         double Syyj = s.Syy[j] - s.p[j];
@@ -389,6 +423,7 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
       for (uint64_t q = R[i]; q < R[i + 1]; q++) {
         uint32_t j = NB[q];
         const double* w = W + 3 * q;
+        if (skip0 && w[0] == 0.) continue;  // q >= 2: W = grad W = 0, every term adds an exact +-0
         double vxj = s.vx[j], vyj = s.vy[j];
         double quad_weight = s.qw[j];
         vx_x += (vxj - vxi) * w[1] * quad_weight;
@@ -408,6 +443,7 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
       for (uint64_t q = R[i]; q < R[i + 1]; q++) {
         uint32_t j = NB[q];
         const double* w = W + 3 * q;
+        if (skip0 && w[0] == 0.) continue;  // q >= 2: W = grad W = 0, every term adds an exact +-0
         double xj = s.x[j], yj = s.y[j], vxj = s.vx[j], vyj = s.vy[j], hj = s.h[j];
         double mj = s.m[j], rhoj = s.rho[j];
         double cj = sqrt(K / rhoj);
@@ -433,6 +469,7 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
       for (uint64_t q = R[i]; q < R[i + 1]; q++) {
         uint32_t j = NB[q];
         const double* w = W + 3 * q;
+        if (skip0 && w[0] == 0.) continue;  // q >= 2: W = grad W = 0, every term adds an exact +-0
         double vijx = vxi - s.vx[j];
         double vijy = vyi - s.vy[j];
         double rhoij = 0.5 * (rhoi + s.rho[j]);

@@ -45,9 +45,13 @@ extern "C" {
 
 /* Within-cell order after the per-stage sort.
  * REFERENCE: std::sort (unstable) of the idx-ordered pointer array by hash,
- *            exactly as src/body.h:137 -> same permutation as the reference.
- * STABLE:    std::stable_sort -> within a cell particles appear in idx order
- *            (the order the GPU library produces). */
+ *            exactly as src/body.h:137 -> same permutation as the reference;
+ *            the pair loops evaluate every stored pair, as the reference
+ *            does (the CPU-baseline mode).
+ * STABLE:    a stable sort (counting sort) -> within a cell particles appear
+ *            in idx order (the order the GPU library produces); the pair
+ *            loops skip stored pairs with q >= 2, whose terms are exact +-0
+ *            (the validation mode). */
 #define ORC_ORDER_REFERENCE 0
 #define ORC_ORDER_STABLE 1
 
