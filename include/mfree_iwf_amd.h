@@ -54,6 +54,7 @@ typedef int32_t mfree_status;
 #define MFREE_ERR_PIPELINE (-7)     /* action list not recognised */
 #define MFREE_ERR_CAPACITY (-9)     /* caller buffer too small */
 #define MFREE_ERR_UNSUPPORTED (-10) /* e.g. non-zero contact forces */
+#define MFREE_ERR_SCRATCH (-11)     /* input larger than an allocation-free entry point's fixed scratch */
 
 /* Methods (selected in the reference by the particle type, src/body.h:60-148). */
 #define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
@@ -164,6 +165,22 @@ mfree_status mfree_get_state(mfree_ctx* ctx, uint32_t body, mfree_particles* hos
  * deviatoric sum (m/rho) S:S/(4G).  Artificial viscosity dissipates, so the
  * total decays; the check compares its trajectory across precision modes. */
 mfree_status mfree_energy(mfree_ctx* ctx, uint32_t body, double* energy /* 3 */);
+
+/* Allocation-free variant of mfree_energy.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs are DEVICE arrays of one body on `device` (m, rho, vx, vy, Sxx, Sxy,
+ * Syy of `dev`; the other fields are ignored), n particles, constants from
+ * p (E, nu, rho0); the three energies (as mfree_energy) are written to the
+ * DEVICE array energy_dev[3].  Work is enqueued on `stream` (a hipStream_t;
+ * NULL = the null stream) and not waited for.  No memory is allocated or
+ * freed: the per-chunk partial sums live in a fixed file-scope device array
+ * of 96 KiB per GPU, so n is limited to 16,777,216 (larger n returns
+ * MFREE_ERR_SCRATCH before any device work).  Calls that overlap on one GPU
+ * share that array and race.  For the same arrays in the same order the
+ * result is bit-identical to mfree_energy's. */
+mfree_status mfree_energy_noalloc(int32_t device, void* stream, const mfree_params* p, const mfree_particles* dev,
+                                  uint32_t n, double* energy_dev);
 mfree_status mfree_get_time(mfree_ctx* ctx, uint32_t body, double* time);
 
 /* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for

@@ -20,6 +20,7 @@ ERR_CELLS = -6
 ERR_PIPELINE = -7
 ERR_CAPACITY = -9
 ERR_UNSUPPORTED = -10
+ERR_SCRATCH = -11
 
 METHOD_UL_MONAGHAN = 0
 METHOD_TL_RL = 1
@@ -95,6 +96,8 @@ SIGNATURES = {
     "mfree_get_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_time": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
     "mfree_energy": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
+    "mfree_energy_noalloc": (C.c_int32, [C.c_int32, C.c_void_p, C.POINTER(Params), C.POINTER(Particles),
+                                         C.c_uint32, C.c_void_p]),
     "mfree_step": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "mfree_eval_derivatives": (C.c_int32, [C.c_void_p, dptr, dptr]),
     "mfree_get_pairs": (C.c_int32, [C.c_void_p, C.c_uint32, u32ptr, u32ptr, C.c_uint64,

@@ -544,6 +544,7 @@ const char* mfree_status_string(mfree_status s) {
     case MFREE_ERR_PIPELINE: return "unrecognised action pipeline";
     case MFREE_ERR_CAPACITY: return "buffer too small";
     case MFREE_ERR_UNSUPPORTED: return "unsupported";
+    case MFREE_ERR_SCRATCH: return "input exceeds the fixed scratch of an allocation-free entry point";
     default: return "unknown status";
   }
 }
@@ -763,76 +764,6 @@ mfree_status mfree_get_state(mfree_ctx* c, uint32_t body, mfree_particles* host)
     for (uint32_t i = 0; i < n; i++) host->X[i] = c->hX[o + i];
   if (host->Y)
     for (uint32_t i = 0; i < n; i++) host->Y[i] = c->hY[o + i];
-  return MFREE_OK;
-}
-
-namespace mfree {
-
-// FP64 energy of one body (the path's energy-conservation check):
-//   kinetic      sum m |v|^2 / 2
-//   volumetric   sum m c0^2 (ln(rho/rho0) + rho0/rho - 1): the work stored by
-//                the linear EOS p = c0^2 (rho - rho0) (src/material.h:37-64),
-//                since d/drho of it is p / rho^2
-//   deviatoric   sum (m/rho) S:S / (4G), S:S = Sxx^2 + 2 Sxy^2 + Syy^2
-// Block partials are summed in a fixed order by one thread: the result does
-// not depend on scheduling.
-constexpr uint32_t kEnergyBlocks = 256;
-__global__ __launch_bounds__(256) void k_energy_partial(Fields8 y, const double* __restrict__ m, uint32_t o,
-                                                        uint32_t n, const SceneConst* __restrict__ sc, uint32_t body,
-                                                        double* __restrict__ part) {
-  const SceneConst s = sc[body];
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint32_t p = o + i;
-    const double mi = m[p], vx = y.f[2][p], vy = y.f[3][p], rho = y.f[4][p];
-    const double sxx = y.f[5][p], sxy = y.f[6][p], syy = y.f[7][p];
-    e0 += 0.5 * mi * (vx * vx + vy * vy);
-    const double d = (rho - s.rho0) / s.rho0;
-    e1 += mi * s.c0sq * (log1p(d) - d / (1.0 + d));
-    e2 += mi / rho * (sxx * sxx + 2.0 * sxy * sxy + syy * syy) / (2.0 * s.G2);
-  }
-  __shared__ double red[3][256];
-  red[0][threadIdx.x] = e0;
-  red[1][threadIdx.x] = e1;
-  red[2][threadIdx.x] = e2;
-  __syncthreads();
-  for (uint32_t w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
-}
-
-__global__ void k_energy_final(const double* __restrict__ part, uint32_t nb, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double e[3] = {0.0, 0.0, 0.0};
-  for (uint32_t b = 0; b < nb; b++)
-    for (int k = 0; k < 3; k++) e[k] += part[b * 3 + k];
-  for (int k = 0; k < 3; k++) out[k] = e[k];
-}
-
-}  // namespace mfree
-
-mfree_status mfree_energy(mfree_ctx* c, uint32_t body, double* energy) {
-  if (!c || body >= c->prm.size() || !energy) return MFREE_ERR_ARG;
-  mfree_status st = commit(c);
-  if (st) return st;
-  HIPCHK(c, hipSetDevice(c->opt.device));
-  if (!c->energy_buf) {
-    HIPCHK(c, hipMalloc(&c->energy_buf, (kEnergyBlocks * 3 + 3) * sizeof(double)));
-    c->allocs.push_back(c->energy_buf);
-  }
-  const uint32_t o = c->off[body], n = c->off[body + 1] - o;
-  const int b = c->cur;
-  const uint32_t nb = std::min<uint32_t>(kEnergyBlocks, (n + 255) / 256);
-  hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(256), 0, c->stream, c->y[b], c->m[b], o, n, c->sc, body,
-                     c->energy_buf);
-  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(1), 0, c->stream, c->energy_buf, nb, c->energy_buf + kEnergyBlocks * 3);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(energy, c->energy_buf + kEnergyBlocks * 3, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MFREE_OK;
 }
 

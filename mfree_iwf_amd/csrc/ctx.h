@@ -66,7 +66,8 @@ struct mfree_ctx {
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
-  double* energy_buf = nullptr;   // mfree_energy partials (in allocs)
+  double* energy_buf = nullptr;   // mfree_energy partials (in allocs), then the 3 results
+  size_t energy_out = 0;          // offset of the results in energy_buf
   // process-unique id: a learned stepper bound to a context compares ids, not
   // addresses (a new context may reuse a destroyed one's address)
   uint64_t serial = next_serial();
@@ -100,6 +101,8 @@ inline mfree_status fail(mfree_ctx* c, mfree_status s, const std::string& msg) {
 
 // Allocates and uploads everything at the first compute call.
 mfree_status commit(mfree_ctx* c);
+// Per-body device constants from the reference's formulas (src/simulation_data.cpp:32-47).
+void scene_consts(const mfree_params& p, uint32_t offset, uint32_t n, SceneConst& s);
 // Reads and clears the device error bits; maps them to a status.
 mfree_status check_err_bits(mfree_ctx* c);
 // Binning / UL-stage arguments over storage buffer b.
