@@ -55,6 +55,7 @@ typedef int32_t mfree_status;
 #define MFREE_ERR_CAPACITY (-9)     /* caller buffer too small */
 #define MFREE_ERR_UNSUPPORTED (-10) /* e.g. non-zero contact forces */
 #define MFREE_ERR_SCRATCH (-11)     /* input larger than an allocation-free entry point's fixed scratch */
+#define MFREE_ERR_SKIN (-12)        /* a particle moved more than half the neighbour-list skin in one step */
 
 /* Methods (selected in the reference by the particle type, src/body.h:60-148). */
 #define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
@@ -100,6 +101,12 @@ typedef struct mfree_options {
                                least tiles/8 gives each XCD one contiguous range */
   double cell_scale;        /* mixed UL: cell side in units of 2 h_max (>= 1); 0 = autotuned with
                                tile_cells (tile_cells = 0) or 1; FP64 contexts always use 1 */
+  int32_t neighbor_list;    /* mixed UL RK4 steps: 0 = default (on), 1 = on, -1 = off.  On: one
+                               binning and one neighbour list per step (cells 2 h_max + skin, skin
+                               >= 4 dt max|v| of the body), every stage evaluates the pairs
+                               r < 2 h_i of its own positions from the list; off: the reference's
+                               per-stage binning (as FP64 contexts always do) */
+  int32_t list_lanes;       /* neighbour-list stages: lanes per particle (1, 2 or 4; 0 = default) */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y

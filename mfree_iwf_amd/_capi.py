@@ -21,6 +21,7 @@ ERR_PIPELINE = -7
 ERR_CAPACITY = -9
 ERR_UNSUPPORTED = -10
 ERR_SCRATCH = -11
+ERR_SKIN = -12
 
 METHOD_UL_MONAGHAN = 0
 METHOD_TL_RL = 1
@@ -45,7 +46,8 @@ class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
                 ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32),
-                ("xcd_chunk", C.c_uint32), ("cell_scale", C.c_double)]
+                ("xcd_chunk", C.c_uint32), ("cell_scale", C.c_double), ("neighbor_list", C.c_int32),
+                ("list_lanes", C.c_int32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")

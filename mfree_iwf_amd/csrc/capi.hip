@@ -79,6 +79,8 @@ mfree_status check_err_bits(mfree_ctx* c) {
   if (bits & EB_NBH) return fail(c, MFREE_ERR_NBH_OVERFLOW, "a particle has >= max_nbh neighbours");
   if (bits & EB_NONEIGH) return fail(c, MFREE_ERR_NO_NEIGHBOR, "a particle has no neighbour");
   if (bits & EB_CELLS) return fail(c, MFREE_ERR_CELLS, "cell grid: num_cell >= n or a particle outside the grid");
+  if (bits & EB_SKIN)
+    return fail(c, MFREE_ERR_SKIN, "a particle moved more than half the neighbour-list skin within one step");
   return fail(c, MFREE_ERR_HIP, "unknown device error bits");
 }
 
@@ -147,6 +149,16 @@ mfree_status commit(mfree_ctx* c) {
     if (c->opt.precision == MFREE_PRECISION_MIXED) {
       HIPCHK(c, dalloc(c, &c->pos32, N));
       HIPCHK(c, dalloc(c, &c->rec32, N));
+      c->use_list = c->opt.pipeline == MFREE_PIPELINE_FUSED && c->opt.neighbor_list >= 0;
+      if (c->use_list) {
+        c->nl_cap = kListCap;
+        c->list_lanes = c->opt.list_lanes ? c->opt.list_lanes : 1;
+        HIPCHK(c, dalloc(c, &c->nl_col, (size_t)kListCap * N));
+        HIPCHK(c, dalloc(c, &c->nl_deg, N));
+        HIPCHK(c, dalloc(c, &c->pos32b, N));
+        HIPCHK(c, dalloc(c, &c->rec32b, N));
+        HIPCHK(c, dalloc(c, &c->vpart, c->nchunks));
+      }
     } else {
       HIPCHK(c, dalloc(c, &c->pos64, N));
       HIPCHK(c, dalloc(c, &c->rec64, N));
@@ -443,8 +455,70 @@ void enqueue_ul_rk(mfree_ctx* c, const RkPlan& plan, std::vector<std::pair<hipEv
   c->cur = b;
 }
 
+// One RK4 step of every UL body on the neighbour-list path (mixed
+// precision): one binning with the skin, the stage-1 prepass (storage moves
+// into the new cell order, records of stage 1), stage 1 (which also writes
+// the step's list), then stages 2-4 from the list; each stage writes the
+// records of the next one into the other record buffer.
+void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
+  hipStream_t st = c->stream;
+  const int a = c->cur, b = 1 - a;
+  BinArgs ba = bin_args(c, c->y[a].f[0], c->y[a].f[1], a);
+  ba.vx = c->y[a].f[2];
+  ba.vy = c->y[a].f[3];
+  ba.vpart = c->vpart;
+  ba.skin_min = kSkinMin;
+  ba.skin_vfac = kSkinVfac;
+  ba.tile_cells = 0;
+  ba.tile_off = nullptr;
+  launch_binning(st, ba);
+  UlArgs u = ul_args(c, a);
+  u.src = c->y[a];
+  u.reorder = 1;
+  u.dst = c->y[b];
+  u.m_dst = c->m[b];
+  u.h_dst = c->h[b];
+  u.scene_dst = c->scene[b];
+  u.gid_dst = c->gid[b];
+  launch_ul_prepass(st, u);
+  UlArgs p = ul_args(c, b);
+  p.home_identity = 1;
+  p.nl_col = c->nl_col;
+  p.nl_deg = c->nl_deg;
+  p.nl_cap = c->nl_cap;
+  float2* pos[2] = {c->pos32, c->pos32b};
+  PairRec32* rec[2] = {c->rec32, c->rec32b};
+  for (int k = 1; k <= 4; k++) {
+    p.stage = k;
+    p.src = k == 1 ? c->y[b] : c->K;
+    p.pos32 = pos[(k - 1) & 1];
+    p.rec32 = rec[(k - 1) & 1];
+    p.pos32_out = k < 4 ? pos[k & 1] : nullptr;
+    p.rec32_out = k < 4 ? rec[k & 1] : nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (evs) {
+      e0 = take_event(c);
+      e1 = take_event(c);
+      if (e0) hipEventRecord(e0, st);
+    }
+    if (k == 1)
+      launch_ul_list_first(st, p);
+    else
+      launch_ul_list(st, p, c->list_lanes);
+    if (evs && e0 && e1) {
+      hipEventRecord(e1, st);
+      evs->push_back({e0, e1});
+    }
+  }
+  hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);
+  c->cur = b;
+}
+
 void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
-  enqueue_ul_rk(c, RkPlan(), evs);
+  if (c->use_list)
+    enqueue_ul_list_step(c, evs);
+  else
+    enqueue_ul_rk(c, RkPlan(), evs);
 }
 
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {
@@ -545,6 +619,7 @@ const char* mfree_status_string(mfree_status s) {
     case MFREE_ERR_CAPACITY: return "buffer too small";
     case MFREE_ERR_UNSUPPORTED: return "unsupported";
     case MFREE_ERR_SCRATCH: return "input exceeds the fixed scratch of an allocation-free entry point";
+    case MFREE_ERR_SKIN: return "neighbour-list skin exceeded within a step";
     default: return "unknown status";
   }
 }
@@ -596,6 +671,8 @@ void mfree_default_options(mfree_options* o) {
   o->pipeline = MFREE_PIPELINE_FUSED;
   o->xcd_chunk = 0;
   o->cell_scale = 0.;
+  o->neighbor_list = 0;
+  o->list_lanes = 0;
 }
 
 mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
@@ -617,6 +694,17 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
   if (c->opt.pipeline != MFREE_PIPELINE_FUSED && c->opt.pipeline != MFREE_PIPELINE_REFERENCE) {
     delete c;
     return MFREE_ERR_ARG;
+  }
+  if (c->opt.neighbor_list < -1 || c->opt.neighbor_list > 1 ||
+      !(c->opt.list_lanes == 0 || c->opt.list_lanes == 1 || c->opt.list_lanes == 2 || c->opt.list_lanes == 4)) {
+    delete c;
+    return MFREE_ERR_ARG;
+  }
+  if (c->opt.neighbor_list == 1 &&
+      (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.precision != MFREE_PRECISION_MIXED ||
+       c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
+    delete c;
+    return MFREE_ERR_UNSUPPORTED;
   }
   if (c->opt.pipeline == MFREE_PIPELINE_REFERENCE &&
       (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.precision != MFREE_PRECISION_FP64 ||

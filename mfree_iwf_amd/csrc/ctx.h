@@ -63,6 +63,14 @@ struct mfree_ctx {
   uint32_t tile_cells = 0;       // 0: per-particle pair kernel
   double cell_scale = 1.0;       // cell side = 2 h_max * cell_scale (mixed UL: autotuned; else 1)
   uint32_t tile_grid = 0;
+  // neighbour-list path (mixed UL RK4 steps)
+  bool use_list = false;
+  int list_lanes = 1;
+  uint32_t nl_cap = 0;
+  uint32_t *nl_col = nullptr, *nl_deg = nullptr;
+  float2* pos32b = nullptr;      // second record buffer (records of the next stage)
+  PairRec32* rec32b = nullptr;
+  double* vpart = nullptr;       // max |v|^2 per binning chunk
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE

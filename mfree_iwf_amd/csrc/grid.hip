@@ -36,14 +36,18 @@ __device__ inline T wave_reduce_max(T v) {
   return v;
 }
 
-// One workgroup per chunk: partial {min x, min y, max x, max y, max h}.
+// One workgroup per chunk: partial {min x, min y, max x, max y, max h}, and
+// max |v|^2 into vpart when velocities are given.
 __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict__ ch,
                                                          const double* __restrict__ x,
                                                          const double* __restrict__ y,
                                                          const double* __restrict__ h,
-                                                         double* __restrict__ part) {
+                                                         double* __restrict__ part,
+                                                         const double* __restrict__ vx,
+                                                         const double* __restrict__ vy,
+                                                         double* __restrict__ vpart) {
   const Chunk c = ch[blockIdx.x];
-  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
+  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX, mxv = 0.0;
   for (uint32_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
     double xi = x[i], yi = y[i];
     mnx = fmin(xi, mnx);
@@ -51,13 +55,15 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
     mxx = fmax(xi, mxx);
     mxy = fmax(yi, mxy);
     mxh = fmax(h[i], mxh);
+    if (vx) mxv = fmax(vx[i] * vx[i] + vy[i] * vy[i], mxv);
   }
   mnx = wave_reduce_min(mnx);
   mny = wave_reduce_min(mny);
   mxx = wave_reduce_max(mxx);
   mxy = wave_reduce_max(mxy);
   mxh = wave_reduce_max(mxh);
-  __shared__ double sh[5][kBlock / 64];
+  mxv = wave_reduce_max(mxv);
+  __shared__ double sh[6][kBlock / 64];
   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0) {
     sh[0][w] = mnx;
@@ -65,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
     sh[2][w] = mxx;
     sh[3][w] = mxy;
     sh[4][w] = mxh;
+    sh[5][w] = mxv;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -74,9 +81,11 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
       sh[2][0] = fmax(sh[2][0], sh[2][k]);
       sh[3][0] = fmax(sh[3][0], sh[3][k]);
       sh[4][0] = fmax(sh[4][0], sh[4][k]);
+      sh[5][0] = fmax(sh[5][0], sh[5][k]);
     }
     double* o = part + 5 * (size_t)blockIdx.x;
     for (int k = 0; k < 5; k++) o[k] = sh[k][0];
+    if (vx) vpart[blockIdx.x] = sh[5][0];
   }
 }
 
@@ -86,9 +95,11 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
                                                        const SceneConst* __restrict__ sc,
                                                        const double* __restrict__ part, double kernel_width,
                                                        double cell_min, uint32_t tile_cells,
-                                                       SceneGeom* __restrict__ geom, uint32_t* __restrict__ err) {
+                                                       SceneGeom* __restrict__ geom, uint32_t* __restrict__ err,
+                                                       const double* __restrict__ vpart, double skin_min,
+                                                       double skin_vfac) {
   const uint32_t s = blockIdx.x;
-  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX;
+  double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX, mxv = 0.0;
   for (uint32_t c = chunk_first[s] + threadIdx.x; c < chunk_first[s + 1]; c += blockDim.x) {
     const double* p = part + 5 * (size_t)c;
     mnx = fmin(p[0], mnx);
@@ -96,13 +107,15 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
     mxx = fmax(p[2], mxx);
     mxy = fmax(p[3], mxy);
     mxh = fmax(p[4], mxh);
+    if (vpart) mxv = fmax(vpart[c], mxv);
   }
   mnx = wave_reduce_min(mnx);
   mny = wave_reduce_min(mny);
   mxx = wave_reduce_max(mxx);
   mxy = wave_reduce_max(mxy);
   mxh = wave_reduce_max(mxh);
-  __shared__ double sh[5][kBlock / 64];
+  mxv = wave_reduce_max(mxv);
+  __shared__ double sh[6][kBlock / 64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0) {
     sh[0][w] = mnx;
@@ -110,6 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
     sh[2][w] = mxx;
     sh[3][w] = mxy;
     sh[4][w] = mxh;
+    sh[5][w] = mxv;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -119,17 +133,22 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
     mxx = fmax(sh[2][k], mxx);
     mxy = fmax(sh[3][k], mxy);
     mxh = fmax(sh[4][k], mxh);
+    mxv = fmax(sh[5][k], mxv);
   }
   mnx = fmin(sh[0][0], mnx);
   mny = fmin(sh[1][0], mny);
   mxx = fmax(sh[2][0], mxx);
   mxy = fmax(sh[3][0], mxy);
   mxh = fmax(sh[4][0], mxh);
+  mxv = fmax(sh[5][0], mxv);
   mnx -= 1e-6;
   mny -= 1e-6;
   mxx += 1e-6;
   mxy += 1e-6;
-  const double dx = fmax(mxh * kernel_width, cell_min);
+  // neighbour-list binning: the skin covers twice the largest displacement
+  // of a step with skin_vfac/2 times the fastest particle's speed
+  const double skin = vpart ? fmax(skin_min * mxh, skin_vfac * sc[s].dt * sqrt(mxv)) : 0.0;
+  const double dx = vpart ? 2.0 * mxh + skin : fmax(mxh * kernel_width, cell_min);
   const double lx = mxx - mnx;
   const double ly = mxy - mny;
   const double fnx = ceil(lx / dx), fny = ceil(ly / dx);
@@ -155,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
   g.tpc = tile_cells ? (g.ny + tile_cells - 1) / tile_cells : 0;
   g.ntiles = g.nx * g.tpc;
   g.pad0 = g.pad1 = 0;
+  g.skin = skin;
   geom[s] = g;
 }
 
@@ -366,9 +386,12 @@ void launch_excl_scan(hipStream_t st, const uint32_t* in, uint32_t n, uint32_t* 
 }
 
 void launch_binning(hipStream_t st, const BinArgs& a) {
-  hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part);
+  const bool skin = a.vx && a.vy && a.vpart;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part,
+                     skin ? a.vx : nullptr, skin ? a.vy : nullptr, skin ? a.vpart : nullptr);
   hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part,
-                     a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err);
+                     a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err, skin ? a.vpart : nullptr,
+                     a.skin_min, a.skin_vfac);
   if (a.tile_cells && a.tile_off)
     hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kBlock), 0, st, a.nscene, a.geom, a.tile_off);
   // a.count is zero here: zeroed at allocation and cleared again by the scan

@@ -27,6 +27,13 @@ __device__ inline uint32_t xcd_tile() {
 // reads stay mostly within one XCD's chunk) while a dense stretch of tiles
 // (a contact zone) is shared by several XCDs.
 constexpr uint32_t kXcdChunkDefault = 64;
+// Neighbour-list path: slots per particle (about 30 entries at rest with the
+// default skin, so up to about 4x compression), the skin's floor in units of
+// h_max and its multiple of dt max|v| (twice the largest pair approach of a
+// step at twice the fastest particle's speed).
+constexpr uint32_t kListCap = 128;
+constexpr double kSkinMin = 0.25;
+constexpr double kSkinVfac = 4.0;
 // The same with the tiles dealt to the XCDs in chunks of `chunk`
 // consecutive tiles (XCD x takes chunks x, x + 8, ...; the remainder is
 // split evenly), so that a costly stretch of tiles is shared by several
@@ -47,6 +54,7 @@ enum : uint32_t {
   EB_NBH = 2u,        // >= max_nbh neighbours under the reference predicate
   EB_NONFINITE = 4u,  // NaN / Inf in a derivative
   EB_NONEIGH = 8u,    // particle without neighbour
+  EB_SKIN = 16u,      // a particle moved more than half the neighbour-list skin within a step
 };
 
 // Per-body constants, device resident.  Derived constants are computed on
@@ -66,6 +74,7 @@ struct SceneGeom {
   double bbmin_x, bbmin_y, cell;
   uint32_t nx, ny, num_cell, err;
   uint32_t tpc, ntiles, pad0, pad1;  // pair tiles: per cell column, in the body
+  double skin;                       // neighbour-list skin of the step (0: per-stage cells)
 };
 
 struct Chunk {
@@ -83,10 +92,12 @@ struct alignas(16) PairRec64 {
 
 // Mixed mode: 64-byte record of FP32 values; positions are stored relative
 // to the particle's own cell corner (exact FP64 subtraction, then rounded).
+// The stress enters the pair terms only as sigma / rho^2, which is stored
+// (computed in FP64).
 struct alignas(16) PairRec32 {
-  float vx, vy, rho, rho21;
+  float vx, vy, rho, pad2;
   float qw, m, h, c;
-  float sxx, sxy, syy, pad0;
+  float sxx_r, sxy_r, syy_r, pad0;  // sigma / rho^2
   float Rxx, Rxy, Ryy, pad1;
 };
 

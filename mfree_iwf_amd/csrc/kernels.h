@@ -28,6 +28,13 @@ struct BinArgs {
   uint32_t* err;
   uint32_t tile_cells;          // > 0: also enumerate pair tiles of this many cells
   uint32_t* tile_off;           // nscene+1: first tile of each body, [nscene] = total
+  // neighbour-list binning (once per step): cells of side 2 h_max + skin with
+  // skin = max(skin_min h_max, skin_vfac dt max|v|) per body; vx, vy null:
+  // cells of side kernel_width h_max (per-stage binning)
+  const double* vx = nullptr;
+  const double* vy = nullptr;
+  double* vpart = nullptr;      // max |v|^2 per chunk
+  double skin_min = 0.0, skin_vfac = 0.0;
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);
@@ -89,7 +96,76 @@ struct UlArgs {
   const uint32_t* tile_off;
   uint32_t tile_grid;           // workgroups of the tile kernel (multiple of 8)
   uint32_t xcd_chunk;           // per-particle pair kernels: tiles per XCD chunk (xcd_tile_chunked)
+  // neighbour-list path (mixed-precision RK4 steps): one binning and one list
+  // per step, cells of side 2 h_max + skin; stages read the list
+  uint32_t* nl_col;             // slot-major: entry k of particle i at nl_col[k * n + i],
+                                // sorted position | (cell dx + 1) << 28 | (cell dy + 1) << 30
+  uint32_t* nl_deg;             // entries per particle
+  uint32_t nl_cap;              // slots per particle
+  float2* pos32_out;            // records of the next stage (stages 1-3)
+  PairRec32* rec32_out;
 };
+
+// Monaghan's artificial stress (src/correctors.h:37-81) without
+// transcendentals: with d = (sxx - syy)/2 and r = sqrt(d^2 + sxy^2),
+// theta = atan2(2 sxy, sxx - syy)/2 has cos 2theta = d/r and sin 2theta = sxy/r
+// (cos 2theta = 1 for r = 0, as atan2(0, 0) = 0), so cos^2 = (1 + cos 2theta)/2,
+// sin^2 = (1 - cos 2theta)/2, cos sin = sin 2theta/2, and the rotated normal
+// stresses are the eigenvalues (sxx + syy)/2 +- r.  Used where the pair terms
+// read R in FP32 (mixed precision).
+__device__ inline void art_stress_alg(double sxx, double sxy, double syy, double rhoi, double eps, double& Rxx,
+                                      double& Rxy, double& Ryy) {
+  const double d = 0.5 * (sxx - syy);
+  const double r = sqrt(d * d + sxy * sxy);
+  double c2t = 1.0, s2t = 0.0;
+  if (r > 0.0) {
+    const double ir = 1.0 / r;
+    c2t = d * ir;
+    s2t = sxy * ir;
+  }
+  const double ct2 = 0.5 * (1.0 + c2t), sn2 = 0.5 * (1.0 - c2t), cs = 0.5 * s2t;
+  const double mean = 0.5 * (sxx + syy);
+  const double rot_sxx = mean + r, rot_syy = mean - r;
+  const double k = -eps / (rhoi * rhoi);
+  const double rot_rxx = rot_sxx > 0 ? k * rot_sxx : 0.0;
+  const double rot_ryy = rot_syy > 0 ? k * rot_syy : 0.0;
+  Rxx = ct2 * rot_rxx + sn2 * rot_ryy;
+  Rxy = cs * (rot_rxx - rot_ryy);
+  Ryy = sn2 * rot_rxx + ct2 * rot_ryy;
+}
+
+// The mixed-precision pair record of a particle from its FP64 stage state:
+// position relative to the corner of its cell (subtracted in FP64), and the
+// per-particle actions that precede the pair sums -- quadrature weight
+// (src/riemann_integration.h:28-33), EOS (src/material.h:37-64), artificial
+// stress -- in FP32.
+__device__ inline void make_rec32(const SceneConst* __restrict__ c, double cx, double cy, double x, double y,
+                                  double vx, double vy, double rho, double Sxx, double Sxy, double Syy, double m,
+                                  double h, float2& pos, PairRec32& r) {
+  const double qw = m / rho;
+  const double pr = c->c0sq * (rho - c->rho0);
+  const double sxx = Sxx - pr, syy = Syy - pr, sxy = Sxy;
+  double Rxx, Rxy, Ryy;
+  art_stress_alg(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
+  const double rho21 = 1. / (rho * rho);
+  pos = make_float2((float)(x - cx), (float)(y - cy));
+  r.vx = (float)vx;
+  r.vy = (float)vy;
+  r.rho = (float)rho;
+  r.pad2 = 0.f;
+  r.qw = (float)qw;
+  r.m = (float)m;
+  r.h = (float)h;
+  r.c = (float)sqrt(c->K / rho);
+  r.sxx_r = (float)(sxx * rho21);
+  r.sxy_r = (float)(sxy * rho21);
+  r.syy_r = (float)(syy * rho21);
+  r.pad0 = 0.f;
+  r.Rxx = (float)Rxx;
+  r.Rxy = (float)Rxy;
+  r.Ryy = (float)Ryy;
+  r.pad1 = 0.f;
+}
 
 // RK stage update of storage index home from its derivatives d.  RK4
 // (src/rk4.h:71-108) runs stages 1-4; Kutta's third-order method runs
@@ -136,6 +212,11 @@ __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __rest
 
 void launch_ul_prepass(hipStream_t st, const UlArgs& a);
 void launch_ul_pairs(hipStream_t st, const UlArgs& a);
+// Neighbour-list path (after the stage-1 binning and prepass): stage 1,
+// which also builds the step's list, and the stages 2-4 from the list with
+// `lanes` lanes per particle.
+void launch_ul_list_first(hipStream_t st, const UlArgs& a);
+void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes);
 
 // Verlet-list emission in the reference's order (parity helper).
 struct PairListArgs {

@@ -78,33 +78,6 @@ __device__ inline void art_stress(double sxx, double sxy, double syy, double rho
   Ryy = sn2 * rot_rxx + ct2 * rot_ryy;
 }
 
-// The same principal-axis construction without transcendentals (mixed mode,
-// whose pair terms read R in FP32): with d = (sxx - syy)/2 and
-// r = sqrt(d^2 + sxy^2), theta = atan2(2 sxy, sxx - syy)/2 has cos 2theta = d/r
-// and sin 2theta = sxy/r (cos 2theta = 1 for r = 0, as atan2(0, 0) = 0), so
-// cos^2 = (1 + cos 2theta)/2, sin^2 = (1 - cos 2theta)/2, cos sin = sin 2theta/2,
-// and the rotated normal stresses are the eigenvalues (sxx + syy)/2 +- r.
-__device__ inline void art_stress_alg(double sxx, double sxy, double syy, double rhoi, double eps, double& Rxx,
-                                      double& Rxy, double& Ryy) {
-  const double d = 0.5 * (sxx - syy);
-  const double r = sqrt(d * d + sxy * sxy);
-  double c2t = 1.0, s2t = 0.0;
-  if (r > 0.0) {
-    const double ir = 1.0 / r;
-    c2t = d * ir;
-    s2t = sxy * ir;
-  }
-  const double ct2 = 0.5 * (1.0 + c2t), sn2 = 0.5 * (1.0 - c2t), cs = 0.5 * s2t;
-  const double mean = 0.5 * (sxx + syy);
-  const double rot_sxx = mean + r, rot_syy = mean - r;
-  const double k = -eps / (rhoi * rhoi);
-  const double rot_rxx = rot_sxx > 0 ? k * rot_sxx : 0.0;
-  const double rot_ryy = rot_syy > 0 ? k * rot_syy : 0.0;
-  Rxx = ct2 * rot_rxx + sn2 * rot_ryy;
-  Rxy = cs * (rot_rxx - rot_ryy);
-  Ryy = sn2 * rot_rxx + ct2 * rot_ryy;
-}
-
 __device__ inline bool finite8(const double* d) {
   bool ok = true;
 #pragma unroll
@@ -128,16 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
   const double qw = m / rho;
   const double pr = c->c0sq * (rho - c->rho0);
   const double sxx = Sxx - pr, syy = Syy - pr, sxy = Sxy;
-  double Rxx, Rxy, Ryy;
-  if (a.mixed)
-    art_stress_alg(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
-  else
-    art_stress(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
-  const double rho21 = 1. / (rho * rho);
-  const double cs = sqrt(c->K / rho);
   const uint32_t key = a.key[p];
   a.meta[i] = make_uint2(key, s);
   if (!a.mixed) {
+    double Rxx, Rxy, Ryy;
+    art_stress(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
+    const double rho21 = 1. / (rho * rho);
+    const double cs = sqrt(c->K / rho);
     a.pos64[i] = make_double2(x, y);
     PairRec64 r;
     r.vx = vx;
@@ -163,24 +133,10 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
     const uint32_t ix = local / g.ny, iy = local - ix * g.ny;
     const double cx = g.bbmin_x + (double)ix * g.cell;
     const double cy = g.bbmin_y + (double)iy * g.cell;
-    a.pos32[i] = make_float2((float)(x - cx), (float)(y - cy));
+    float2 pos;
     PairRec32 r;
-    r.vx = (float)vx;
-    r.vy = (float)vy;
-    r.rho = (float)rho;
-    r.rho21 = (float)rho21;
-    r.qw = (float)qw;
-    r.m = (float)m;
-    r.h = (float)h;
-    r.c = (float)cs;
-    r.sxx = (float)sxx;
-    r.sxy = (float)sxy;
-    r.syy = (float)syy;
-    r.pad0 = 0.f;
-    r.Rxx = (float)Rxx;
-    r.Rxy = (float)Rxy;
-    r.Ryy = (float)Ryy;
-    r.pad1 = 0.f;
+    make_rec32(c, cx, cy, x, y, vx, vy, rho, Sxx, Sxy, Syy, m, h, pos, r);
+    a.pos32[i] = pos;
     a.rec32[i] = r;
   }
   if (a.reorder) {

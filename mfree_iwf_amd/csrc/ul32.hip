@@ -31,7 +31,7 @@ namespace mfree {
 namespace {
 
 struct Acc {
-  float s, c;
+  float s = 0.f, c = 0.f;
 };
 
 // Kahan compensated summation: c carries the negated low-order part lost
@@ -46,9 +46,9 @@ __device__ inline void acc_add(Acc& a, float x) {
 __device__ inline double acc_val(const Acc& a) { return (double)a.s - (double)a.c; }
 
 struct Sums {
-  Acc Sxx_x{0.f, 0.f}, Sxy_y{0.f, 0.f}, Sxy_x{0.f, 0.f}, Syy_y{0.f, 0.f};
-  Acc vx_x{0.f, 0.f}, vx_y{0.f, 0.f}, vy_x{0.f, 0.f}, vy_y{0.f, 0.f};
-  Acc ax{0.f, 0.f}, ay{0.f, 0.f}, xs{0.f, 0.f}, ys{0.f, 0.f};
+  Acc Sxx_x, Sxy_y, Sxy_x, Syy_y;
+  Acc vx_x, vx_y, vy_x, vy_y;
+  Acc ax, ay, xs, ys;
 };
 
 // What a neighbour contributes (sigma pre-multiplied by 1/rho^2).
@@ -100,9 +100,9 @@ __device__ inline Nb nb_from_rec(const PairRec32& r) {
   n.m = r.m;
   n.h = r.h;
   n.c = r.c;
-  n.sxx = r.sxx * r.rho21;
-  n.sxy = r.sxy * r.rho21;
-  n.syy = r.syy * r.rho21;
+  n.sxx = r.sxx_r;
+  n.sxy = r.sxy_r;
+  n.syy = r.syy_r;
   n.Rxx = r.Rxx;
   n.Rxy = r.Rxy;
   n.Ryy = r.Ryy;
@@ -556,6 +556,261 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
     }
     __syncthreads();  // LDS is reused by the next tile
   }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list path: one list per RK4 step
+//
+// The step's single binning uses cells of side 2 h_max + skin, with the skin
+// at least four times the distance the fastest particle of the body covers
+// in one time step.  Stage 1 (k_ul_list_first32) scans the 3x3 block once
+// and keeps every candidate within 2 h_i + skin; between stage 1
+// and any later stage of the step no pair's separation changes by more than
+// the skin, so the list holds every pair with r < 2 h_i at every stage, and
+// each stage evaluates exactly the pairs q < 2 of its own positions (the set
+// the reference re-bins for at every stage).  The epilogue of each stage
+// checks that the particle stays within half the skin of its stage-1
+// position (error bit EB_SKIN otherwise).  Sums run in list order.
+
+constexpr uint32_t kListMask = 0x0FFFFFFFu;
+
+// Epilogue of a list stage: FP64 derivatives of sorted particle i (storage
+// index i: the stage-1 prepass moved storage into sorted order), the RK4
+// stage update, and the record of the next stage.  The L lanes of the
+// particle hold the same sums; lane q stores the fields f with f % L == q.
+template <int L>
+__device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict__ c, const SceneGeom& g,
+                                   uint32_t key, uint32_t i, uint32_t q, const double* v) {
+  const double vxi = a.src.f[2][i], vyi = a.src.f[3][i], rhoi = a.src.f[4][i];
+  const double Sxx = a.src.f[5][i], Sxy = a.src.f[6][i], Syy = a.src.f[7][i];
+  const double dSxx_x = v[0] * rhoi, dSxy_y = v[1] * rhoi, dSxy_x = v[2] * rhoi, dSyy_y = v[3] * rhoi;
+  double t00, t01, t11;
+  jaumann64(c->G2, v[4], v[5], v[6], v[7], Sxx, Sxy, Syy, t00, t01, t11);
+  double d[8];
+  d[0] = v[10] + vxi;
+  d[1] = v[11] + vyi;
+  d[2] = v[8] + 1. / rhoi * (dSxx_x + dSxy_y);
+  d[3] = v[9] + 1. / rhoi * (dSxy_x + dSyy_y);
+  d[4] = -rhoi * (v[4] + v[7]);
+  d[5] = t00;
+  d[6] = t01;
+  d[7] = t11;
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 8; k++) ok = ok && isfinite(d[k]);
+  if (!ok && q == 0) atomicOr(a.err, EB_NONFINITE);
+  if (a.stage == 4) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k % L == (int)q) a.y.f[k][i] = a.y.f[k][i] + c->dt6 * (a.acc.f[k][i] + d[k]);
+    return;
+  }
+  const double dts = a.stage == 3 ? c->dt : c->half_dt;
+  double kn[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    kn[k] = a.y.f[k][i] + d[k] * dts;
+    if (k % L == (int)q) {
+      a.acc.f[k][i] = a.stage == 1 ? d[k] : a.acc.f[k][i] + 2 * d[k];
+      if (k >= 2) a.K.f[k][i] = kn[k];  // the positions of a stage live in its records
+    }
+  }
+  const double ux = d[0] * dts, uy = d[1] * dts, half = 0.5 * g.skin;
+  if (q == 0 && ux * ux + uy * uy > half * half) atomicOr(a.err, EB_SKIN);
+  const uint32_t local = key - c->cell_offset;
+  const uint32_t ix = local / g.ny, iy = local - ix * g.ny;
+  float2 pos;
+  PairRec32 r;
+  make_rec32(c, g.bbmin_x + (double)ix * g.cell, g.bbmin_y + (double)iy * g.cell, kn[0], kn[1], kn[2], kn[3], kn[4],
+             kn[5], kn[6], kn[7], a.m[i], a.h[i], pos, r);
+  if (q == 0) a.pos32_out[i] = pos;
+  const float4* rv = reinterpret_cast<const float4*>(&r);
+  float4* out = reinterpret_cast<float4*>(a.rec32_out + i);
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (k % L == (int)q) out[k] = rv[k];
+}
+
+// One RK4 stage (a.stage 1-4) from the step's list, L lanes per particle:
+// lane q takes the entries q, q + L, ...  In chunks of kListBatch entries per
+// lane, the entries inside the cutoff at this stage's positions are first
+// compacted in LDS (list word and position only), then evaluated with the
+// next one's position and record loaded ahead, so the expensive pair terms
+// run with every lane busy (about 70 % of the entries are inside the
+// cutoff).  The compensated partial sums of the L lanes are combined in
+// FP64 with cross-lane shuffles.
+constexpr int kListBatch = 32;
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_ul_list32(UlArgs a) {
+  static_assert(L == 1 || L == 2 || L == 4, "lanes per particle");
+  __shared__ uint32_t nb_lds[kListBatch * kBlock];
+  uint32_t* const my = nb_lds + threadIdx.x;
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * (kBlock / L) + threadIdx.x / L;
+  const uint32_t q = threadIdx.x % L;
+  if (i >= a.n) return;  // the L lanes of a particle leave together
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const float cellf = (float)g.cell;
+  const float2 pi = a.pos32[i];
+  const Me me = make_me(c, nb_from_rec(a.rec32[i]));
+  const uint32_t deg = a.nl_deg[i];
+  const uint32_t* __restrict__ col = a.nl_col + i;
+  Sums S;
+  auto sep = [&](uint32_t e, float2 pj, float& xij, float& yij) {
+    xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
+    yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
+  };
+  for (uint32_t base = q; base < deg; base += kListBatch * L) {
+    const uint32_t end = min(deg, base + kListBatch * L);
+    int nb = 0;
+    for (uint32_t k0 = base; k0 < end; k0 += kScanBatch * L) {
+      uint32_t e[kScanBatch];
+      float2 pj[kScanBatch];
+#pragma unroll
+      for (int t = 0; t < kScanBatch; t++) {
+        const uint32_t k = k0 + t * L;
+        e[t] = col[(size_t)(k < end ? k : base) * a.n];
+      }
+#pragma unroll
+      for (int t = 0; t < kScanBatch; t++) pj[t] = a.pos32[e[t] & kListMask];
+#pragma unroll
+      for (int t = 0; t < kScanBatch; t++) {
+        float xij, yij;
+        sep(e[t], pj[t], xij, yij);
+        if (k0 + t * L < end && xij * xij + yij * yij <= me.cut2) my[nb++ * kBlock] = e[t];
+      }
+    }
+    // the compacted entries in order, the next one's position and record
+    // loaded while the current one is evaluated (the last load repeats the
+    // last entry; with no entry it loads the particle itself)
+    uint32_t en = nb > 0 ? my[0] : i;
+    float2 pn = a.pos32[en & kListMask];
+    PairRec32 rn = a.rec32[en & kListMask];
+    for (int t = 0; t < nb; t++) {
+      const uint32_t e = en;
+      const float2 pj = pn;
+      const PairRec32 rj = rn;
+      en = my[min(t + 1, nb - 1) * kBlock];
+      pn = a.pos32[en & kListMask];
+      rn = a.rec32[en & kListMask];
+      float xij, yij;
+      sep(e, pj, xij, yij);
+      pair_terms(me, nb_from_rec(rj), xij, yij, xij * xij + yij * yij, S);
+    }
+  }
+  double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
+                  acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
+                  acc_val(S.ax),    acc_val(S.ay),    acc_val(S.xs),    acc_val(S.ys)};
+#pragma unroll
+  for (int o = 1; o < L; o <<= 1)
+#pragma unroll
+    for (int t = 0; t < 12; t++) v[t] += __shfl_xor(v[t], o, 64);
+  finish_list<L>(a, c, g, mt.x, i, q, v);
+}
+
+// Stage 1 of a list step in one pass: the 3x3 block of the step's cells is
+// scanned once; every candidate within 2 h_i + skin goes to the step's list
+// (slot-major), those inside the cutoff also to the LDS buffer, which is
+// evaluated as in k_ul_pairs32 (collective flushes before a column that may
+// overflow a lane's buffer).  The epilogue is the list stages' one.
+__global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
+  __shared__ uint32_t nb_lds[kNbCap * kBlock];
+  uint32_t* const my = nb_lds + threadIdx.x;
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+  const float cellf = (float)g.cell;
+  const float2 pi = a.pos32[i];
+  const Me me = make_me(c, nb_from_rec(a.rec32[i]));
+  const float rl = (float)(2.0 * a.h[i] + g.skin);
+  const float rl2 = rl * rl * 1.0001f;
+  Sums S;
+  int nb = 0;
+  uint32_t deg = 0;
+  auto flush = [&]() {
+    uint32_t en = nb > 0 ? my[0] : 0u;
+    float2 pn = a.pos32[en & kListMask];
+    PairRec32 rn = a.rec32[en & kListMask];
+    for (int k = 0; k < nb; k++) {
+      const uint32_t e = en;
+      const float2 pj = pn;
+      const PairRec32 rj = rn;
+      en = my[min(k + 1, nb - 1) * kBlock];
+      pn = a.pos32[en & kListMask];
+      rn = a.rec32[en & kListMask];
+      const float xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
+      const float yij = (pi.y - pj.y) + (float)((int)(e >> 30) - 1) * cellf;
+      pair_terms(me, nb_from_rec(rj), xij, yij, xij * xij + yij * yij, S);
+    }
+    nb = 0;
+  };
+  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
+    const uint32_t rb = c->cell_offset + ni * g.ny + lo_j;
+    const uint32_t nc = hi_j - lo_j;  // 1..3 consecutive cells
+    const uint32_t b0 = a.cell_start[rb], b3 = a.cell_start[rb + nc];
+    const uint32_t b1 = nc > 1 ? a.cell_start[rb + 1] : 0xFFFFFFFFu;
+    const uint32_t b2 = nc > 2 ? a.cell_start[rb + 2] : 0xFFFFFFFFu;
+    // about a fifth of a column's candidates lie inside the cutoff
+    if (__any(nb + ((b3 - b0 + 2) >> 2) + kScanBatch > (uint32_t)kNbCap)) flush();
+    const float offx = (float)((int)gi - (int)ni) * cellf;
+    const uint32_t tx = (gi - ni + 1) << 28;
+    for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
+      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      float2 pj[kScanBatch];
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) pj[e] = a.pos32[min(j0 + e, b3 - 1)];
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) {
+        const uint32_t j = j0 + e;
+        if (j < b3 && j != i) {
+          const uint32_t nj = lo_j + (j >= b1 ? 1u : 0u) + (j >= b2 ? 1u : 0u);
+          const float xij = (pi.x - pj[e].x) + offx;
+          const float yij = (pi.y - pj[e].y) + (float)((int)gj - (int)nj) * cellf;
+          const float r2 = xij * xij + yij * yij;
+          if (r2 <= rl2) {
+            const uint32_t word = j | tx | ((gj - nj + 1) << 30);
+            if (deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = word;
+            deg++;
+            if (r2 <= me.cut2) my[nb++ * kBlock] = word;
+          }
+        }
+      }
+    }
+  }
+  flush();
+  if (deg > a.nl_cap) {
+    atomicOr(a.err, EB_NBH);
+    deg = a.nl_cap;
+  }
+  a.nl_deg[i] = deg;
+  const double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
+                        acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
+                        acc_val(S.ax),    acc_val(S.ay),    acc_val(S.xs),    acc_val(S.ys)};
+  finish_list<1>(a, c, g, mt.x, i, 0, v);
+}
+
+void launch_ul_list_first(hipStream_t st, const UlArgs& a) {
+  const dim3 grid(xcd_grid((a.n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_ul_list_first32, grid, dim3(kBlock), 0, st, a);
+}
+
+void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes) {
+  const uint32_t per = kBlock / (uint32_t)lanes;
+  const dim3 grid(xcd_grid((a.n + per - 1) / per));
+  if (lanes == 4)
+    hipLaunchKernelGGL(k_ul_list32<4>, grid, dim3(kBlock), 0, st, a);
+  else if (lanes == 2)
+    hipLaunchKernelGGL(k_ul_list32<2>, grid, dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_ul_list32<1>, grid, dim3(kBlock), 0, st, a);
 }
 
 void launch_ul_pairs32(hipStream_t st, const UlArgs& a) {

@@ -1,0 +1,94 @@
+"""GPU checks of the neighbour-list step of mixed-precision UL contexts (one
+binning and one list per RK4 step, cells of side 2 h_max + skin), through
+the C ABI: against the oracle, which bins every stage as the reference does,
+against the same context with per-stage binning, across lanes per particle,
+in batches, in dense bodies and under graph replay."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _states import perturbed_rings
+from mfree_iwf_amd import PRECISION_MIXED, Solver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(scene, nsteps, **kw):
+    with Solver(precision=PRECISION_MIXED, **kw) as sv:
+        sv.add_body(scene.particles, scene.params)
+        sv.step(nsteps)
+        return sv.state(0)
+
+
+def _errs(a, b):
+    return (O.rel_l2(np.r_[a.x, a.y], np.r_[b.x, b.y]), O.rel_l2(np.r_[a.vx, a.vy], np.r_[b.vx, b.vy]),
+            O.rel_l2(np.r_[a.Sxx, a.Sxy, a.Syy], np.r_[b.Sxx, b.Sxy, b.Syy]), O.rel_l2(a.rho, b.rho))
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_list_steps_vs_oracle(lanes):
+    # every field perturbed, shuffled idx order: the pair set of each stage is
+    # the oracle's per-stage set (r < 2h at that stage's positions)
+    s = perturbed_rings(nbox=61, seed=21, dv=5.0, dS=5e5, shuffle=True)
+    print(f"[list lanes={lanes}] n={s.particles.n}", flush=True)
+    g = _run(s, 30, list_lanes=lanes)
+    rc, o = O.ul_step(s.particles, s.params, 30, order=O.ORDER_STABLE)
+    assert rc == O.OK
+    ep, ev, es, er = _errs(g, o)
+    print(f"[list lanes={lanes}] pos {ep:.2e} vel {ev:.2e} stress {es:.2e} rho {er:.2e}", flush=True)
+    assert ep < 1e-7 and ev < 1e-5 and es < 1e-4 and er < 1e-6
+
+
+def test_list_collision_100_steps_vs_oracle_and_stage_binning():
+    # the ring collision of 4,408 particles through contact (the rings meet
+    # after about 40 steps): list vs per-stage binning vs oracle
+    s = scenes.rings_config1()
+    g = _run(s, 100)
+    b = _run(s, 100, neighbor_list=-1)
+    rc, o = O.ul_step(s.particles, s.params, 100, order=O.ORDER_STABLE)
+    assert rc == O.OK
+    eg, eb = _errs(g, o), _errs(b, o)
+    print(f"[list 100] list {eg} stage binning {eb}", flush=True)
+    assert eg[0] < 1e-5 and eg[2] < 1e-3
+    # the list is as accurate as per-stage binning (same pairs, other order)
+    assert eg[0] < 3 * eb[0] + 1e-12 and eg[2] < 3 * eb[2] + 1e-12
+
+
+def test_list_dense_bodies_vs_oracle():
+    # h = 2.2 dx: about 90 list entries per particle, several LDS batches per stage
+    s = perturbed_rings(nbox=41, seed=22, dv=3.0, dS=3e5, hdx=2.2)
+    g = _run(s, 10)
+    rc, o = O.ul_step(s.particles, s.params, 10, order=O.ORDER_STABLE)
+    assert rc == O.OK
+    ep, ev, es, er = _errs(g, o)
+    print(f"[list dense] pos {ep:.2e} vel {ev:.2e} stress {es:.2e} rho {er:.2e}", flush=True)
+    assert ep < 1e-7 and ev < 1e-5 and es < 1e-4 and er < 1e-6
+
+
+def test_list_batched_bodies_equal_single_bodies():
+    specs = scenes.batch_specs(6, seed=4, nmin=500, nmax=4000)
+    bodies = [scenes.make_rings(sp["nbox"], ri=sp["ri"], ro=sp["ro"], centre=sp["centre"], v0=sp["v0"], E=sp["E"],
+                                jitter=0.5, shuffle=True, seed=k) for k, sp in enumerate(specs)]
+    with Solver(precision=PRECISION_MIXED) as sv:
+        for b in bodies:
+            sv.add_body(b.particles, b.params)
+        sv.step(15)
+        together = [sv.state(k).state8() for k in range(len(bodies))]
+    for k, b in enumerate(bodies):
+        alone = _run(b, 15).state8()
+        assert np.array_equal(alone, together[k]), k
+
+
+def test_list_graph_replay_equals_eager():
+    s = perturbed_rings(nbox=41, seed=23, dv=5.0, dS=5e5, shuffle=True)
+    a = _run(s, 12, use_graph=True).state8()
+    b = _run(s, 12, use_graph=False).state8()
+    assert np.array_equal(a, b)
+
+
+def test_list_option_validation():
+    from mfree_iwf_amd import MfreeError, PRECISION_FP64
+    with pytest.raises(MfreeError):
+        Solver(precision=PRECISION_FP64, neighbor_list=1)
+    with pytest.raises(MfreeError):
+        Solver(precision=PRECISION_MIXED, list_lanes=3)
