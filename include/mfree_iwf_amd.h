@@ -215,7 +215,9 @@ mfree_status mfree_get_pairs(mfree_ctx* ctx, uint32_t body, uint32_t* row_ptr, u
                              uint64_t* npairs);
 
 /* Timing of the dominant (pair) kernel, measured with HIP events on the
- * context's stream while enabled (HIP graphs are not used while enabled). */
+ * context's stream while enabled (HIP graphs are not used while enabled):
+ * the per-stage pair pass, or on the neighbour-list path the list-stage
+ * kernel of stages 2-4. */
 mfree_status mfree_profile_enable(mfree_ctx* ctx, int32_t on);
 mfree_status mfree_profile_read(mfree_ctx* ctx, double* pair_ms_total, uint64_t* pair_launches,
                                 double* step_ms_total, uint64_t* steps);

@@ -195,7 +195,9 @@ mfree_status commit(mfree_ctx* c) {
     c->cell_scale = c->opt.cell_scale >= 1.0 ? c->opt.cell_scale : 1.0;
     if (c->opt.tile_cells > 0) {
       c->tile_cells = (uint32_t)c->opt.tile_cells;
-    } else if (c->opt.tile_cells == 0) {
+    } else if (c->opt.tile_cells == 0 && !c->use_list) {
+      // RK4 steps of a list context do not use the per-stage pass: its shape
+      // (derivative evaluations, learned base steps) stays the untiled one
       mfree_status st = autotune_tiles(c);
       if (st != MFREE_OK) return st;
     }
@@ -495,8 +497,9 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
     p.rec32 = rec[(k - 1) & 1];
     p.pos32_out = k < 4 ? pos[k & 1] : nullptr;
     p.rec32_out = k < 4 ? rec[k & 1] : nullptr;
+    // the profile times the list-stage kernel (stages 2-4), the dominant one
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (evs) {
+    if (evs && k > 1) {
       e0 = take_event(c);
       e1 = take_event(c);
       if (e0) hipEventRecord(e0, st);

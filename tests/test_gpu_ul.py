@@ -273,8 +273,9 @@ def test_mixed_larger_cells(tc):
 
 
 def test_mixed_autotuned_steps_vs_oracle():
+    # per-stage binning (the autotuner chooses that pass's shape)
     s = perturbed_rings(nbox=81, seed=15, dv=2.0, dS=1e5, shuffle=True)
-    with solver_for(s, precision=PRECISION_MIXED) as sv:
+    with solver_for(s, precision=PRECISION_MIXED, neighbor_list=-1) as sv:
         print(f"[autotune] tile_cells={sv.tile_cells}", flush=True)
         sv.step(20)
         g = sv.state(0)
