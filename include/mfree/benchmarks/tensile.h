@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../body.h"
+#include "neighbor_search.h"
 
 template <typename T>
 static void tl_strip_left(T* p) {

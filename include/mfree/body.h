@@ -27,36 +27,16 @@
 #include <vector>
 
 #include "../mfree_iwf_amd.h"
+#include "actions.h"
+#include "boundary.h"
 #include "boundary_conditions.h"
+#include "gauss_integration.h"
 #include "particle.h"
+#include "precomp_shape_functions.h"
 #include "simulation_data.h"
 #include "simulation_time.h"
 
-template <typename T>
-class body;
-
 namespace mfree_adapter {
-
-[[noreturn]] inline void die(const std::string& msg) {
-  fprintf(stderr, "mfree: %s\n", msg.c_str());
-  abort();
-}
-
-[[noreturn]] inline void direct_call(const char* name) {
-  die(std::string(name) + ": actions run inside body<T>::step() on the GPU and cannot be called directly");
-}
-
-// Set-up requests of TL drivers (find_neighbors + precomp_randles_libersky
-// before the body exists, src/benchmarks/tensile.cpp:271-272), keyed by the
-// particle array.
-struct TlRequest {
-  double h = 0.;
-  bool rl = false;
-};
-inline std::map<const void*, TlRequest>& tl_requests() {
-  static std::map<const void*, TlRequest> r;
-  return r;
-}
 
 // Arithmetic of the pair sums for bodies created afterwards.
 inline int32_t& precision() {
@@ -72,50 +52,6 @@ struct Gpu {
 };
 
 }  // namespace mfree_adapter
-
-// ---- actions (src/derivatives.h, src/correctors.h, src/cont_mech.h,
-//      src/material.h, src/riemann_integration.h) ------------------------
-#define MFREE_ACTION(name) \
-  template <typename T>    \
-  void name(body<T>&) {    \
-    mfree_adapter::direct_call(#name); \
-  }
-MFREE_ACTION(riemann_update_integration_weights)
-MFREE_ACTION(material_eos)
-MFREE_ACTION(correctors_mghn_artificial_stress)
-MFREE_ACTION(derive_stress_monaghan)
-MFREE_ACTION(derive_velocity)
-MFREE_ACTION(correctors_mghn_artificial_viscosity)
-MFREE_ACTION(correctors_xsph)
-MFREE_ACTION(material_stress_rate_jaumann)
-MFREE_ACTION(contmech_continuity)
-MFREE_ACTION(contmech_momentum_ul)
-MFREE_ACTION(contmech_advection)
-MFREE_ACTION(derive_displacement)
-MFREE_ACTION(derive_velocity_tl)
-MFREE_ACTION(contmech_velocity_gradient_tl)
-MFREE_ACTION(contmech_cauchy_to_nominal)
-MFREE_ACTION(derive_stress_tl)
-MFREE_ACTION(contmech_momentum_tl)
-#undef MFREE_ACTION
-
-// src/benchmarks/neighbor_search.h:26-28,116-147: recorded, executed on the
-// GPU by the body's first step.
-enum distance_type { distance_manhattan, distance_euclidian };
-
-template <typename T>
-void find_neighbors(T** particles, unsigned int num_part, double h, distance_type type) {
-  (void)num_part;
-  if (type != distance_euclidian) mfree_adapter::die("find_neighbors: only distance_euclidian is supported");
-  mfree_adapter::tl_requests()[(const void*)particles].h = h;
-}
-
-// src/precomp_shape_functions.h:231-290: recorded, executed on the GPU.
-template <typename T>
-void precomp_randles_libersky(T** particles, unsigned int np) {
-  (void)np;
-  mfree_adapter::tl_requests()[(const void*)particles].rl = true;
-}
 
 // src/rk4.h:224-273: the stepper sets the global dt; the integration itself
 // runs on the GPU.
@@ -142,12 +78,15 @@ class body {
   body() {}
   body(T** particles, unsigned int n, simulation_data data, double dt,
        boundary_conditions<T> bc = boundary_conditions<T>(), T** quad_points = 0, unsigned int nq = 0,
-       void* gauss_integration = 0)
-      : m_data(data), m_bc(bc), m_num_part(n), m_particles(particles), m_stepper(n, nq, dt) {
-    if (quad_points || nq || gauss_integration) mfree_adapter::die("weak-form bodies are not part of this library");
-  }
+       gauss_int* gauss_integration = 0)
+      : m_data(data), m_bc(bc), m_num_part(n), m_particles(particles), m_stepper(n, nq, dt),
+        m_weak(quad_points != 0 || nq != 0 || gauss_integration != 0) {}
 
   void add_action(void (*action)(body<T>& b)) { m_actions.push_back(action); }
+
+  // src/body.h:196-212: the outline used by the penalty contact
+  void set_boundary(boundary<T>* b) { m_boundary = b; }
+  boundary<T>* get_boundary() const { return m_boundary; }
 
   // src/body.h:223-225 -> src/rk4.h:71-108
   void step() {
@@ -165,9 +104,32 @@ class body {
   T** get_cur_particles() const { return get_particles(); }
   unsigned int get_num_part() const { return m_num_part; }
   simulation_data get_sim_data() const { return m_data; }
-  bool has_boundary() const { return false; }
-  void update_boundary() {}
+  bool has_boundary() const { return m_boundary != nullptr; }
+  void update_boundary() {
+    if (m_boundary) m_boundary->update(get_particles());
+  }
   mfree_ctx* context() const { return m_gpu ? m_gpu->ctx : nullptr; }
+
+  // The library method the registered actions, particle type and boundary
+  // conditions map to (MFREE_METHOD_*), or -1 when step() would refuse the
+  // body (refusal() says why); needs no GPU.
+  int32_t method() const { return refusal().empty() ? pipeline() : -1; }
+  std::string refusal() const {
+    if (m_weak) return "weak-form and FEM bodies (quadrature points) are not part of this library";
+    const int32_t method = pipeline();
+    if (method < 0) return "the action list is not one of the supported pipelines (MFREE_ERR_PIPELINE)";
+    if (!m_bc.list().empty()) {
+      if (method != MFREE_METHOD_TL_RL) return "boundary conditions on a UL body are not supported";
+      double speed = -1.;
+      for (const auto& e : m_bc.list()) {
+        double a = 0.;
+        if (!probe_strip(std::get<1>(e), &a)) return "boundary function is not a velocity strip";
+        if (speed >= 0. && fabs(a) != speed) return "strips must move with equal and opposite speed";
+        speed = fabs(a);
+      }
+    }
+    return std::string();
+  }
 
  private:
   simulation_data m_data;
@@ -176,6 +138,8 @@ class body {
   unsigned int m_num_part = 0;
   T** m_particles = nullptr;
   rk4<T> m_stepper;
+  bool m_weak = false;
+  boundary<T>* m_boundary = nullptr;
   std::shared_ptr<mfree_adapter::Gpu> m_gpu;
   std::shared_ptr<bool> m_stale = std::make_shared<bool>(false);
 
@@ -231,8 +195,9 @@ class body {
   }
 
   void setup() {
+    const std::string why = refusal();
+    if (!why.empty()) mfree_adapter::die("body::step: " + why);
     const int32_t method = pipeline();
-    if (method < 0) mfree_adapter::die("body::step: action list is not one of the supported pipelines (MFREE_ERR_PIPELINE)");
     for (unsigned int i = 0; i < m_num_part; i++)
       if (m_particles[i]->idx != i) mfree_adapter::die("body: particle i must have idx == i");
     mfree_options opt;
