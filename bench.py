@@ -87,6 +87,25 @@ def errors(a, b):
     )
 
 
+# The oracle's end state of the workload after the accuracy horizon (idx
+# order), when given with --oracle-state: every mode's error is then also
+# reported against it.
+ORACLE_STATE = None
+
+
+def load_oracle_state(path):
+    """x, y, vx, vy, rho, Sxx, Sxy, Syy arrays of an npz file written from the
+    oracle's state (numpy arrays only, no pickles)."""
+    from types import SimpleNamespace
+    with np.load(path, allow_pickle=False) as z:
+        return SimpleNamespace(**{k: np.array(z[k]) for k in ("x", "y", "vx", "vy", "rho", "Sxx", "Sxy", "Syy")})
+
+
+def vs_oracle(state, res, key="error_vs_oracle"):
+    if ORACLE_STATE is not None:
+        res[key] = errors(state, ORACLE_STATE)
+
+
 def accuracy(scene, nsteps, log):
     """Mixed vs FP64 GPU after nsteps RK4 steps from the same state.
     Returns (summary, FP64 end state)."""
@@ -106,6 +125,10 @@ def accuracy(scene, nsteps, log):
     res = dict(steps=nsteps, **errors(a, b),
                max_abs_stress_pa=float(np.abs(np.concatenate([b.Sxx, b.Sxy, b.Syy])).max()),
                energy_ratio=energy)
+    if ORACLE_STATE is not None:
+        res["fp64_vs_oracle"] = errors(b, ORACLE_STATE)
+        res["mixed_vs_oracle"] = errors(a, ORACLE_STATE)
+        log(f"  vs oracle after {nsteps} steps: fp64 {res['fp64_vs_oracle']} mixed {res['mixed_vs_oracle']}")
     return res, b
 
 
@@ -165,11 +188,14 @@ def _ablation(su, scene, args, ref_state, log):
         with Solver(precision=PRECISION_MIXED) as sv:
             sv.add_body(scene.particles, scene.params)
             su.rollout(sv, args.accuracy_steps, group=args.sur_group, threshold=args.sur_threshold)
-            err = errors(sv.state(0), ref_state)
+            st = sv.state(0)
+            err = errors(st, ref_state)
     finally:
         su.set_params(w, norm)
     log(f"  base step alone vs fp64 after {args.accuracy_steps} steps: {err}")
-    return dict(error_vs_fp64=err)
+    res = dict(error_vs_fp64=err)
+    vs_oracle(st, res)
+    return res
 
 
 def _rollout(su, scene, args, ref_state, res, log):
@@ -195,6 +221,7 @@ def _rollout(su, scene, args, ref_state, res, log):
     if ref_state is not None and nsteps == args.accuracy_steps:
         res["error_vs_fp64"] = errors(state, ref_state)
         log(f"  surrogate vs fp64 after {nsteps} steps: {res['error_vs_fp64']}")
+        vs_oracle(state, res)
     return res
 
 
@@ -394,7 +421,13 @@ def main():
     ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
+    ap.add_argument("--oracle-state", default="",
+                    help="npz of the oracle's state of the workload after --accuracy-steps steps: errors of every "
+                         "mode against it are added")
     args = ap.parse_args()
+    if args.oracle_state:
+        global ORACLE_STATE
+        ORACLE_STATE = load_oracle_state(args.oracle_state)
 
     def log(msg):
         if not args.quiet:
