@@ -55,7 +55,8 @@ typedef int32_t mfree_status;
 #define MFREE_ERR_CAPACITY (-9)     /* caller buffer too small */
 #define MFREE_ERR_UNSUPPORTED (-10) /* e.g. non-zero contact forces */
 #define MFREE_ERR_SCRATCH (-11)     /* input larger than an allocation-free entry point's fixed scratch */
-#define MFREE_ERR_SKIN (-12)        /* a particle moved more than half the neighbour-list skin in one step */
+#define MFREE_ERR_SKIN (-12)        /* a particle moved more than half the neighbour-list skin in one step,
+                                       or the fastest particle moves more than h/2 per step */
 
 /* Methods (selected in the reference by the particle type, src/body.h:60-148). */
 #define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
@@ -107,6 +108,13 @@ typedef struct mfree_options {
                                r < 2 h_i of its own positions from the list; off: the reference's
                                per-stage binning (as FP64 contexts always do) */
   int32_t list_lanes;       /* neighbour-list stages: lanes per particle (1, 2 or 4; 0 = default) */
+  int32_t isolate_bodies;   /* UL, fused pipeline: 1 = a body that fails (>= max_nbh neighbours, a
+                               non-finite derivative, no neighbour, its cell grid or skin) is frozen
+                               at the state it started the failing step with, its time stops, and
+                               the other bodies keep stepping: mfree_step returns MFREE_OK and
+                               mfree_body_status reports the body's error; mfree_set_state on the
+                               body thaws it.  0 (default): the first failure fails the step, as
+                               the reference's assertions stop its run */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
@@ -189,6 +197,11 @@ mfree_status mfree_energy(mfree_ctx* ctx, uint32_t body, double* energy /* 3 */)
 mfree_status mfree_energy_noalloc(int32_t device, void* stream, const mfree_params* p, const mfree_particles* dev,
                                   uint32_t n, double* energy_dev);
 mfree_status mfree_get_time(mfree_ctx* ctx, uint32_t body, double* time);
+/* The status of a body of a context with isolate_bodies: MFREE_OK, or the
+ * error (MFREE_ERR_NBH_OVERFLOW, ...) that froze it.  No counterpart in the
+ * reference, which stops the run at the first failure; for batches of
+ * independent bodies.  Always MFREE_OK without isolate_bodies. */
+mfree_status mfree_body_status(mfree_ctx* ctx, uint32_t body, mfree_status* status);
 
 /* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for
  * every body: nsteps RK4 steps.  Each body advances its own time by its dt
@@ -344,19 +357,38 @@ mfree_status mfree_surrogate_record_velocity(mfree_surrogate* s, mfree_ctx* ctx)
  * moves ctx's storage to its other buffer. */
 mfree_status mfree_surrogate_step(mfree_surrogate* s, mfree_ctx* ctx, int32_t use_net);
 
+/* Rollout accounting: learned_steps counts the learned steps whose result
+ * was kept (4 dt each), plain_steps the plain RK4 steps, so
+ * 4 learned_steps + plain_steps = nsteps.  max_deviation / last_deviation:
+ * the judge's position deviation (below), largest over bodies;
+ * max_ratio_*: the largest network-to-base error ratios of velocities,
+ * density and stresses; worst_body: the body that decided the last
+ * rejection; numeric_fallbacks: groups rolled back because their learned
+ * steps failed numerically (non-finite values, neighbour overflow). */
 typedef struct mfree_judge_stats {
   uint32_t groups, fallbacks, learned_steps, plain_steps;
   double max_deviation, last_deviation;
   double ms_learned, ms_judge, ms_fallback;
+  double max_ratio_vel, max_ratio_rho, max_ratio_stress;
+  uint32_t worst_body, numeric_fallbacks;
 } mfree_judge_stats;
 
 /* nsteps time steps (dt units) of ctx with the learned stepper: 4 plain
  * warm-up steps, then groups of `group` learned steps; the last one of each
  * group is judged against 4 plain RK4 steps from the same state (the plain
- * result is kept).  If the relative L2 deviation of all positions exceeds
- * `threshold` the group is rolled back and redone with plain steps.  The
- * last group may be shorter (at least 2 learned steps); fewer than 8
- * remaining steps run plain. */
+ * result is kept).  The judge works per body on the increments over those
+ * 4 dt from the judged state: the position deviation is the relative L2
+ * error of the learned displacement against the plain one; for velocities,
+ * density and stresses it is the ratio of the learned step's error against
+ * the plain steps to that of the same step's base step without the network
+ * (the latter floored at 1e-5 c0, rho0 and K per particle, where the base
+ * step is nearly exact).
+ * The group is rolled back and redone with plain steps if, for any body, the
+ * position deviation exceeds `threshold` or a ratio exceeds 2 (the network
+ * made the step twice as wrong as no network), or if a learned step fails
+ * numerically; threshold = +inf accepts every group whose learned steps
+ * complete.  The last group may be shorter (at least 2 learned steps);
+ * fewer than 8 remaining steps run plain. */
 mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* ctx, uint32_t nsteps, uint32_t group,
                                      double threshold, mfree_judge_stats* stats);
 

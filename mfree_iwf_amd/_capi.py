@@ -47,7 +47,7 @@ class Options(C.Structure):
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
                 ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32),
                 ("xcd_chunk", C.c_uint32), ("cell_scale", C.c_double), ("neighbor_list", C.c_int32),
-                ("list_lanes", C.c_int32)]
+                ("list_lanes", C.c_int32), ("isolate_bodies", C.c_int32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")
@@ -77,7 +77,9 @@ class SurrogateOptions(C.Structure):
 class JudgeStats(C.Structure):
     _fields_ = [("groups", C.c_uint32), ("fallbacks", C.c_uint32), ("learned_steps", C.c_uint32),
                 ("plain_steps", C.c_uint32), ("max_deviation", C.c_double), ("last_deviation", C.c_double),
-                ("ms_learned", C.c_double), ("ms_judge", C.c_double), ("ms_fallback", C.c_double)]
+                ("ms_learned", C.c_double), ("ms_judge", C.c_double), ("ms_fallback", C.c_double),
+                ("max_ratio_vel", C.c_double), ("max_ratio_rho", C.c_double), ("max_ratio_stress", C.c_double),
+                ("worst_body", C.c_uint32), ("numeric_fallbacks", C.c_uint32)]
 
 
 SIGNATURES = {
@@ -97,6 +99,7 @@ SIGNATURES = {
     "mfree_set_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_time": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
+    "mfree_body_status": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32)]),
     "mfree_energy": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
     "mfree_energy_noalloc": (C.c_int32, [C.c_int32, C.c_void_p, C.POINTER(Params), C.POINTER(Particles),
                                          C.c_uint32, C.c_void_p]),

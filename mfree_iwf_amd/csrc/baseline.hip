@@ -491,7 +491,7 @@ void ref_enqueue_step(mfree_ctx* c) {
   ref_derivs(c, r->k, r->d[3]);
   hipLaunchKernelGGL(k_ref_rk_final, dim3(g), dim3(kBlock), 0, s, c->N, c->scene[0], c->sc, y, r->d[0], r->d[1],
                      r->d[2], r->d[3]);
-  hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, s, c->S, c->sc);
+  hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, s, c->S, c->sc, nullptr);
 }
 
 void ref_enqueue_eval(mfree_ctx* c, double* d8) {

@@ -66,22 +66,56 @@ void scene_consts(const mfree_params& p, uint32_t offset, uint32_t n, SceneConst
   s.pow4 = (p.corr_exp == 4.0) ? 1u : 0u;
 }
 
-mfree_status check_err_bits(mfree_ctx* c) {
-  uint32_t bits = 0;
-  HIPCHK(c, hipMemcpyAsync(&bits, c->errb, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (!bits) return MFREE_OK;
-  uint32_t zero = 0;
-  HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
-  // a non-finite derivative is reported first: it makes later positions
-  // leave the grid as well
-  if (bits & EB_NONFINITE) return fail(c, MFREE_ERR_NONFINITE, "non-finite derivative");
-  if (bits & EB_NBH) return fail(c, MFREE_ERR_NBH_OVERFLOW, "a particle has >= max_nbh neighbours");
-  if (bits & EB_NONEIGH) return fail(c, MFREE_ERR_NO_NEIGHBOR, "a particle has no neighbour");
-  if (bits & EB_CELLS) return fail(c, MFREE_ERR_CELLS, "cell grid: num_cell >= n or a particle outside the grid");
+// Status of a set of error bits: a non-finite derivative is reported first
+// (it makes later positions leave the grid as well).
+static mfree_status bits_status(uint32_t bits, const char** msg) {
+  if (bits & EB_NONFINITE) return *msg = "non-finite derivative", MFREE_ERR_NONFINITE;
+  if (bits & EB_NBH) return *msg = "a particle has >= max_nbh neighbours", MFREE_ERR_NBH_OVERFLOW;
+  if (bits & EB_NONEIGH) return *msg = "a particle has no neighbour", MFREE_ERR_NO_NEIGHBOR;
+  if (bits & EB_CELLS) return *msg = "cell grid: num_cell >= n or a particle outside the grid", MFREE_ERR_CELLS;
   if (bits & EB_SKIN)
-    return fail(c, MFREE_ERR_SKIN, "a particle moved more than half the neighbour-list skin within one step");
-  return fail(c, MFREE_ERR_HIP, "unknown device error bits");
+    return *msg = "a particle moved more than half the neighbour-list skin within one step, or faster than h/2 "
+                  "per step",
+           MFREE_ERR_SKIN;
+  return *msg = "unknown device error bits", MFREE_ERR_HIP;
+}
+
+mfree_status check_err_bits(mfree_ctx* c) {
+  // word 0: the context's bits; words 1..S: each body's (UL kernels)
+  const size_t nw = 1 + (size_t)c->S;
+  std::vector<uint32_t> w(nw, 0u);
+  HIPCHK(c, hipMemcpyAsync(w.data(), c->errb, nw * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->opt.isolate_bodies) {
+    // failed bodies stay frozen at their last good state and keep their
+    // (sticky) words; the step itself succeeds unless bits were raised
+    // outside any body
+    uint32_t in_bodies = 0;
+    for (uint32_t s = 0; s < c->S; s++) {
+      c->body_bits[s] |= w[1 + s];
+      in_bodies |= w[1 + s];
+    }
+    if (!w[0]) return MFREE_OK;
+    const uint32_t zero = 0;
+    HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    const uint32_t rest = w[0] & ~in_bodies;
+    if (!rest) return MFREE_OK;
+    const char* msg = nullptr;
+    const mfree_status st = bits_status(rest, &msg);
+    return fail(c, st, msg);
+  }
+  if (!w[0]) return MFREE_OK;
+  const std::vector<uint32_t> zero(nw, 0u);
+  HIPCHK(c, hipMemcpy(c->errb, zero.data(), nw * sizeof(uint32_t), hipMemcpyHostToDevice));
+  const char* msg = nullptr;
+  const mfree_status st = bits_status(w[0], &msg);
+  // name the first body that raised the reported condition
+  for (uint32_t s = 0; s < c->S; s++) {
+    const char* m2 = nullptr;
+    if (w[1 + s] && bits_status(w[1 + s], &m2) == st)
+      return fail(c, st, "body " + std::to_string(s) + ": " + msg);
+  }
+  return fail(c, st, msg);
 }
 
 mfree_status autotune_tiles(mfree_ctx* c);
@@ -112,8 +146,12 @@ mfree_status commit(mfree_ctx* c) {
   }
   HIPCHK(c, alloc8(c, c->K, N));
   HIPCHK(c, alloc8(c, c->acc, N));
-  HIPCHK(c, dalloc(c, &c->errb, 1));
-  HIPCHK(c, hipMemset(c->errb, 0, sizeof(uint32_t)));
+  // error words: [0] the context's, [1 + s] body s's (UL kernels raise both)
+  HIPCHK(c, dalloc(c, &c->errb, 1 + (size_t)c->S));
+  HIPCHK(c, hipMemset(c->errb, 0, (1 + (size_t)c->S) * sizeof(uint32_t)));
+  c->berr = c->errb + 1;
+  c->body_bits.assign(c->S, 0u);
+  if (c->opt.isolate_bodies) HIPCHK(c, dalloc(c, &c->perm_keep, N));
   // bodies
   c->h_sc.resize(c->S);
   std::vector<Chunk> ch;
@@ -233,6 +271,7 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.cell_start = c->cell_start;
   a.perm = c->perm;
   a.err = c->errb;
+  a.berr = c->opt.method == MFREE_METHOD_TL_RL ? nullptr : c->berr;
   a.tile_cells = c->tile_cells;
   a.tile_off = c->tile_cells ? c->tile_off : nullptr;
   return a;
@@ -262,6 +301,7 @@ UlArgs ul_args(mfree_ctx* c, int b) {
   u.acc = c->acc;
   u.max_nbh = u.mixed ? 0 : c->opt.max_nbh;
   u.err = c->errb;
+  u.berr = c->berr;
   u.nscene = c->S;
   u.tile_cells = c->tile_cells;
   u.tile_off = c->tile_cells ? c->tile_off : nullptr;
@@ -405,9 +445,26 @@ mfree_status autotune_tiles(mfree_ctx* c) {
     std::lock_guard<std::mutex> lk(cache_mu);
     cache.push_back({key, best_tc, best_scale});
   }
-  uint32_t zero = 0;
-  HIPCHK(c, hipMemcpy(c->errb, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+  // the timing passes are no steps: no body has failed
+  const std::vector<uint32_t> zero(1 + (size_t)c->S, 0u);
+  HIPCHK(c, hipMemcpy(c->errb, zero.data(), zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   return MFREE_OK;
+}
+
+// End of a UL step from storage a to b: in a context that isolates its
+// bodies, a body whose error word is set keeps the state it started the step
+// with (and stays there: the word is sticky); then the time of every body
+// that did not fail advances by its dt.
+static void end_of_step(mfree_ctx* c, int a, int b, bool advance_time) {
+  hipStream_t st = c->stream;
+  const bool iso = c->opt.isolate_bodies != 0;
+  if (iso)
+    hipLaunchKernelGGL(k_restore_failed, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, c->N,
+                       c->scene[b], c->berr, c->perm_keep, c->y[a], c->y[b]);
+  if (advance_time)
+    hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc,
+                       iso ? c->berr : nullptr);
+  c->cur = b;
 }
 
 // Enqueues one RK step of every UL body (src/rk4.h:71-108 with the UL
@@ -428,6 +485,7 @@ void enqueue_ul_rk(mfree_ctx* c, const RkPlan& plan, std::vector<std::pair<hipEv
   u.h_dst = c->h[b];
   u.scene_dst = c->scene[b];
   u.gid_dst = c->gid[b];
+  u.perm_keep = c->perm_keep;
   launch_ul_prepass(st, u);
   UlArgs p = ul_args(c, b);
   p.sc = sc;
@@ -453,8 +511,7 @@ void enqueue_ul_rk(mfree_ctx* c, const RkPlan& plan, std::vector<std::pair<hipEv
     if (stages[k] == 3) q.k3s = plan.k3s;
     pair_launch(c, q, evs);
   }
-  if (plan.advance_time) hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);
-  c->cur = b;
+  end_of_step(c, a, b, plan.advance_time);
 }
 
 // One RK4 step of every UL body on the neighbour-list path (mixed
@@ -482,6 +539,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
   u.h_dst = c->h[b];
   u.scene_dst = c->scene[b];
   u.gid_dst = c->gid[b];
+  u.perm_keep = c->perm_keep;
   launch_ul_prepass(st, u);
   UlArgs p = ul_args(c, b);
   p.home_identity = 1;
@@ -513,8 +571,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
       evs->push_back({e0, e1});
     }
   }
-  hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, st, c->S, c->sc);
-  c->cur = b;
+  end_of_step(c, a, b, true);
 }
 
 void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {
@@ -545,7 +602,7 @@ mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {
     }
     if (tl) {
       tl_enqueue_step(c->tl, tl_shared(c), c->prof ? &evs : nullptr, [c]() { return take_event(c); });
-      hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc);
+      hipLaunchKernelGGL(k_advance_time, dim3((c->S + 63) / 64), dim3(64), 0, c->stream, c->S, c->sc, nullptr);
       continue;
     }
     if (c->opt.use_graph && !c->prof) {
@@ -580,9 +637,25 @@ mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {
       c->pair_n++;
     }
   }
-  for (uint32_t s = 0; s < c->S; s++)
-    for (uint32_t k = 0; k < nsteps; k++) c->time[s] = c->time[s] + c->prm[s].dt;
-  return check_err_bits(c);
+  if (!c->opt.isolate_bodies) {
+    for (uint32_t s = 0; s < c->S; s++)
+      for (uint32_t k = 0; k < nsteps; k++) c->time[s] = c->time[s] + c->prm[s].dt;
+    return check_err_bits(c);
+  }
+  // isolated bodies: a failed body's time stopped at its last good step, on
+  // the device (the same additions of dt as above for the others)
+  mfree_status st = check_err_bits(c);
+  if (st != MFREE_OK) return st;
+  std::vector<SceneConst> hsc(c->S);
+  HIPCHK(c, hipMemcpy(hsc.data(), c->sc, c->S * sizeof(SceneConst), hipMemcpyDeviceToHost));
+  for (uint32_t s = 0; s < c->S; s++) c->time[s] = hsc[s].time;
+  return MFREE_OK;
+}
+
+mfree_status body_status(const mfree_ctx* c, uint32_t body) {
+  if (!c->committed || body >= c->body_bits.size() || !c->body_bits[body]) return MFREE_OK;
+  const char* msg = nullptr;
+  return bits_status(c->body_bits[body], &msg);
 }
 
 bool valid_particles(const mfree_particles* p, bool tl) {
@@ -676,6 +749,7 @@ void mfree_default_options(mfree_options* o) {
   o->cell_scale = 0.;
   o->neighbor_list = 0;
   o->list_lanes = 0;
+  o->isolate_bodies = 0;
 }
 
 mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
@@ -712,6 +786,15 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
   if (c->opt.pipeline == MFREE_PIPELINE_REFERENCE &&
       (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.precision != MFREE_PRECISION_FP64 ||
        c->opt.max_nbh > 200)) {
+    delete c;
+    return MFREE_ERR_UNSUPPORTED;
+  }
+  if (c->opt.isolate_bodies != 0 && c->opt.isolate_bodies != 1) {
+    delete c;
+    return MFREE_ERR_ARG;
+  }
+  if (c->opt.isolate_bodies &&
+      (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
     delete c;
     return MFREE_ERR_UNSUPPORTED;
   }
@@ -808,6 +891,13 @@ mfree_status mfree_set_state(mfree_ctx* c, uint32_t body, const mfree_particles*
   if (st) return st;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const uint32_t o = c->off[body], n = c->off[body + 1] - o;
+  c->state_version++;
+  if (c->opt.isolate_bodies && c->body_bits[body]) {
+    // a new state thaws a failed body
+    const uint32_t zero = 0;
+    HIPCHK(c, hipMemcpy(c->berr + body, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->body_bits[body] = 0;
+  }
   // the body's storage range is [o, o+n) in any order; rewrite it in idx order
   const double* src[8] = {host->x, host->y, host->vx, host->vy, host->rho, host->Sxx, host->Sxy, host->Syy};
   const int b = c->cur;
@@ -861,6 +951,12 @@ mfree_status mfree_get_state(mfree_ctx* c, uint32_t body, mfree_particles* host)
 mfree_status mfree_get_time(mfree_ctx* c, uint32_t body, double* t) {
   if (!c || body >= c->prm.size() || !t) return MFREE_ERR_ARG;
   *t = c->time[body];
+  return MFREE_OK;
+}
+
+mfree_status mfree_body_status(mfree_ctx* c, uint32_t body, mfree_status* status) {
+  if (!c || body >= c->prm.size() || !status) return MFREE_ERR_ARG;
+  *status = body_status(c, body);
   return MFREE_OK;
 }
 

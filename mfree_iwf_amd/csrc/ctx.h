@@ -48,6 +48,9 @@ struct mfree_ctx {
   int cur = 0;
   uint32_t *key = nullptr, *rank = nullptr, *count = nullptr, *cell_start = nullptr, *perm = nullptr;
   uint32_t *tsum = nullptr, *errb = nullptr;
+  uint32_t* berr = nullptr;       // errb + 1: the bodies' error words
+  uint32_t* perm_keep = nullptr;  // isolated bodies: the step's stage-1 permutation
+  std::vector<uint32_t> body_bits;  // isolated bodies: error bits of each body (sticky)
   uint2* meta = nullptr;
   double2* pos64 = nullptr;
   PairRec64* rec64 = nullptr;
@@ -79,6 +82,7 @@ struct mfree_ctx {
   // process-unique id: a learned stepper bound to a context compares ids, not
   // addresses (a new context may reuse a destroyed one's address)
   uint64_t serial = next_serial();
+  uint64_t state_version = 0;  // advanced by mfree_set_state (h may change)
   static uint64_t next_serial() {
     static std::atomic<uint64_t> n{0};
     return ++n;
@@ -118,6 +122,8 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b);
 UlArgs ul_args(mfree_ctx* c, int b);
 // nsteps RK4 steps of every body (graphs when enabled).
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps);
+// Status of a body of a context that isolates its bodies (MFREE_OK: not failed).
+mfree_status body_status(const mfree_ctx* c, uint32_t body);
 
 // One explicit RK step of every UL body, enqueued on the context's stream
 // (kernels only, capturable).  Storage moves to the other buffer.

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
                                                        const double* __restrict__ part, double kernel_width,
                                                        double cell_min, uint32_t tile_cells,
                                                        SceneGeom* __restrict__ geom, uint32_t* __restrict__ err,
+                                                       uint32_t* __restrict__ berr,
                                                        const double* __restrict__ vpart, double skin_min,
                                                        double skin_vfac) {
   const uint32_t s = blockIdx.x;
@@ -152,6 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
   const double lx = mxx - mnx;
   const double ly = mxy - mny;
   const double fnx = ceil(lx / dx), fny = ceil(ly / dx);
+  const bool skin_ok = !(skin > kSkinMax * mxh);
   SceneGeom g;
   g.bbmin_x = mnx;
   g.bbmin_y = mny;
@@ -160,12 +162,13 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
   // num_cell < n is asserted by the reference (src/grid.h:114); the same
   // bound keeps every key inside the body's cell range here.
   const double ncell = fnx * fny;
-  if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n)) {
+  if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n) || !skin_ok) {
+    const uint32_t bit = skin_ok ? EB_CELLS : EB_SKIN;
     g.nx = 1;
     g.ny = 1;
     g.num_cell = 1;
-    g.err = EB_CELLS;
-    atomicOr(err, EB_CELLS);
+    g.err = bit;
+    raise_err(err, berr, s, bit);
   } else {
     g.nx = (unsigned int)fnx;
     g.ny = (unsigned int)fny;
@@ -187,7 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_hash_count(uint32_t n, const uint32_
                                                        const SceneConst* __restrict__ sc,
                                                        const SceneGeom* __restrict__ geom,
                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
-                                                       uint32_t* __restrict__ count, uint32_t* __restrict__ err) {
+                                                       uint32_t* __restrict__ count, uint32_t* __restrict__ err,
+                                                       uint32_t* __restrict__ berr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   uint32_t k = 0xFFFFFFFFu;
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_hash_count(uint32_t n, const uint32_
       ix = (unsigned int)fx;
       iy = (unsigned int)fy;
     } else if (!g.err) {
-      atomicOr(err, EB_CELLS);
+      raise_err(err, berr, s, EB_CELLS);
     }
     k = sc[s].cell_offset + ix * g.ny + iy;
   }
@@ -328,14 +332,20 @@ __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* 
 }
 
 // One thread per cell: insertion sort of the cell's entries by global id,
-// so that within a cell particles appear in idx order.
+// so that within a cell particles appear in idx order.  Entries arrive
+// nearly sorted (storage is in the previous step's cell order), so a cell
+// costs about its size; a body whose grid fell back to one cell (its step
+// fails) is left unsorted, since sorting that cell is quadratic in the body.
 __global__ __launch_bounds__(kBlock) void k_cell_sort(uint32_t ncell, const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ gid,
+                                                      const uint32_t* __restrict__ scene,
+                                                      const SceneGeom* __restrict__ geom,
                                                       uint32_t* __restrict__ perm) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   uint32_t a = cell_start[c], b = cell_start[c + 1];
   if (b - a < 2) return;
+  if (geom[scene[perm[a]]].err) return;
   for (uint32_t i = a + 1; i < b; i++) {
     uint32_t p = perm[i];
     uint32_t g = gid[p];
@@ -390,13 +400,13 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_bbox_partial, dim3(a.nchunks), dim3(kBlock), 0, st, a.chunks, a.x, a.y, a.h, a.part,
                      skin ? a.vx : nullptr, skin ? a.vy : nullptr, skin ? a.vpart : nullptr);
   hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part,
-                     a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err, skin ? a.vpart : nullptr,
+                     a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err, a.berr, skin ? a.vpart : nullptr,
                      a.skin_min, a.skin_vfac);
   if (a.tile_cells && a.tile_off)
     hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kBlock), 0, st, a.nscene, a.geom, a.tile_off);
   // a.count is zero here: zeroed at allocation and cleared again by the scan
   hipLaunchKernelGGL(k_hash_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.scene, a.x, a.y, a.sc,
-                     a.geom, a.key, a.rank, a.count, a.err);
+                     a.geom, a.key, a.rank, a.count, a.err, a.berr);
   uint32_t nt = nblk(a.ncell_cap, kScanTile);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum, a.cell_start,
@@ -405,7 +415,7 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
                      a.cell_start, a.perm);
   if (a.deterministic)
     hipLaunchKernelGGL(k_cell_sort, dim3(nblk(a.ncell_cap, kBlock)), dim3(kBlock), 0, st, a.ncell_cap,
-                       a.cell_start, a.gid, a.perm);
+                       a.cell_start, a.gid, a.scene, a.geom, a.perm);
 }
 
 }  // namespace mfree

@@ -34,6 +34,10 @@ constexpr uint32_t kXcdChunkDefault = 64;
 constexpr uint32_t kListCap = 128;
 constexpr double kSkinMin = 0.25;
 constexpr double kSkinVfac = 4.0;
+// a skin above kSkinMax h_max (a particle moving h/2 in one step) is no
+// stable step: the body's binning fails (EB_SKIN) instead of degenerating
+// into one cell
+constexpr double kSkinMax = 2.0;
 // The same with the tiles dealt to the XCDs in chunks of `chunk`
 // consecutive tiles (XCD x takes chunks x, x + 8, ...; the remainder is
 // split evenly), so that a costly stretch of tiles is shared by several
@@ -54,8 +58,17 @@ enum : uint32_t {
   EB_NBH = 2u,        // >= max_nbh neighbours under the reference predicate
   EB_NONFINITE = 4u,  // NaN / Inf in a derivative
   EB_NONEIGH = 8u,    // particle without neighbour
-  EB_SKIN = 16u,      // a particle moved more than half the neighbour-list skin within a step
+  EB_SKIN = 16u,      // a particle moved more than half the neighbour-list skin within a step,
+                      // or the skin exceeds kSkinMax h_max
 };
+
+// Raises error bits: in the context word err[0] and, when berr is set, in
+// the word of the body (berr[body]; per-body words of a context that
+// isolates its bodies)
+__device__ inline void raise_err(uint32_t* err, uint32_t* berr, uint32_t body, uint32_t bits) {
+  atomicOr(err, bits);
+  if (berr) atomicOr(berr + body, bits);
+}
 
 // Per-body constants, device resident.  Derived constants are computed on
 // the host with the reference's formulas (src/simulation_data.cpp:32-47).

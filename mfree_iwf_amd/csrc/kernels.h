@@ -26,6 +26,7 @@ struct BinArgs {
   double* part;                 // 5 per chunk
   uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
   uint32_t* err;
+  uint32_t* berr = nullptr;     // per-body error words (null: context word only)
   uint32_t tile_cells;          // > 0: also enumerate pair tiles of this many cells
   uint32_t* tile_off;           // nscene+1: first tile of each body, [nscene] = total
   // neighbour-list binning (once per step): cells of side 2 h_max + skin with
@@ -82,6 +83,10 @@ struct UlArgs {
   double* aux;                  // stage 0: diagnostics by global id (may be null)
   uint32_t max_nbh;
   uint32_t* err;
+  uint32_t* berr;               // per-body error words (null: context word only)
+  // isolated bodies: stage 1 keeps its permutation (sorted position -> storage
+  // index of y_n) so a failed body can be restored at the end of the step
+  uint32_t* perm_keep;
   // learned-step graph emitted by the mixed per-particle pass (stage 1 of
   // the learned step's base step, bodies of uniform h): edges r < 2h over
   // sorted positions
@@ -237,7 +242,9 @@ struct PairListArgs {
 };
 void launch_pair_list(hipStream_t st, const PairListArgs& a);
 
-__global__ void k_advance_time(uint32_t nscene, SceneConst* sc);
+__global__ void k_advance_time(uint32_t nscene, SceneConst* sc, const uint32_t* berr);
+__global__ void k_restore_failed(uint32_t n, const uint32_t* scene_new, const uint32_t* berr, const uint32_t* perm1,
+                                 Fields8 y_old, Fields8 y_new);
 __global__ void k_advance_time_n(uint32_t nscene, SceneConst* sc, uint32_t nsteps);
 
 void launch_calib_copy(hipStream_t st, const double* a, double* b, uint64_t n);

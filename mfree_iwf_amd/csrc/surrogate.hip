@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_graph(GraphArgs a) {
   const double xi = pi.x, yi = pi.y, hi = pi.z;
   uint32_t d = 0;
   float ws = kSplineNorm;  // the node itself (q = 0)
-  const int cx0 = max(ix - 1, 0), cx1 = min(ix + 1, (int)g.nx - 1);
+  // a body whose grid fell back to one cell fails the step: scan no cells
+  const int cx0 = max(ix - 1, 0), cx1 = g.err ? cx0 - 1 : min(ix + 1, (int)g.nx - 1);
   const int cy0 = max(iy - 1, 0), cy1 = min(iy + 1, (int)g.ny - 1);
   for (int cx = cx0; cx <= cx1; cx++) {
     const uint32_t c0 = co + (uint32_t)cx * g.ny;
@@ -582,47 +583,63 @@ __global__ void k_iota(uint32_t n, uint32_t* __restrict__ out) {
   if (i < n) out[i] = i;
 }
 
-// Judge: positions by global id of the learned candidate.
-__global__ void k_pos_by_gid(uint32_t n, const double* __restrict__ x, const double* __restrict__ y,
-                             const uint32_t* __restrict__ gid, double* __restrict__ out) {
+// Judge: the 8 state fields of storage buffer y by global id (out: 8 x N,
+// field-major).
+__global__ void k_state_by_gid(uint32_t n, Fields8 y, const uint32_t* __restrict__ gid, double* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t g = gid[i];
-  out[2 * (size_t)g] = x[i];
-  out[2 * (size_t)g + 1] = y[i];
+  const size_t g = gid[i];
+#pragma unroll
+  for (int k = 0; k < 8; k++) out[k * (size_t)n + g] = y.f[k][i];
 }
 
-// Judge: partial sums of |x - x_cand|^2 and |x|^2 (FP64).
-constexpr uint32_t kDevBlocks = 512;
-__global__ __launch_bounds__(256) void k_deviation(uint32_t n, const double* __restrict__ x,
-                                                   const double* __restrict__ y, const uint32_t* __restrict__ gid,
-                                                   const double* __restrict__ cand, double* __restrict__ part) {
-  double a = 0.0, b = 0.0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t g = gid[i];
-    const double ex = x[i] - cand[2 * (size_t)g], ey = y[i] - cand[2 * (size_t)g + 1];
-    a += ex * ex + ey * ey;
-    b += x[i] * x[i] + y[i] * y[i];
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_xor(a, o, 64);
-    b += __shfl_xor(b, o, 64);
-  }
-  __shared__ double sh[2][256 / 64];
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][threadIdx.x >> 6] = a;
-    sh[1][threadIdx.x >> 6] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double ta = 0.0, tb = 0.0;
-    for (int k = 0; k < 256 / 64; k++) {
-      ta += sh[0][k];
-      tb += sh[1][k];
+// Judge: per chunk of one body (Chunk ranges of the storage buffer y),
+// partial sums for the field groups positions (x, y), velocities, density
+// and stresses of: the squared difference of the learned and the plain
+// increments, the same for the base step's increment (base, 8 per gid), the
+// squared plain increment, and the squared judged state.  Increments are
+// taken from the judged state s0 (8 x N by gid); cand is the learned step's
+// end state (8 x N by gid); y is the plain result.
+constexpr int kJudgeGroups = 4;
+constexpr int kJudgeSums = 4;
+// A judged step whose network correction leaves the velocities, density or
+// stresses of a body further from the plain steps than this many times the
+// base step alone is rejected.
+constexpr double kNetRatioLimit = 2.0;
+constexpr double kNetFloor = 1e-5;
+__device__ inline int judge_group(int k) { return k < 2 ? 0 : (k < 4 ? 1 : (k == 4 ? 2 : 3)); }
+__global__ __launch_bounds__(256) void k_judge_partial(const Chunk* __restrict__ ch, uint32_t n, Fields8 y,
+                                                       const uint32_t* __restrict__ gid, const double* __restrict__ s0,
+                                                       const double* __restrict__ cand,
+                                                       const double* __restrict__ base, double* __restrict__ part) {
+  constexpr int kS = kJudgeSums * kJudgeGroups;
+  const Chunk c = ch[blockIdx.x];
+  double acc[kS];
+#pragma unroll
+  for (int k = 0; k < kS; k++) acc[k] = 0.0;
+  for (uint32_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+    const size_t g = gid[i];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double a = s0[k * (size_t)n + g];
+      const double dp = y.f[k][i] - a, dl = cand[k * (size_t)n + g] - a, db = base[8 * g + k] - a;
+      const int q = judge_group(k);
+      acc[kJudgeSums * q] += (dl - dp) * (dl - dp);
+      acc[kJudgeSums * q + 1] += (db - dp) * (db - dp);
+      acc[kJudgeSums * q + 2] += dp * dp;
+      acc[kJudgeSums * q + 3] += a * a;
     }
-    part[2 * blockIdx.x] = ta;
-    part[2 * blockIdx.x + 1] = tb;
   }
+  __shared__ double red[kS][256];
+#pragma unroll
+  for (int k = 0; k < kS; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < kS; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < kS) part[blockIdx.x * kS + threadIdx.x] = red[threadIdx.x][0];
 }
 
 }  // namespace
@@ -676,9 +693,12 @@ struct mfree_surrogate {
   // per-context scratch
   uint64_t bound = 0;  // serial of the bound context (0: none)
   uint32_t bN = 0;
+  uint64_t bver = 0;   // state version of the bound context when uniform_h was determined
   bool uniform_h = false;  // every body of the bound context has one smoothing length
   std::vector<void*> callocs;
-  double *base = nullptr, *vprev = nullptr, *cand = nullptr, *devp = nullptr;
+  double *base = nullptr, *vprev = nullptr;
+  double *jstart = nullptr, *cand = nullptr, *devp = nullptr;  // judge: judged state, learned end (8 x N by gid),
+                                                               // per-chunk sums
   SceneConst* sc4 = nullptr;  // constants of the base step (dt scaled by kStepsPerLearned)
   Fields8 k1s{}, k2s{}, k3s{};  // stage rates of the base step (storage order; fields 2..7)
   double *cpart = nullptr, *cmean = nullptr;  // correction sums per chunk (5), means per body (4)
@@ -731,22 +751,48 @@ void drop_graphs(mfree_surrogate* s) {
     }
 }
 
+// Whether every body of the context has one smoothing length (h changes only
+// through mfree_set_state, which advances the context's state version).
+mfree_status check_uniform_h(mfree_surrogate* s, mfree_ctx* c) {
+  const size_t N = c->N;
+  std::vector<double> hh(N);
+  SCHK(s, hipMemcpy(hh.data(), c->h[c->cur], N * sizeof(double), hipMemcpyDeviceToHost));
+  bool u = true;
+  for (size_t b = 0; b < c->S && u; b++)
+    for (uint32_t k = c->off[b] + 1; k < c->off[b + 1] && u; k++) u = hh[k] == hh[c->off[b]];
+  s->uniform_h = u;
+  return MFREE_OK;
+}
+
 // Per-context buffers (base update, previous velocities, snapshots); reallocated when the surrogate meets another context.
 mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   mfree_status st = commit(c);
   if (st != MFREE_OK) return sfail(s, st, std::string("context: ") + c->err);
   if (c->opt.method != MFREE_METHOD_UL_MONAGHAN)
     return sfail(s, MFREE_ERR_UNSUPPORTED, "the learned stepper drives UL (Gray-Monaghan) contexts");
+  if (c->opt.isolate_bodies)
+    return sfail(s, MFREE_ERR_UNSUPPORTED, "the learned stepper's rollback covers the whole context: no isolated bodies");
   if (c->opt.device != s->opt.device) return sfail(s, MFREE_ERR_ARG, "context and surrogate use different devices");
-  if (s->bound == c->serial && s->bN == c->N) return MFREE_OK;
+  if (s->bound == c->serial && s->bN == c->N) {
+    if (s->bver == c->state_version) return MFREE_OK;
+    // new state written into the context: h may have changed, and with it
+    // the pair pass the base step may emit the graph from
+    SCHK(s, hipStreamSynchronize(c->stream));
+    drop_graphs(s);
+    mfree_status st2 = check_uniform_h(s, c);
+    if (st2 != MFREE_OK) return st2;
+    s->bver = c->state_version;
+    return MFREE_OK;
+  }
   SCHK(s, hipDeviceSynchronize());
   drop_graphs(s);
   free_list(s->callocs);
   const size_t N = c->N;
   SCHK(s, salloc(s->callocs, &s->base, 8 * N));
   SCHK(s, salloc(s->callocs, &s->vprev, 2 * N));
-  SCHK(s, salloc(s->callocs, &s->cand, 2 * N));
-  SCHK(s, salloc(s->callocs, &s->devp, 2 * kDevBlocks));
+  SCHK(s, salloc(s->callocs, &s->jstart, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->cand, 8 * N));
+  SCHK(s, salloc(s->callocs, &s->devp, kJudgeSums * kJudgeGroups * (size_t)c->nchunks));
   SCHK(s, salloc(s->callocs, &s->inv, N));
   SCHK(s, salloc(s->callocs, &s->comp, c->S));
   SCHK(s, salloc(s->callocs, &s->snap, 10 * N));
@@ -767,17 +813,11 @@ mfree_status bind_ctx(mfree_surrogate* s, mfree_ctx* c) {
   std::string msg;
   if (mp_graph_reserve(s->graph, c->N, msg) != MFREE_OK || mp_acts_reserve(s->acts, c->N, false, msg) != MFREE_OK)
     return sfail(s, MFREE_ERR_HIP, msg);
-  {
-    // h is constant over a run: one check at binding
-    std::vector<double> hh(N);
-    SCHK(s, hipMemcpy(hh.data(), c->h[c->cur], N * sizeof(double), hipMemcpyDeviceToHost));
-    bool u = true;
-    for (size_t b = 0; b < c->S && u; b++)
-      for (uint32_t k = c->off[b] + 1; k < c->off[b + 1] && u; k++) u = hh[k] == hh[c->off[b]];
-    s->uniform_h = u;
-  }
+  st = check_uniform_h(s, c);
+  if (st != MFREE_OK) return st;
   s->bound = c->serial;
   s->bN = c->N;
+  s->bver = c->state_version;
   s->vprev_valid = false;
   return MFREE_OK;
 }
@@ -874,9 +914,10 @@ void advance_time_host(mfree_ctx* c, uint32_t nsteps) {
 
 // Device work of one learned step of every body (capturable: kernels only);
 // storage moves to the other buffer.
-void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
+void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net, double* keep_base = nullptr) {
   enqueue_base(s, c, use_net, true);
   StepArgs a = step_args(s, c);
+  a.base = keep_base;  // the base step's result by gid, before the correction
   if (use_net) {
     a.X0 = s->acts.X0;
     // features and the encoder first (they read the state), then the fused
@@ -901,9 +942,9 @@ void enqueue_learned_device(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
 // One learned step of every body (no synchronisation).  With graphs enabled
 // on the context, the network step is replayed from a graph captured once
 // per starting storage buffer.
-mfree_status enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
+mfree_status enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net, double* keep_base = nullptr) {
   const int a = c->cur;
-  if (use_net && c->opt.use_graph) {
+  if (use_net && c->opt.use_graph && !keep_base) {
     if (!s->lexec[a]) {
       hipGraph_t g;
       SCHK(s, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -920,7 +961,7 @@ mfree_status enqueue_learned(mfree_surrogate* s, mfree_ctx* c, bool use_net) {
     SCHK(s, hipGraphLaunch(s->lexec[a], c->stream));
     c->cur = 1 - a;
   } else {
-    enqueue_learned_device(s, c, use_net);
+    enqueue_learned_device(s, c, use_net, keep_base);
   }
   advance_time_host(c, kStepsPerLearned);
   return MFREE_OK;
@@ -1143,20 +1184,62 @@ mfree_status sync_ctx(mfree_surrogate* s, mfree_ctx* c) {
   return MFREE_OK;
 }
 
-// Judge deviation: rel L2 between the ctx positions and s->cand (by gid).
-mfree_status deviation(mfree_surrogate* s, mfree_ctx* c, double* out) {
+// Judge of a learned step, per body, from the increments over the judged
+// interval of the learned step (u_L), of its own base step (u_B, the same
+// step without the network's correction) and of the plain RK4 steps (u_P),
+// all from the judged state u_0:
+//   positions: || (x_L - x_0) - (x_P - x_0) || / || x_P - x_0 ||  (floored at
+//              1e-6 || x_0 ||): the relative error of the displacement;
+//   velocities, density, stresses: || u_L - u_P || / || u_B - u_P ||, how
+//              many times the network's correction leaves the step further
+//              from the plain steps than the base step alone is (the
+//              network should bring it closer: a ratio below 1).  Where the
+//              base step is nearly exact (a body in free flight), the
+//              denominator is floored at kNetFloor of the body's scale of the
+//              field (c0, rho0, K: the units of the network's output) per
+//              particle, so that errors below that level always pass.
+// out[0..3]: the largest of each measure over the bodies; *worst_body: the
+// body of the largest position deviation or ratio excess.
+struct JudgeResult {
+  double pos = 0.0, ratio[3] = {0.0, 0.0, 0.0};
+  uint32_t body = 0;
+};
+mfree_status judge(mfree_surrogate* s, mfree_ctx* c, JudgeResult* r) {
   const int b = c->cur;
-  hipLaunchKernelGGL(k_deviation, dim3(kDevBlocks), dim3(256), 0, c->stream, c->N, c->y[b].f[0], c->y[b].f[1],
-                     c->gid[b], s->cand, s->devp);
-  std::vector<double> h(2 * kDevBlocks);
+  constexpr int kS = kJudgeSums * kJudgeGroups;
+  hipLaunchKernelGGL(k_judge_partial, dim3(c->nchunks), dim3(256), 0, c->stream, c->chunks, c->N, c->y[b],
+                     c->gid[b], s->jstart, s->cand, s->base, s->devp);
+  std::vector<double> h(kS * (size_t)c->nchunks);
   SCHK(s, hipMemcpyAsync(h.data(), s->devp, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   SCHK(s, hipStreamSynchronize(c->stream));
-  double a = 0.0, bb = 0.0;
-  for (uint32_t k = 0; k < kDevBlocks; k++) {
-    a += h[2 * k];
-    bb += h[2 * k + 1];
+  *r = JudgeResult();
+  double worst = -1.0;
+  // chunks were built body by body, kChunk particles each
+  uint32_t ch = 0;
+  for (uint32_t bd = 0; bd < c->S; bd++) {
+    double t[kS] = {};
+    for (uint32_t o = c->off[bd]; o < c->off[bd + 1]; o += kChunk, ch++)
+      for (int k = 0; k < kS; k++) t[k] += h[ch * kS + k];
+    const double dpos = t[2] + 1e-12 * t[3];
+    const double pos = dpos > 0.0 ? sqrt(t[0] / dpos) : (t[0] > 0.0 ? INFINITY : 0.0);
+    double bad = pos;
+    if (!(pos <= r->pos)) r->pos = pos;
+    const SceneConst& k = c->h_sc[bd];
+    const double scale[3] = {sqrt(k.c0sq), k.rho0, k.K};
+    const double np = (double)(c->off[bd + 1] - c->off[bd]);
+    for (int q = 1; q < kJudgeGroups; q++) {
+      const double nl = t[kJudgeSums * q], nb = t[kJudgeSums * q + 1];
+      const double fl = kNetFloor * scale[q - 1];
+      const double den = nb + fl * fl * np;
+      const double ratio = den > 0.0 ? sqrt(nl / den) : (nl > 0.0 ? INFINITY : 0.0);
+      if (!(ratio <= r->ratio[q - 1])) r->ratio[q - 1] = ratio;
+      if (!(ratio <= bad)) bad = ratio;
+    }
+    if (!(bad <= worst)) {
+      worst = bad;
+      r->body = bd;
+    }
   }
-  *out = bb > 0.0 ? sqrt(a / bb) : 0.0;
   return MFREE_OK;
 }
 
@@ -1471,6 +1554,12 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
   }
   std::vector<double> times;
   const size_t N = c->N;
+  // a numerical failure of the learned steps (not of the plain ones) makes
+  // the group fall back like a rejected one
+  auto numeric = [](mfree_status r) {
+    return r == MFREE_ERR_NONFINITE || r == MFREE_ERR_NBH_OVERFLOW || r == MFREE_ERR_NO_NEIGHBOR ||
+           r == MFREE_ERR_CELLS || r == MFREE_ERR_SKIN;
+  };
   // groups of `group` learned steps; the last one may be shorter (>= 2)
   while (nsteps - done >= 2 * kStepsPerLearned) {
     const uint32_t glen = std::min(group, (nsteps - done) / kStepsPerLearned);
@@ -1482,40 +1571,66 @@ mfree_status mfree_surrogate_rollout(mfree_surrogate* s, mfree_ctx* c, uint32_t 
       if (st != MFREE_OK) return st;
     }
     st = sync_ctx(s, c);
-    if (st != MFREE_OK) return st;
+    bool failed = numeric(st);
+    if (st != MFREE_OK && !failed) return st;
     js.ms_learned += ms_since(t0);
-    // judge: the last learned step and 4 plain steps from the same state
     auto t1 = std::chrono::steady_clock::now();
-    // (the learned step writes the other storage buffer: the judged state
-    // stays in this one)
-    const int a0 = c->cur;
-    copy_async(s->jsc, c->sc, c->S * sizeof(SceneConst), c->stream);
-    const std::vector<double> jtimes = c->time;
-    st = enqueue_learned(s, c, true);  // also records the velocities of this state
-    if (st != MFREE_OK) return st;
-    const int b = c->cur;
-    hipLaunchKernelGGL(k_pos_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b].f[0],
-                       c->y[b].f[1], c->gid[b], s->cand);
-    c->cur = a0;
-    copy_async(c->sc, s->jsc, c->S * sizeof(SceneConst), c->stream);
-    c->time = jtimes;
-    st = sync_ctx(s, c);
-    if (st != MFREE_OK) return st;
-    st = plain(kStepsPerLearned);
-    if (st != MFREE_OK) return st;
-    double dev = 0.0;
-    st = deviation(s, c, &dev);
-    if (st != MFREE_OK) return st;
-    js.ms_judge += ms_since(t1);
+    JudgeResult jr;
+    if (!failed) {
+      // judge: the last learned step (its base step kept) and 4 plain steps
+      // from the same state (the learned step writes the other storage
+      // buffer: the judged state stays in this one)
+      const int a0 = c->cur;
+      copy_async(s->jsc, c->sc, c->S * sizeof(SceneConst), c->stream);
+      const std::vector<double> jtimes = c->time;
+      hipLaunchKernelGGL(k_state_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[a0],
+                         c->gid[a0], s->jstart);
+      st = enqueue_learned(s, c, true, s->base);  // also records the velocities of this state
+      if (st != MFREE_OK) return st;
+      const int b = c->cur;
+      hipLaunchKernelGGL(k_state_by_gid, dim3(nblk(N, 256)), dim3(256), 0, c->stream, (uint32_t)N, c->y[b],
+                         c->gid[b], s->cand);
+      c->cur = a0;
+      copy_async(c->sc, s->jsc, c->S * sizeof(SceneConst), c->stream);
+      c->time = jtimes;
+      st = sync_ctx(s, c);
+      failed = numeric(st);
+      if (st != MFREE_OK && !failed) return st;
+    }
+    if (!failed) {
+      st = plain(kStepsPerLearned);
+      if (st != MFREE_OK) return st;
+      st = judge(s, c, &jr);
+      if (st != MFREE_OK) return st;
+      js.ms_judge += ms_since(t1);
+      js.last_deviation = jr.pos;
+      if (!(jr.pos <= js.max_deviation)) js.max_deviation = jr.pos;
+      js.max_ratio_vel = std::max(js.max_ratio_vel, jr.ratio[0]);
+      js.max_ratio_rho = std::max(js.max_ratio_rho, jr.ratio[1]);
+      js.max_ratio_stress = std::max(js.max_ratio_stress, jr.ratio[2]);
+    }
     js.groups++;
-    js.last_deviation = dev;
-    js.max_deviation = std::max(js.max_deviation, dev);
-    js.learned_steps += glen;
-    if (!(dev <= threshold)) {
-      // roll back the group and redo it with plain RK4 steps
+    // (an infinite threshold accepts every group that completes)
+    const bool ratios_ok = std::isinf(threshold) && threshold > 0
+                               ? true
+                               : jr.ratio[0] <= kNetRatioLimit && jr.ratio[1] <= kNetRatioLimit &&
+                                     jr.ratio[2] <= kNetRatioLimit;
+    const bool accept = !failed && jr.pos <= threshold && ratios_ok;
+    if (accept) {
+      // kept: the group's learned steps but the judged one, whose 4 plain
+      // steps are counted in plain_steps
+      js.learned_steps += glen - 1;
+    } else {
+      // roll back the group and redo it with plain RK4 steps (the judge's
+      // plain steps are discarded with it)
       auto t2 = std::chrono::steady_clock::now();
       js.fallbacks++;
-      js.learned_steps -= glen;
+      if (failed) {
+        js.numeric_fallbacks++;
+      } else {
+        js.worst_body = jr.body;
+        js.plain_steps -= kStepsPerLearned;
+      }
       restore(s, c, times);
       st = plain(gspan - kStepsPerLearned);
       if (st != MFREE_OK) return st;

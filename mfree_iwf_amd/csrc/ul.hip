@@ -140,6 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
     a.rec32[i] = r;
   }
   if (a.reorder) {
+    if (a.perm_keep) a.perm_keep[i] = p;
 #pragma unroll
     for (int k = 0; k < 8; k++) a.dst.f[k][i] = v[k];
     a.m_dst[i] = m;
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
 // Epilogue + RK4 stage update shared by both precisions.  d = derivatives.
 __device__ inline void ul_finish(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t i, uint32_t home,
                                  double* d, const double* auxv) {
-  if (!finite8(d)) atomicOr(a.err, EB_NONFINITE);
+  if (!finite8(d)) raise_err(a.err, a.berr, (uint32_t)(c - a.sc), EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t g = a.gid[home];
     double* o = a.d8 + 8 * (size_t)g;
@@ -186,7 +187,9 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   const SceneGeom g = a.geom[s];
   const uint32_t local = mt.x - c->cell_offset;
   const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  // a body whose grid fell back to one cell (g.err) fails the step: scan no
+  // cells rather than all n particles, and let the epilogue write finite values
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
   const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
 
   const double2 pi = a.pos64[i];
@@ -323,8 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
     }
   }
   flush();
-  if (a.max_nbh && cnt >= a.max_nbh) atomicOr(a.err, EB_NBH);
-  if (cnt == 0) atomicOr(a.err, EB_NONEIGH);
+  if (a.max_nbh && cnt >= a.max_nbh) raise_err(a.err, a.berr, s, EB_NBH);
+  if (cnt == 0 && !g.err) raise_err(a.err, a.berr, s, EB_NONEIGH);
 
   const uint32_t home = a.home_identity ? i : a.perm[i];
   const double rhoi = ri.rho;
@@ -366,10 +369,25 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   ul_finish(a, c, i, home, d, auxv);
 }
 
-// Time advance of every body (src/rk4.h:107, simulation_time::increment_time).
-__global__ void k_advance_time(uint32_t nscene, SceneConst* sc) {
+// Time advance of every body (src/rk4.h:107, simulation_time::increment_time);
+// with per-body error words, a failed (frozen) body keeps its time.
+__global__ void k_advance_time(uint32_t nscene, SceneConst* sc, const uint32_t* berr) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nscene) sc[s].time = sc[s].time + sc[s].dt;
+  if (s < nscene && !(berr && berr[s])) sc[s].time = sc[s].time + sc[s].dt;
+}
+
+// A failed body of a context that isolates its bodies keeps the state it
+// started the step with: the step's storage (sorted at stage 1) takes y_n
+// back from the previous storage through the stage-1 permutation.
+__global__ __launch_bounds__(kBlock) void k_restore_failed(uint32_t n, const uint32_t* __restrict__ scene_new,
+                                                           const uint32_t* __restrict__ berr,
+                                                           const uint32_t* __restrict__ perm1, Fields8 y_old,
+                                                           Fields8 y_new) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !berr[scene_new[i]]) return;
+  const uint32_t p = perm1[i];
+#pragma unroll
+  for (int k = 0; k < 8; k++) y_new.f[k][i] = y_old.f[k][p];
 }
 
 // nsteps additions of dt, one after the other (the same rounding as nsteps

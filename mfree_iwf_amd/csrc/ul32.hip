@@ -195,7 +195,7 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < 8; k++) ok = ok && isfinite(d[k]);
-  if (!ok) atomicOr(a.err, EB_NONFINITE);
+  if (!ok) raise_err(a.err, a.berr, (uint32_t)(c - a.sc), EB_NONFINITE);
   if (a.stage == 0) {
     const uint32_t gid = a.gid[home];
     double* o = a.d8 + 8 * (size_t)gid;
@@ -234,7 +234,9 @@ __device__ inline void particle_global(const UlArgs& a, uint32_t i) {
   const SceneGeom g = a.geom[mt.y];
   const uint32_t local = mt.x - c->cell_offset;
   const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  // a body whose grid fell back to one cell (g.err) fails the step: scan no
+  // cells rather than all n particles, and let the epilogue write finite values
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
   const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
   const float cellf = (float)g.cell;
   const float2 pi = a.pos32[i];
@@ -284,7 +286,9 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   const SceneGeom g = a.geom[mt.y];
   const uint32_t local = mt.x - c->cell_offset;
   const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  // a body whose grid fell back to one cell (g.err) fails the step: scan no
+  // cells rather than all n particles, and let the epilogue write finite values
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
   const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
   const float cellf = (float)g.cell;
   const float2 pi = a.pos32[i];
@@ -377,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
   flush();
   if (graph) {
     if (gd > kMaxGraphDeg) {
-      atomicOr(a.err, EB_NBH);
+      raise_err(a.err, a.berr, mt.y, EB_NBH);
       gd = kMaxGraphDeg;
     }
     a.gdeg[i] = gd;
@@ -598,7 +602,7 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < 8; k++) ok = ok && isfinite(d[k]);
-  if (!ok && q == 0) atomicOr(a.err, EB_NONFINITE);
+  if (!ok && q == 0) raise_err(a.err, a.berr, (uint32_t)(c - a.sc), EB_NONFINITE);
   if (a.stage == 4) {
 #pragma unroll
     for (int k = 0; k < 8; k++)
@@ -616,7 +620,7 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
     }
   }
   const double ux = d[0] * dts, uy = d[1] * dts, half = 0.5 * g.skin;
-  if (q == 0 && ux * ux + uy * uy > half * half) atomicOr(a.err, EB_SKIN);
+  if (q == 0 && ux * ux + uy * uy > half * half) raise_err(a.err, a.berr, (uint32_t)(c - a.sc), EB_SKIN);
   const uint32_t local = key - c->cell_offset;
   const uint32_t ix = local / g.ny, iy = local - ix * g.ny;
   float2 pos;
@@ -725,7 +729,9 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
   const SceneGeom g = a.geom[mt.y];
   const uint32_t local = mt.x - c->cell_offset;
   const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = min(gi + 2, g.nx);
+  // a body whose grid fell back to one cell (g.err) fails the step: scan no
+  // cells rather than all n particles, and let the epilogue write finite values
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
   const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
   const float cellf = (float)g.cell;
   const float2 pi = a.pos32[i];
@@ -787,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
   }
   flush();
   if (deg > a.nl_cap) {
-    atomicOr(a.err, EB_NBH);
+    raise_err(a.err, a.berr, mt.y, EB_NBH);
     deg = a.nl_cap;
   }
   a.nl_deg[i] = deg;

@@ -14,7 +14,8 @@ from .scenes import ParticleSet
 class Solver:
     def __init__(self, method=_capi.METHOD_UL_MONAGHAN, precision=_capi.PRECISION_FP64, use_graph=True,
                  deterministic=True, max_nbh=200, device=0, tl_h_search=0.0, tile_cells=0,
-                 pipeline=_capi.PIPELINE_FUSED, xcd_chunk=0, cell_scale=0.0, neighbor_list=0, list_lanes=0):
+                 pipeline=_capi.PIPELINE_FUSED, xcd_chunk=0, cell_scale=0.0, neighbor_list=0, list_lanes=0,
+                 isolate_bodies=False):
         L = _capi.lib()
         opt = _capi.Options()
         L.mfree_default_options(C.byref(opt))
@@ -31,6 +32,7 @@ class Solver:
         opt.cell_scale = cell_scale
         opt.neighbor_list = neighbor_list
         opt.list_lanes = list_lanes
+        opt.isolate_bodies = 1 if isolate_bodies else 0
         h = C.c_void_p()
         _capi.check(L.mfree_create(C.byref(opt), C.byref(h)), what="mfree_create")
         self._h = h
@@ -99,6 +101,16 @@ class Solver:
         t = C.c_double(0)
         self._chk(self._L.mfree_get_time(self._h, body, C.byref(t)), "mfree_get_time")
         return t.value
+
+    def body_status(self, body=0):
+        """0 for a body that steps; the error status that froze it in a
+        context created with isolate_bodies=True."""
+        st = C.c_int32(0)
+        self._chk(self._L.mfree_body_status(self._h, body, C.byref(st)), "mfree_body_status")
+        return st.value
+
+    def failed_bodies(self):
+        return {k: s for k in range(len(self.sizes)) if (s := self.body_status(k)) != 0}
 
     def energy(self, body=0):
         """FP64 energy of a body: (kinetic, volumetric, deviatoric) in J per

@@ -258,9 +258,30 @@ def test_judge_always_rejecting_equals_plain_solver():
         st = su.rollout(a, 4 + 2 * 32 + 3, group=8, threshold=-1.0)
         b.step(4 + 2 * 32 + 3)
         assert st["groups"] == 2 and st["fallbacks"] == 2 and st["learned_steps"] == 0
+        assert st["plain_steps"] == 4 + 2 * 32 + 3
         sa, sb = a.state(0).state8(), b.state(0).state8()
         assert np.array_equal(sa, sb)
         assert a.time(0) == pytest.approx(b.time(0), rel=1e-12)
+
+
+def test_judge_rejects_a_bad_network_at_the_default_threshold():
+    # a seeded network whose corrections are scaled to the state itself (target
+    # std 1 in units of h, c0, rho0, K) moves particles by about h per step:
+    # every group is rejected -- by the judge or as a numerical failure -- and
+    # the rollout equals the plain solver bit for bit
+    s = scenes.rings_config1()
+    nsteps = 4 + 3 * 8 * 4 + 3
+    with _ctx(s, PRECISION_MIXED) as a, _ctx(s, PRECISION_MIXED) as b, sg.Surrogate(seed=11) as su:
+        su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.ones(8)]))
+        st = su.rollout(a, nsteps, group=8)  # default threshold
+        print(f"[judge bad network] {st}", flush=True)
+        b.step(nsteps)
+        assert st["groups"] == 3 and st["fallbacks"] == 3 and st["learned_steps"] == 0
+        assert st["plain_steps"] == nsteps
+        # a body whose positions blew up takes no quadratic work before it fails
+        # (its binning fails at once; no cell sort, no all-pairs scan)
+        assert st["ms_learned"] < 5000.0
+        assert np.array_equal(a.state(0).state8(), b.state(0).state8())
 
 
 def test_rebind_to_a_new_context_of_the_same_size():
@@ -289,8 +310,13 @@ def test_rollout_accepting_counts_steps():
         su.set_params(norm=np.concatenate([np.zeros(16), np.ones(16), np.zeros(8), np.full(8, 1e-9)]))
         st = su.rollout(a, 4 + 3 * 64 + 5, group=16, threshold=np.inf)
         assert st["groups"] == 3 and st["fallbacks"] == 0
-        assert st["learned_steps"] == 48 and st["plain_steps"] == 4 + 3 * 4 + 5
-        assert 0.0 < st["max_deviation"] < 1e-3
+        # kept learned steps: 15 per group (the judged one's result is the
+        # plain steps'); every one of the 201 steps is counted once
+        assert st["learned_steps"] == 45 and st["plain_steps"] == 4 + 3 * 4 + 5
+        assert 4 * st["learned_steps"] + st["plain_steps"] == 4 + 3 * 64 + 5
+        # the network's output is switched off: the judged step is its base step
+        assert 0.0 < st["max_deviation"] < 1e-2
+        assert st["max_ratio_stress"] <= 1.1
         t = a.time(0)
     assert t == pytest.approx((4 + 3 * 64 + 5) * s.params.dt, rel=1e-9)
 
@@ -311,7 +337,8 @@ def test_batched_rollout_equals_separate_rollouts():
             sv.add_body(b.particles, b.params)
         su.set_params(w, norm)
         st = su.rollout(sv, nsteps, group=8, threshold=np.inf)
-        assert st["fallbacks"] == 0 and st["learned_steps"] == 2 * 8
+        assert st["fallbacks"] == 0 and st["learned_steps"] == 2 * 7  # the judged step of each group is replaced
+        assert 4 * st["learned_steps"] + st["plain_steps"] == nsteps
         together = [sv.state(k).state8() for k in range(3)]
     for k, b in enumerate(bodies):
         with Solver(precision=PRECISION_MIXED, **fixed) as sv, sg.Surrogate(seed=2) as su:
