@@ -203,6 +203,30 @@ mfree_status mfree_get_time(mfree_ctx* ctx, uint32_t body, double* time);
  * independent bodies.  Always MFREE_OK without isolate_bodies. */
 mfree_status mfree_body_status(mfree_ctx* ctx, uint32_t body, mfree_status* status);
 
+/* Results files.  Replaces vtk_writer_write (src/vtk_writer.h:27-127), which
+ * src/mfree_iwf.cpp:99-103 calls after every 10th step with
+ * results/out_%06d.vtk: a legacy ASCII VTK unstructured grid of every body
+ * (bodies in order, particles in idx order), POINTS (x, y, 0), one vertex
+ * cell per particle, POINT_DATA h, density, Sxx, Sxy, Syy (scalars) and
+ * velocity (vector), every value printed with %f, the same text as the
+ * reference's writer for the same state.
+ *
+ * mfree_vtk_write_async snapshots the current state on the context's stream
+ * (a gather into idx order, into a staging buffer of its own), copies it to
+ * pinned host memory on a second stream and formats the file on a host
+ * thread; it returns once the snapshot is enqueued, so steps may follow at
+ * once.  One file is in flight per context: the next call, mfree_vtk_wait
+ * and mfree_destroy wait for it; mfree_vtk_wait returns its status.
+ * mfree_vtk_write writes synchronously. */
+mfree_status mfree_vtk_write(mfree_ctx* ctx, const char* path);
+mfree_status mfree_vtk_write_async(mfree_ctx* ctx, const char* path);
+mfree_status mfree_vtk_wait(mfree_ctx* ctx);
+/* The same file from host arrays (x, y, h, rho, Sxx, Sxy, Syy, vx, vy of
+ * each body are read): nbodies bodies, body k with counts[k] particles.
+ * Needs no device. */
+mfree_status mfree_vtk_write_host(const char* path, uint32_t nbodies, const mfree_particles* bodies,
+                                  const uint32_t* counts);
+
 /* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for
  * every body: nsteps RK4 steps.  Each body advances its own time by its dt
  * per step. */

@@ -100,6 +100,10 @@ SIGNATURES = {
     "mfree_get_state": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Particles)]),
     "mfree_get_time": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
     "mfree_body_status": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int32)]),
+    "mfree_vtk_write": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "mfree_vtk_write_async": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "mfree_vtk_wait": (C.c_int32, [C.c_void_p]),
+    "mfree_vtk_write_host": (C.c_int32, [C.c_char_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]),
     "mfree_energy": (C.c_int32, [C.c_void_p, C.c_uint32, dptr]),
     "mfree_energy_noalloc": (C.c_int32, [C.c_int32, C.c_void_p, C.POINTER(Params), C.POINTER(Particles),
                                          C.c_uint32, C.c_void_p]),

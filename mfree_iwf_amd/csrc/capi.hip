@@ -18,17 +18,6 @@ using namespace mfree;
 
 namespace mfree {
 
-template <typename T>
-hipError_t dalloc(mfree_ctx* c, T** p, size_t count) {
-  void* q = nullptr;
-  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e == hipSuccess) {
-    c->allocs.push_back(q);
-    *p = (T*)q;
-  }
-  return e;
-}
 
 hipError_t alloc8(mfree_ctx* c, Fields8& f, size_t n) {
   for (int k = 0; k < 8; k++) {
@@ -675,6 +664,10 @@ __global__ __launch_bounds__(256) void k_to_idx_order(uint32_t n, uint32_t o, co
   dst[gid[o + s]] = src[o + s];
 }
 
+void launch_to_idx_order(hipStream_t st, uint32_t n, const uint32_t* gid, const double* src, double* dst) {
+  if (n) hipLaunchKernelGGL(k_to_idx_order, dim3((n + 255) / 256), dim3(256), 0, st, n, 0u, gid, src, dst);
+}
+
 }  // namespace mfree
 
 
@@ -809,6 +802,7 @@ void mfree_destroy(mfree_ctx* c) {
   for (int k = 0; k < 2; k++)
     if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
   for (auto e : c->ev_pool) hipEventDestroy(e);
+  vtk_free(c);
   tl_free(c->tl);
   ref_free(c);
   for (void* p : c->allocs) hipFree(p);

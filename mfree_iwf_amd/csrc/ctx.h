@@ -2,6 +2,7 @@
 // host helpers shared by capi.hip and surrogate.hip.
 #pragma once
 
+#include <algorithm>
 #include <atomic>
 #include <functional>
 #include <string>
@@ -21,7 +22,8 @@ using mfree::SceneGeom;
 using mfree::TlState;
 
 namespace mfree {
-struct RefPipe;  // reference-structured FP64 pipeline (baseline.hip)
+struct RefPipe;   // reference-structured FP64 pipeline (baseline.hip)
+struct VtkState;  // VTK writer: staging buffers, copy stream, worker (vtk.hip)
 }
 
 struct mfree_ctx {
@@ -77,6 +79,7 @@ struct mfree_ctx {
 
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
+  mfree::VtkState* vtk = nullptr;  // VTK writer (created by the first write)
   double* energy_buf = nullptr;   // mfree_energy partials (in allocs), then the 3 results
   size_t energy_out = 0;          // offset of the results in energy_buf
   // process-unique id: a learned stepper bound to a context compares ids, not
@@ -111,6 +114,19 @@ inline mfree_status fail(mfree_ctx* c, mfree_status s, const std::string& msg) {
       return fail(ctx, MFREE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
   } while (0)
 
+// Device allocation owned by the context (freed by mfree_destroy).
+template <typename T>
+hipError_t dalloc(mfree_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e == hipSuccess) {
+    c->allocs.push_back(q);
+    *p = (T*)q;
+  }
+  return e;
+}
+
 // Allocates and uploads everything at the first compute call.
 mfree_status commit(mfree_ctx* c);
 // Per-body device constants from the reference's formulas (src/simulation_data.cpp:32-47).
@@ -122,6 +138,10 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b);
 UlArgs ul_args(mfree_ctx* c, int b);
 // nsteps RK4 steps of every body (graphs when enabled).
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps);
+// Idx-order gather of one storage-order field of every body: dst[gid[s]] = src[s].
+void launch_to_idx_order(hipStream_t st, uint32_t n, const uint32_t* gid, const double* src, double* dst);
+// Joins a pending VTK write and frees the writer.
+void vtk_free(mfree_ctx* c);
 // Status of a body of a context that isolates its bodies (MFREE_OK: not failed).
 mfree_status body_status(const mfree_ctx* c, uint32_t body);
 

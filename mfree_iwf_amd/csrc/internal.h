@@ -27,11 +27,13 @@ __device__ inline uint32_t xcd_tile() {
 // reads stay mostly within one XCD's chunk) while a dense stretch of tiles
 // (a contact zone) is shared by several XCDs.
 constexpr uint32_t kXcdChunkDefault = 64;
-// Neighbour-list path: slots per particle (about 30 entries at rest with the
-// default skin, so up to about 4x compression), the skin's floor in units of
-// h_max and its multiple of dt max|v| (twice the largest pair approach of a
-// step at twice the fastest particle's speed).
-constexpr uint32_t kListCap = 128;
+// Neighbour-list path: slots per particle, the skin's floor in units of h_max
+// and its multiple of dt max|v| (twice the largest pair approach of a step at
+// twice the fastest particle's speed).  A list reaches 2 h + skin: about 30
+// entries at rest for slow bodies, up to about 85 at the largest skin (2 h_max,
+// Mach > 2 impacts), so 512 slots (2 KiB per particle) hold a 6-fold
+// compression before the list overflows (EB_NBH).
+constexpr uint32_t kListCap = 512;
 constexpr double kSkinMin = 0.25;
 constexpr double kSkinVfac = 4.0;
 // a skin above kSkinMax h_max (a particle moving h/2 in one step) is no

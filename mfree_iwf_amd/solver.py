@@ -38,6 +38,7 @@ class Solver:
         self._h = h
         self._L = L
         self.sizes = []
+        self._params = []
 
     # -- lifecycle
     def close(self):
@@ -67,6 +68,7 @@ class Solver:
         self._chk(self._L.mfree_add_body(self._h, C.byref(params), C.byref(cp), particles.n, C.byref(bid)),
                   "mfree_add_body")
         self.sizes.append(particles.n)
+        self._params.append(params)
         return bid.value
 
     def set_strip_bc(self, body, left, right, v_bc):
@@ -108,6 +110,20 @@ class Solver:
         st = C.c_int32(0)
         self._chk(self._L.mfree_body_status(self._h, body, C.byref(st)), "mfree_body_status")
         return st.value
+
+    def dt(self, body=0):
+        return self._params[body].dt
+
+    def write_vtk(self, path, wait=True):
+        """Writes every body's current state as a legacy VTK file (the
+        reference's results format).  wait=False returns once the snapshot
+        is taken; the file is written while later steps run."""
+        import os
+        f = self._L.mfree_vtk_write if wait else self._L.mfree_vtk_write_async
+        self._chk(f(self._h, os.fsencode(path)), "mfree_vtk_write")
+
+    def vtk_wait(self):
+        self._chk(self._L.mfree_vtk_wait(self._h), "mfree_vtk_wait")
 
     def failed_bodies(self):
         return {k: s for k in range(len(self.sizes)) if (s := self.body_status(k)) != 0}

@@ -4,19 +4,25 @@
 //   <n> <time>
 //   idx x y vx vy rho Sxx Sxy Syy        (one line per particle)
 //
-//   reference_main rings <nbox> <steps> [mixed]
-//   reference_main tensile <nbox> <steps> [mixed]
+//   reference_main rings <nbox> <steps> [mixed] [vtk]
+//   reference_main tensile <nbox> <steps> [mixed] [vtk]
+// vtk: results/out_%06d.vtk after every 10th step, as the reference's main
+// loop writes them (src/mfree_iwf.cpp:82,99-103).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "mfree/benchmarks/rings.h"
 #include "mfree/benchmarks/tensile.h"
+#include "mfree/vtk_writer.h"
 
 template <typename T>
-static int run(std::vector<body<T>> bodies, unsigned int steps) {
-  for (unsigned int s = 0; s < steps; s++)
+static int run(std::vector<body<T>> bodies, unsigned int steps, bool vtk) {
+  const unsigned int freq = 10;
+  for (unsigned int s = 0; s < steps; s++) {
     for (auto& b : bodies) b.step();
+    if (vtk && s % freq == 0) vtk_writer_write(bodies, s);
+  }
   for (auto& b : bodies) {
     T** p = b.get_particles();
     printf("%u %.17g\n", b.get_num_part(), simulation_time::getInstance().get_time());
@@ -33,9 +39,13 @@ int main(int argc, char** argv) {
     return 2;
   }
   const unsigned int nbox = (unsigned int)atoi(argv[2]), steps = (unsigned int)atoi(argv[3]);
-  if (argc > 4 && !strcmp(argv[4], "mixed")) mfree_adapter::precision() = MFREE_PRECISION_MIXED;
-  if (!strcmp(argv[1], "rings")) return run(rings_monaghan(nbox), steps);
-  if (!strcmp(argv[1], "tensile")) return run(tensile_tl(nbox), steps);
+  bool vtk = false;
+  for (int a = 4; a < argc; a++) {
+    if (!strcmp(argv[a], "mixed")) mfree_adapter::precision() = MFREE_PRECISION_MIXED;
+    if (!strcmp(argv[a], "vtk")) vtk = true;
+  }
+  if (!strcmp(argv[1], "rings")) return run(rings_monaghan(nbox), steps, vtk);
+  if (!strcmp(argv[1], "tensile")) return run(tensile_tl(nbox), steps, vtk);
   fprintf(stderr, "unknown driver %s\n", argv[1]);
   return 2;
 }

@@ -70,3 +70,25 @@ def test_tensile_tl_driver_vs_oracle():
     es = O.rel_l2(np.concatenate([d[:, 6], d[:, 7], d[:, 8]]), np.concatenate([o.Sxx, o.Sxy, o.Syy]))
     assert ep <= 1e-10 and es <= 1e-10, (ep, es)
     assert np.abs(o.Sxx).max() > 0
+
+
+def test_rings_driver_writes_the_reference_results_files(tmp_path):
+    """vtk_writer_write (include/mfree/vtk_writer.h) in the reference's main
+    loop cadence: results/out_%06d.vtk after steps 0, 10, 20; the last file
+    holds the state the driver prints after its 21 steps."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+    from vtk_format import vtk_text
+    assert os.path.exists(EXE), "build tests/cpp first (make -C tests/cpp)"
+    (tmp_path / "results").mkdir()
+    out = subprocess.run([EXE, "rings", "41", "21", "vtk"], cwd=tmp_path, check=True, capture_output=True, text=True,
+                         timeout=120).stdout
+    assert sorted(os.listdir(tmp_path / "results")) == ["out_000000.vtk", "out_000010.vtk", "out_000020.vtk"]
+    lines = out.strip().splitlines()
+    d = np.array([list(map(float, ln.split())) for ln in lines[1:]])
+    s = scenes.reference_rings(41)
+    st = ParticleSet.zeros(s.particles.n)
+    st.h[:] = s.particles.h  # h is constant in UL
+    for k, f in enumerate(("x", "y", "vx", "vy", "rho", "Sxx", "Sxy", "Syy")):
+        getattr(st, f)[:] = d[:, 1 + k]
+    assert (tmp_path / "results" / "out_000020.vtk").read_text() == vtk_text([st])

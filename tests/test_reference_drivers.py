@@ -1,9 +1,12 @@
 """The reference's own benchmark drivers (src/benchmarks/rings.cpp and
-tensile.cpp), compiled unchanged against include/mfree and linked with the
-library (tests/cpp/build_reference_drivers.sh, in a temporary directory:
-nothing of the reference enters the repository or travels with it).  The
-built program constructs the drivers' bodies on the CPU and reports which
-library method each would step with."""
+tensile.cpp) and its main program (src/mfree_iwf.cpp: the stepping loop,
+contact calls and the results writer every 10th step), compiled unchanged
+against include/mfree and linked with the library
+(tests/cpp/build_reference_drivers.sh, in a temporary directory: nothing of
+the reference enters the repository or travels with it).  A probe program
+constructs the drivers' bodies on the CPU and reports which library method
+each would step with; the reference's main, which steps forever, is linked
+only."""
 import os
 import subprocess
 
@@ -22,6 +25,8 @@ def test_reference_drivers_compile_link_and_map_to_pipelines(tmp_path):
         pytest.skip("library not built")
     subprocess.run([os.path.join(ROOT, "tests", "cpp", "build_reference_drivers.sh"), str(tmp_path), REF_SRC],
                    check=True, capture_output=True, text=True)
+    # the reference's main program links unchanged against the surface
+    assert os.access(tmp_path / "reference_main", os.X_OK)
     out = subprocess.run([str(tmp_path / "reference_drivers")], check=True, capture_output=True, text=True).stdout
     got = {}
     for line in out.splitlines():
