@@ -162,13 +162,17 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restric
   // num_cell < n is asserted by the reference (src/grid.h:114); the same
   // bound keeps every key inside the body's cell range here.
   const double ncell = fnx * fny;
-  if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n) || !skin_ok) {
-    const uint32_t bit = skin_ok ? EB_CELLS : EB_SKIN;
+  // a body that already failed (its error word is set: within this call of
+  // the steps, or frozen in a context that isolates its bodies) gets the
+  // one-cell fallback too, so its pair passes scan nothing
+  const uint32_t failed = berr ? berr[s] : 0u;
+  if (!(fnx >= 1.0 && fny >= 1.0 && ncell < (double)sc[s].n) || !skin_ok || failed) {
+    const uint32_t bit = failed ? failed : skin_ok ? EB_CELLS : EB_SKIN;
     g.nx = 1;
     g.ny = 1;
     g.num_cell = 1;
     g.err = bit;
-    raise_err(err, berr, s, bit);
+    if (!failed) raise_err(err, berr, s, bit);
   } else {
     g.nx = (unsigned int)fnx;
     g.ny = (unsigned int)fny;

@@ -103,8 +103,9 @@ def test_set_state_thaws_a_failed_body():
 
 def test_mixed_list_path_freezes_a_list_overflow():
     # mixed precision (neighbour-list steps): a disc whose lists exceed the
-    # list capacity at the first step
-    a, b, c = _good(6), scenes.make_rings(41, ri=0.0, hdx=5.0), _good(7)
+    # list capacity (512 entries within 2 h + skin) at the first step: h = 8 dx
+    # gives about 1,000
+    a, b, c = _good(6), scenes.make_rings(41, ri=0.0, hdx=8.0), _good(7)
     with Solver(precision=PRECISION_MIXED, isolate_bodies=True) as sv:
         for s in (a, b, c):
             sv.add_body(s.particles, s.params)
