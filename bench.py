@@ -33,6 +33,13 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one stage launch of the pair kernel, per particle:
 # read state8 + m, h (80 B) and write 8 derivatives (64 B).
 ALG_BYTES_PER_PARTICLE_STAGE = 144.0
+# algorithmic flops of one stage per particle: 128 per active pair (kernel 29,
+# stress divergence 40, velocity gradient 14, viscosity 34, XSPH 11) times the
+# 20.85 active pairs per particle of the 1M ring workload, plus 100 for the
+# per-particle algebra and the RK update
+N_FLOP_PER_PARTICLE_STAGE = 20.85 * 128 + 100
+FP32_VALU_PEAK_TFLOPS = 157.3
+FP64_VALU_PEAK_TFLOPS = 78.6
 ALG_BYTES_PER_PARTICLE_STEP = 704.0
 
 
@@ -399,11 +406,16 @@ def bench_plate(steps, warmup, log, accuracy_steps=2000):
     return n, out
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(kernel, particles):
+    """The committed rocprof traffic figure of this kernel at this size, or None."""
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    for e in d.get("entries", [d]):
+        if e.get("kernel") == kernel and e.get("particles") == particles:
+            return e
     return None
 
 
@@ -545,9 +557,17 @@ def main():
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                 traffic=None, kernel="k_ul_list32<1>" if head is modes.get("mixed") else "k_ul_pairs64",
                 algorithmic_bytes_per_launch=ALG_BYTES_PER_PARTICLE_STAGE * n)
-    tr = traffic_from_profiles()
-    if tr and tr.get("kernel") == roof["kernel"] and tr.get("particles") == n:
+    tr = traffic_from_profiles(roof["kernel"], n)
+    if tr:
         roof["traffic"] = tr.get("hbm_bytes_per_launch")
+    # the arithmetic ceiling beside it: the frozen flop count of one stage
+    # (128 per active pair, 100 per particle, 20.85 active pairs per particle
+    # on this workload) against the vector peak of the pair terms' type
+    flops = N_FLOP_PER_PARTICLE_STAGE * n
+    af = flops / (head["pair_kernel_ms"] * 1e-3) / 1e12
+    vpeak = FP32_VALU_PEAK_TFLOPS if head is modes.get("mixed") else FP64_VALU_PEAK_TFLOPS
+    roof_valu = dict(bound="valu", achieved=af, peak=vpeak, unit="TFLOP/s", frac=af / vpeak, kernel=roof["kernel"],
+                     algorithmic_flops_per_launch=flops)
 
     cpu = None
     if rank == 0 and not args.no_cpu_baseline:
@@ -585,6 +605,7 @@ def main():
         "modes": {k: {kk: v for kk, v in r.items()} for k, r in modes.items()},
         "accuracy": acc,
         "roofline": roof,
+        "roofline_valu": roof_valu,
         "cpu_baseline": cpu,
         "copy_calibration": calib,
         "workloads": extra,
