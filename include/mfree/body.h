@@ -3,9 +3,10 @@
 // Drivers build bodies exactly as with the reference: new'ed particle
 // arrays, simulation_data, dt, boundary conditions, add_action(&...).  At
 // the first step() the registered action list is matched, by function
-// identity, against the two pipelines of the hot path:
+// identity, against the pipelines of the library:
 //   Gray-Monaghan UL  (src/benchmarks/rings.cpp:86-96)   -> particle_monaghan
 //   Total-Lagrangian  (src/benchmarks/tensile.cpp:276-286) -> particle_tl
+//   Godunov UL        (src/benchmarks/rings.cpp:151-158)  -> particle_ul (FP64)
 // and the body is moved to the GPU (SoA, idx order).  Any other action
 // list, particle type or boundary function is rejected with a message
 // (MFREE_ERR_PIPELINE): there is no CPU execution of the actions here.
@@ -167,8 +168,17 @@ class body {
                                &correctors_mghn_artificial_viscosity<T>,
                                &contmech_momentum_tl<T>,
                                &contmech_advection<T>};
+    // src/benchmarks/rings.cpp:151-158
+    const std::vector<A> gd = {&riemann_update_integration_weights<T>,
+                               &material_eos<T>,
+                               &derive_velocity<T>,
+                               &material_stress_rate_jaumann<T>,
+                               &godunov_continuity<T>,
+                               &godunov_momentum<T>,
+                               &contmech_advection<T>};
     if (m_actions == ul && std::is_base_of<particle_monaghan, T>::value) return MFREE_METHOD_UL_MONAGHAN;
     if (m_actions == tl && std::is_base_of<particle_tl, T>::value) return MFREE_METHOD_TL_RL;
+    if (m_actions == gd && std::is_base_of<particle_ul, T>::value) return MFREE_METHOD_UL_GODUNOV;
     return -1;
   }
 
@@ -203,7 +213,8 @@ class body {
     mfree_options opt;
     mfree_default_options(&opt);
     opt.method = method;
-    opt.precision = mfree_adapter::precision();
+    // the Godunov pair sums run in FP64 only
+    opt.precision = method == MFREE_METHOD_UL_GODUNOV ? MFREE_PRECISION_FP64 : mfree_adapter::precision();
     if (method == MFREE_METHOD_TL_RL) {
       auto it = mfree_adapter::tl_requests().find((const void*)m_particles);
       if (it == mfree_adapter::tl_requests().end() || !it->second.rl)

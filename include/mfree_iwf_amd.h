@@ -61,6 +61,9 @@ typedef int32_t mfree_status;
 /* Methods (selected in the reference by the particle type, src/body.h:60-148). */
 #define MFREE_METHOD_UL_MONAGHAN 0 /* particle_monaghan, rings_monaghan action list */
 #define MFREE_METHOD_TL_RL 1       /* particle_tl, tensile_tl action list */
+#define MFREE_METHOD_UL_GODUNOV 2  /* particle_ul, rings_godunov action list (src/benchmarks/rings.cpp:151-158):
+                                      Parshikov's Godunov-type continuity and momentum (src/godunov.h:41-189);
+                                      FP64 fused pipeline only */
 
 /* Pipelines of a UL FP64 context. */
 #define MFREE_PIPELINE_FUSED 0     /* fused per-stage kernels (default) */

@@ -271,6 +271,7 @@ UlArgs ul_args(mfree_ctx* c, int b) {
   memset(&u, 0, sizeof(u));
   u.n = c->N;
   u.mixed = c->opt.precision == MFREE_PRECISION_MIXED;
+  u.method = c->opt.method;
   u.sc = c->sc;
   u.geom = c->geom;
   u.perm = c->perm;
@@ -753,9 +754,16 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     c->opt = *opt;
   else
     mfree_default_options(&c->opt);
-  if (c->opt.method != MFREE_METHOD_UL_MONAGHAN && c->opt.method != MFREE_METHOD_TL_RL) {
+  if (c->opt.method != MFREE_METHOD_UL_MONAGHAN && c->opt.method != MFREE_METHOD_TL_RL &&
+      c->opt.method != MFREE_METHOD_UL_GODUNOV) {
     delete c;
     return MFREE_ERR_ARG;
+  }
+  // the Godunov pair sums exist in FP64 on the fused pipeline
+  if (c->opt.method == MFREE_METHOD_UL_GODUNOV &&
+      (c->opt.precision != MFREE_PRECISION_FP64 || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
+    delete c;
+    return MFREE_ERR_UNSUPPORTED;
   }
   if (c->opt.precision != MFREE_PRECISION_FP64 && c->opt.precision != MFREE_PRECISION_MIXED) {
     delete c;
@@ -786,8 +794,7 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     delete c;
     return MFREE_ERR_ARG;
   }
-  if (c->opt.isolate_bodies &&
-      (c->opt.method != MFREE_METHOD_UL_MONAGHAN || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
+  if (c->opt.isolate_bodies && (c->opt.method == MFREE_METHOD_TL_RL || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
     delete c;
     return MFREE_ERR_UNSUPPORTED;
   }

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mfree_iwf_amd.h"
+
 namespace mfree {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
@@ -97,6 +99,9 @@ struct Chunk {
 };
 
 // Sorted per-stage pair inputs, FP64 mode: one 128-byte record per particle.
+// Godunov bodies (src/godunov.h) store p in rho21, S_yy without the pressure
+// in syy (the reference's Godunov stress tensor, src/godunov.h:72-73) and no
+// artificial stress.
 struct alignas(16) PairRec64 {
   double vx, vy, rho, rho21;  // rho21 = 1/(rho*rho)
   double qw, m, h, c;         // qw = m/rho, c = sqrt(K/rho)

@@ -48,6 +48,10 @@ void launch_excl_scan(hipStream_t st, const uint32_t* in, uint32_t n, uint32_t* 
 struct UlArgs {
   uint32_t n;
   int mixed;                    // 0: FP64 pair records, 1: FP32 records
+  int method;                   // MFREE_METHOD_UL_MONAGHAN or MFREE_METHOD_UL_GODUNOV (FP64 only)
+  // contact forces by global id (src/particle.h:57, set between steps by the
+  // penalty contact); null: zero
+  const double *fcx, *fcy;
   const SceneConst* sc;
   const SceneGeom* geom;
   const uint32_t* perm;         // sorted position -> storage index

@@ -104,9 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
   const uint32_t key = a.key[p];
   a.meta[i] = make_uint2(key, s);
   if (!a.mixed) {
-    double Rxx, Rxy, Ryy;
-    art_stress(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
-    const double rho21 = 1. / (rho * rho);
+    // a Godunov body's record carries p in rho21 and S_yy without the pressure
+    const bool gd = a.method == MFREE_METHOD_UL_GODUNOV;
+    double Rxx = 0., Rxy = 0., Ryy = 0.;
+    if (!gd) art_stress(sxx, sxy, syy, rho, c->mghn_eps, Rxx, Rxy, Ryy);
+    const double rho21 = gd ? pr : 1. / (rho * rho);
     const double cs = sqrt(c->K / rho);
     a.pos64[i] = make_double2(x, y);
     PairRec64 r;
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
     r.c = cs;
     r.sxx = sxx;
     r.sxy = sxy;
-    r.syy = syy;
+    r.syy = gd ? Syy : syy;
     r.Rxx = Rxx;
     r.Rxy = Rxy;
     r.Ryy = Ryy;
@@ -176,7 +178,13 @@ constexpr int kNbCap = 32;
 // faster than 4 and 3.5 % faster than 2 at 1M particles).
 constexpr int kScanBatch64 = 8;
 
+// M = MFREE_METHOD_UL_MONAGHAN: the rings_monaghan action list;
+// M = MFREE_METHOD_UL_GODUNOV: the rings_godunov list (src/benchmarks/rings.cpp:151-158),
+// derive_velocity, godunov_continuity and godunov_momentum (src/godunov.h:91-189)
+// over the same candidate scan.
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
+  constexpr bool kGd = M == MFREE_METHOD_UL_GODUNOV;
   __shared__ uint32_t nb_lds[kNbCap * kBlock];
   uint32_t* const my = nb_lds + threadIdx.x;
   const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
@@ -210,6 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   double Sxx_x = 0., Sxy_y = 0., Sxy_x = 0., Syy_y = 0.;
   double vx_x = 0., vx_y = 0., vy_x = 0., vy_y = 0.;
   double ax = 0., ay = 0., xs = 0., ys = 0.;
+  double rho_t = 0.;  // Godunov continuity
   uint32_t cnt = 0;
   int nb = 0;
 
@@ -237,6 +246,56 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
         wx = xij * der * fac;
         wy = yij * der * fac;
       }
+    }
+    if constexpr (kGd) {
+      // derive_velocity
+      vx_x += (rj.vx - ri.vx) * wx * rj.qw;
+      vx_y += (rj.vx - ri.vx) * wy * rj.qw;
+      vy_x += (rj.vy - ri.vy) * wx * rj.qw;
+      vy_y += (rj.vy - ri.vy) * wy * rj.qw;
+      // godunov_continuity with the acoustic Riemann solver (src/godunov.h:41-56,
+      // 155-189); p is in rho21, c = sqrt(K/rho)
+      const double nx = xij / rij;
+      const double ny = yij / rij;
+      const double vir = ri.vx * nx + ri.vy * ny;
+      const double vjr = rj.vx * nx + rj.vy * ny;
+      const double frac = 1. / (ri.rho * ri.c + rj.rho * rj.c);
+      const double vstar = frac * (vir * ri.rho * ri.c + vjr * rj.rho * rj.c - ri.rho21 + rj.rho21);
+      rho_t += 2 * ri.rho * (vir - vstar) * (wx * nx + wy * ny) * rj.qw;
+      // godunov_momentum (src/godunov.h:58-153): R = normalize(x_j - x_i),
+      // S = normalize(R rotated by -90 degrees), glm::normalize(v) =
+      // v * (1/sqrt(dot(v, v))); the transformation l = glm::mat2x2(R, S) is
+      // single precision (its entries are R, S rounded to float); sigma has
+      // S_yy without the pressure; SRijstar[1] reads sigma_j twice -- all as
+      // the reference computes them.
+      const double Rx = pj.x - pi.x;
+      const double Ry = pj.y - pi.y;
+      const double Sx = Ry, Sy = -Rx;
+      const double iR = 1. / sqrt(Rx * Rx + Ry * Ry);
+      const double iS = 1. / sqrt(Sx * Sx + Sy * Sy);
+      const double Rnx = Rx * iR, Rny = Ry * iR;
+      const double Snx = Sx * iS, Sny = Sy * iS;
+      const double l00 = (double)(float)Rnx, l01 = (double)(float)Rny;
+      const double l10 = (double)(float)Snx, l11 = (double)(float)Sny;
+      // sigma * R, then l * (.): (M v)[r] = M[0][r] v.x + M[1][r] v.y
+      const double siR0 = ri.sxx * Rnx + ri.sxy * Rny, siR1 = ri.sxy * Rnx + ri.syy * Rny;
+      const double sjR0 = rj.sxx * Rnx + rj.sxy * Rny, sjR1 = rj.sxy * Rnx + rj.syy * Rny;
+      const double si0 = l00 * siR0 + l10 * siR1;
+      const double sj0 = l00 * sjR0 + l10 * sjR1, sj1 = l01 * sjR0 + l11 * sjR1;
+      const double vi0 = l00 * ri.vx + l10 * ri.vy, vi1 = l01 * ri.vx + l11 * ri.vy;
+      const double vj0 = l00 * rj.vx + l10 * rj.vy, vj1 = l01 * rj.vx + l11 * rj.vy;
+      const double rfrac = 1. / (ri.rho * ri.c + rj.rho * rj.c);
+      const double SR0 =
+          rfrac * (sj0 * ri.rho * ri.c + si0 * rj.rho * rj.c + ri.rho * ri.c * rj.rho * rj.c * (vj0 - vi0));
+      const double SR1 =
+          rfrac * (sj1 * ri.rho * ri.c + sj1 * rj.rho * rj.c + ri.rho * ri.c * rj.rho * rj.c * (vj1 - vi1));
+      // glm::transpose(l) * SR: [r] = l[r][0] SR0 + l[r][1] SR1
+      const double SRij0 = l00 * SR0 + l01 * SR1;
+      const double SRij1 = l10 * SR0 + l11 * SR1;
+      const double mag_r = sqrt(Rx * Rx + Ry * Ry);
+      if (fabs(Rx) > 0.) ax += 2. / ri.rho * wx * mag_r / Rx * SRij0 * rj.qw;
+      if (fabs(Ry) > 0.) ay += 2. / ri.rho * wy * mag_r / Ry * SRij1 * rj.qw;
+      return;
     }
     // derive_stress_monaghan
     double Rxx = 0., Rxy = 0., Ryy = 0.;
@@ -331,6 +390,8 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
 
   const uint32_t home = a.home_identity ? i : a.perm[i];
   const double rhoi = ri.rho;
+  // contact forces (zero unless the penalty contact set them)
+  const double fcx = a.fcx ? a.fcx[a.gid[home]] : 0., fcy = a.fcy ? a.fcy[a.gid[home]] : 0.;
   Sxx_x = Sxx_x * rhoi;
   Sxy_y = Sxy_y * rhoi;
   Sxy_x = Sxy_x * rhoi;
@@ -341,16 +402,24 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
   double d[8];
   d[0] = xs + ri.vx;
   d[1] = ys + ri.vy;
-  d[2] = ax + (1. / rhoi * (Sxx_x + Sxy_y) + 0. / ri.m);
-  d[3] = ay + (1. / rhoi * (Sxy_x + Syy_y) + 0. / ri.m);
-  d[4] = 0. - rhoi * (vx_x + vy_y);
+  if constexpr (kGd) {
+    // godunov_momentum adds f_c/m after its pair sum; godunov_continuity
+    // assigns rho_t (src/godunov.h:148-152,187)
+    d[2] = ax + fcx / ri.m;
+    d[3] = ay + fcy / ri.m;
+    d[4] = rho_t;
+  } else {
+    d[2] = ax + (1. / rhoi * (Sxx_x + Sxy_y) + fcx / ri.m);
+    d[3] = ay + (1. / rhoi * (Sxy_x + Syy_y) + fcy / ri.m);
+    d[4] = 0. - rhoi * (vx_x + vy_y);
+  }
   d[5] = 0. + t00;
   d[6] = 0. + t01;
   d[7] = 0. + t11;
   double auxv[16];
   if (a.stage == 0 && a.aux) {
     auxv[0] = ri.qw;
-    auxv[1] = c->c0sq * (rhoi - c->rho0);
+    auxv[1] = c->c0sq * (rhoi - c->rho0);  // zero sums below for Godunov bodies (no R, no stress divergence)
     auxv[2] = ri.Rxx;
     auxv[3] = ri.Rxy;
     auxv[4] = ri.Ryy;
@@ -448,8 +517,12 @@ void launch_ul_pairs32(hipStream_t st, const UlArgs& a);
 void launch_ul_pairs(hipStream_t st, const UlArgs& a) {
   if (a.mixed)
     launch_ul_pairs32(st, a);
+  else if (a.method == MFREE_METHOD_UL_GODUNOV)
+    hipLaunchKernelGGL(k_ul_pairs64<MFREE_METHOD_UL_GODUNOV>, dim3(xcd_grid(nblk(a.n, kBlock))), dim3(kBlock), 0,
+                       st, a);
   else
-    hipLaunchKernelGGL(k_ul_pairs64, dim3(xcd_grid(nblk(a.n, kBlock))), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_ul_pairs64<MFREE_METHOD_UL_MONAGHAN>, dim3(xcd_grid(nblk(a.n, kBlock))), dim3(kBlock), 0,
+                       st, a);
 }
 
 void launch_pair_list(hipStream_t st, const PairListArgs& a) {

@@ -147,6 +147,15 @@ def reference_rings(nbox):
     return make_rings(nbox, centre=0.04 + 0.001, hdx=1.7, rho0=1.0, v0=180.0, dt=1e-6)
 
 
+def reference_rings_godunov(nbox, **kw):
+    """rings_godunov (src/benchmarks/rings.cpp:101-161): the same lattice and
+    material as rings_monaghan, dt = 1e-7 and no corrector constants
+    (correction_constants() is all zero); the body is a particle_ul stepped
+    with the Godunov action list (method METHOD_UL_GODUNOV)."""
+    return make_rings(nbox, centre=0.04 + 0.001, hdx=1.7, rho0=1.0, v0=180.0, dt=1e-7, alpha=0.0, beta=0.0,
+                      eta=0.0, art_eps=0.0, corr_exp=0.0, xsph_eps=0.0, **kw)
+
+
 @dataclass
 class PlateScene:
     particles: ParticleSet

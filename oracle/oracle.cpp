@@ -294,8 +294,13 @@ int build_lists(uint32_t n, const double* x, const double* y, const double* h, i
   return ORC_OK;
 }
 
-// One UL Monaghan derivative evaluation on a stage state (idx order).
-int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint32_t max_nbh,
+void godunov_particle(const orc_consts* c, UlStage& s, const uint64_t* R, const uint32_t* NB, const double* W,
+                      bool skip0, int64_t i);
+
+// One UL derivative evaluation on a stage state (idx order): the
+// Gray-Monaghan action list (method ORC_METHOD_MONAGHAN) or the Godunov one
+// (ORC_METHOD_GODUNOV).
+int ul_stage(const orc_consts* c, int method, uint32_t n, const orc_soa* st, int order, uint32_t max_nbh,
              UlStage& s, int threads, double* d8, double* aux) {
   int nt = set_threads(threads);
   int rc = build_lists(n, st->x, st->y, st->h, order, max_nbh, s, nullptr, nt);
@@ -345,6 +350,7 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
     // material_eos, src/material.h:37-44
     double c0 = sqrt(K / c->rho0);
     s.p[i] = c0 * c0 * (s.rho[i] - c->rho0);
+    if (method == ORC_METHOD_GODUNOV) continue;  // src/benchmarks/rings.cpp:151-158: no artificial stress
     // correctors_mghn_artificial_stress, src/correctors.h:37-81
     double rhoi = s.rho[i];
     double sxx = s.Sxx[i] - s.p[i];
@@ -380,6 +386,10 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
   const double alpha = c->alpha, beta = c->beta, eta = c->eta, xeps = c->xsph_eps;
 #pragma omp parallel for schedule(static) num_threads(nt)
   for (int64_t i = 0; i < (int64_t)n; i++) {
+    if (method == ORC_METHOD_GODUNOV) {
+      godunov_particle(c, s, R, NB, W, skip0, i);
+      continue;
+    }
     // derive_stress_monaghan, src/derivatives.h:378-443
     {
       double Sxxi = s.Sxx[i] - s.p[i];
@@ -537,6 +547,133 @@ int ul_stage(const orc_consts* c, uint32_t n, const orc_soa* st, int order, uint
   return ORC_OK;
 }
 
+// The actions of a Godunov (Parshikov) UL body after the quadrature weight and
+// the EOS, src/benchmarks/rings.cpp:151-158: derive_velocity, Jaumann rate,
+// godunov_continuity, godunov_momentum (src/godunov.h:91-189), advection.
+// Each action loops over all particles in the reference; they read only
+// state and the per-particle results of earlier actions, so evaluating them
+// per particle i gives the same values.
+void godunov_particle(const orc_consts* c, UlStage& s, const uint64_t* R, const uint32_t* NB, const double* W,
+                      bool skip0, int64_t i) {
+  const double K = K_of(c->E, c->nu);
+  const double G = G_of(c->E, c->nu);
+  const double xi = s.x[i], yi = s.y[i], vxi = s.vx[i], vyi = s.vy[i], rhoi = s.rho[i];
+  // derive_velocity, src/derivatives.h:32-66
+  {
+    double vx_x = 0., vx_y = 0., vy_x = 0., vy_y = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* w = W + 3 * q;
+      if (skip0 && w[0] == 0.) continue;  // q >= 2: W = grad W = 0, every term adds an exact +-0
+      double quad_weight = s.qw[j];
+      vx_x += (s.vx[j] - vxi) * w[1] * quad_weight;
+      vx_y += (s.vx[j] - vxi) * w[2] * quad_weight;
+      vy_x += (s.vy[j] - vyi) * w[1] * quad_weight;
+      vy_y += (s.vy[j] - vyi) * w[2] * quad_weight;
+    }
+    s.vx_x[i] = vx_x;
+    s.vx_y[i] = vx_y;
+    s.vy_x[i] = vy_x;
+    s.vy_y[i] = vy_y;
+  }
+  // material_stress_rate_jaumann, src/material.h:66-100
+  {
+    double vx_x = s.vx_x[i], vx_y = s.vx_y[i], vy_x = s.vy_x[i], vy_y = s.vy_y[i];
+    M2 epsdot = mk(vx_x, 0.5 * (vx_y + vy_x), 0.5 * (vx_y + vy_x), vy_y);
+    M2 omega = mk(0., 0.5 * (vy_x - vx_y), 0.5 * (vx_y - vy_x), 0.);
+    M2 S = mk(s.Sxx[i], s.Sxy[i], s.Sxy[i], s.Syy[i]);
+    M2 I = mk(1., 0., 0., 1.);
+    double tr = epsdot.m[0][0] + epsdot.m[1][1];
+    M2 St = add(sub(smul(2 * G, sub(epsdot, smul(1. / 3. * tr, I))), mul(S, omega)), mul(omega, S));
+    s.Sxx_t[i] += St.m[0][0];
+    s.Sxy_t[i] += St.m[0][1];
+    s.Syy_t[i] += St.m[1][1];
+  }
+  // godunov_continuity, src/godunov.h:155-189, with riemann_solver_acoustic
+  // (src/godunov.h:41-56); the self pair is skipped by idx (:170)
+  {
+    double rho_t = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      if ((int64_t)j == i) continue;
+      const double* w = W + 3 * q;
+      if (skip0 && w[0] == 0.) continue;  // q >= 2: grad W = 0, the term is an exact +-0
+      double quad_weight = s.qw[j];
+      double dx = xi - s.x[j];
+      double dy = yi - s.y[j];
+      double mag_r = sqrt(dx * dx + dy * dy);
+      double nx = dx / mag_r;
+      double ny = dy / mag_r;
+      double vir = vxi * nx + vyi * ny;
+      double vjr = s.vx[j] * nx + s.vy[j] * ny;
+      double rhol = rhoi, rhor = s.rho[j], pl = s.p[i], pr = s.p[j];
+      double csl = sqrt(K / rhol);
+      double csr = sqrt(K / rhor);
+      double frac = 1. / (rhol * csl + rhor * csr);
+      double vstar = frac * (vir * rhol * csl + vjr * rhor * csr - pl + pr);
+      rho_t += 2 * rhoi * (vir - vstar) * (w[1] * nx + w[2] * ny) * quad_weight;
+    }
+    s.rho_t[i] = rho_t;
+  }
+  // godunov_momentum, src/godunov.h:91-153, with godunov_get_trafo (:58-89).
+  // glm::normalize(v) = v * (1/sqrt(dot(v, v))).  The transformation
+  // t.l = glm::mat2x2(R, S) (:69) is a single-precision matrix: its entries
+  // are R and S rounded to float, then widened back when assigned to the
+  // double matrix.  SRijstar[1] takes sigma_j twice (:127), and sigma has
+  // S_yy without the pressure (:72-73); both as the reference does.
+  {
+    double ci = sqrt(K / rhoi);
+    double sxi = s.Sxx[i] - s.p[i], sxyi = s.Sxy[i], syi = s.Syy[i];
+    double vx_t = s.vx_t[i], vy_t = s.vy_t[i];
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      if ((int64_t)j == i) continue;
+      const double* w = W + 3 * q;
+      if (skip0 && w[0] == 0.) continue;  // q >= 2: grad W = 0, both terms are exact +-0
+      double Rx = s.x[j] - xi;
+      double Ry = s.y[j] - yi;
+      double Sx = Ry;
+      double Sy = -Rx;
+      double iR = 1. / sqrt(Rx * Rx + Ry * Ry);
+      double Rnx = Rx * iR, Rny = Ry * iR;
+      double iS = 1. / sqrt(Sx * Sx + Sy * Sy);
+      double Snx = Sx * iS, Sny = Sy * iS;
+      // columns of l: (R) and (S), single precision
+      double l00 = (double)(float)Rnx, l01 = (double)(float)Rny;
+      double l10 = (double)(float)Snx, l11 = (double)(float)Sny;
+      double sxj = s.Sxx[j] - s.p[j], sxyj = s.Sxy[j], syj = s.Syy[j];
+      // sigma * R, glm: (M v)[r] = M[0][r] v.x + M[1][r] v.y
+      double siR0 = sxi * Rnx + sxyi * Rny, siR1 = sxyi * Rnx + syi * Rny;
+      double sjR0 = sxj * Rnx + sxyj * Rny, sjR1 = sxyj * Rnx + syj * Rny;
+      double si0 = l00 * siR0 + l10 * siR1, si1 = l01 * siR0 + l11 * siR1;
+      (void)si1;  // the reference computes it, but SRijstar[1] reads sigma_j twice
+      double sj0 = l00 * sjR0 + l10 * sjR1, sj1 = l01 * sjR0 + l11 * sjR1;
+      double vi0 = l00 * vxi + l10 * vyi, vi1 = l01 * vxi + l11 * vyi;
+      double vxj = s.vx[j], vyj = s.vy[j];
+      double vj0 = l00 * vxj + l10 * vyj, vj1 = l01 * vxj + l11 * vyj;
+      double cj = sqrt(K / s.rho[j]);
+      double rhoj = s.rho[j];
+      double quad_weight = s.qw[j];
+      double Riemann_frac = 1. / (rhoi * ci + rhoj * cj);
+      double SR0 = Riemann_frac * (sj0 * rhoi * ci + si0 * rhoj * cj + rhoi * ci * rhoj * cj * (vj0 - vi0));
+      double SR1 = Riemann_frac * (sj1 * rhoi * ci + sj1 * rhoj * cj + rhoi * ci * rhoj * cj * (vj1 - vi1));
+      // glm::transpose(l) * SR: [r] = l[r][0] SR0 + l[r][1] SR1
+      double SRij0 = l00 * SR0 + l01 * SR1;
+      double SRij1 = l10 * SR0 + l11 * SR1;
+      double rx = s.x[j] - xi;
+      double ry = s.y[j] - yi;
+      double mag_r = sqrt(rx * rx + ry * ry);
+      if (fabs(rx) > 0.) vx_t += 2. / rhoi * w[1] * mag_r / rx * SRij0 * quad_weight;
+      if (fabs(ry) > 0.) vy_t += 2. / rhoi * w[2] * mag_r / ry * SRij1 * quad_weight;
+    }
+    s.vx_t[i] = vx_t + s.fcx[i] / s.m[i];
+    s.vy_t[i] = vy_t + s.fcy[i] / s.m[i];
+  }
+  // contmech_advection, src/cont_mech.h:134-142
+  s.x_t[i] += s.vx[i];
+  s.y_t[i] += s.vy[i];
+}
+
 // The 8 integrated fields of a state, src/rk4.h:52-69.
 struct State8 {
   std::vector<double> x, y, vx, vy, rho, Sxx, Sxy, Syy;
@@ -685,16 +822,27 @@ int orc_find_neighbors(uint32_t n, const double* x, const double* y, double h, u
 
 int orc_ul_derivatives(const orc_consts* c, uint32_t n, const orc_soa* s, int order, uint32_t max_nbh,
                        double* d8, double* aux) {
-  if (!c || !s || !d8 || n < 2) return ORC_ERR_ARG;
+  return orc_ul_derivatives_m(c, ORC_METHOD_MONAGHAN, n, s, order, max_nbh, d8, aux);
+}
+
+int orc_ul_derivatives_m(const orc_consts* c, int method, uint32_t n, const orc_soa* s, int order,
+                         uint32_t max_nbh, double* d8, double* aux) {
+  if (!c || !s || !d8 || n < 2 || (method != ORC_METHOD_MONAGHAN && method != ORC_METHOD_GODUNOV))
+    return ORC_ERR_ARG;
   UlStage st;
   resize_all(st, n);
-  return ul_stage(c, n, s, order, max_nbh, st, 1, d8, aux);
+  return ul_stage(c, method, n, s, order, max_nbh, st, 1, d8, aux);
 }
 
 int orc_ul_step(const orc_consts* c, uint32_t n, orc_soa* s, double dt, uint32_t nsteps, int order,
                 uint32_t max_nbh, int threads) {
-  // src/rk4.h:71-108 (step_strong) for a particle_monaghan body
-  if (!c || !s || n < 2) return ORC_ERR_ARG;
+  return orc_ul_step_m(c, ORC_METHOD_MONAGHAN, n, s, dt, nsteps, order, max_nbh, threads);
+}
+
+int orc_ul_step_m(const orc_consts* c, int method, uint32_t n, orc_soa* s, double dt, uint32_t nsteps, int order,
+                  uint32_t max_nbh, int threads) {
+  // src/rk4.h:71-108 (step_strong) for a particle_monaghan / particle_ul body
+  if (!c || !s || n < 2 || (method != ORC_METHOD_MONAGHAN && method != ORC_METHOD_GODUNOV)) return ORC_ERR_ARG;
   int nt = set_threads(threads);
   UlStage st;
   resize_all(st, n);
@@ -703,14 +851,14 @@ int orc_ul_step(const orc_consts* c, uint32_t n, orc_soa* s, double dt, uint32_t
   std::vector<double> d1(8 * (size_t)n), d2(8 * (size_t)n), d3(8 * (size_t)n), d4(8 * (size_t)n);
   for (uint32_t step = 0; step < nsteps; step++) {
     k.copy_from(n, s);
-    int rc = ul_stage(c, n, &k.soa, order, max_nbh, st, nt, d1.data(), nullptr);
+    int rc = ul_stage(c, method, n, &k.soa, order, max_nbh, st, nt, d1.data(), nullptr);
     if (rc) return rc;
     rk_add(n, s, d1.data(), 0.5 * dt, &k.soa, nt);
-    if ((rc = ul_stage(c, n, &k.soa, order, max_nbh, st, nt, d2.data(), nullptr))) return rc;
+    if ((rc = ul_stage(c, method, n, &k.soa, order, max_nbh, st, nt, d2.data(), nullptr))) return rc;
     rk_add(n, s, d2.data(), 0.5 * dt, &k.soa, nt);
-    if ((rc = ul_stage(c, n, &k.soa, order, max_nbh, st, nt, d3.data(), nullptr))) return rc;
+    if ((rc = ul_stage(c, method, n, &k.soa, order, max_nbh, st, nt, d3.data(), nullptr))) return rc;
     rk_add(n, s, d3.data(), dt, &k.soa, nt);
-    if ((rc = ul_stage(c, n, &k.soa, order, max_nbh, st, nt, d4.data(), nullptr))) return rc;
+    if ((rc = ul_stage(c, method, n, &k.soa, order, max_nbh, st, nt, d4.data(), nullptr))) return rc;
     rk_combine(n, s, d1.data(), d2.data(), d3.data(), d4.data(), dt, nt);
     for (uint32_t i = 0; i < n; i++)
       if (!isfinite(s->x[i]) || !isfinite(s->y[i]) || !isfinite(s->vx[i]) || !isfinite(s->vy[i]))

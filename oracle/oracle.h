@@ -115,6 +115,21 @@ int orc_ul_derivatives(const orc_consts* c, uint32_t n, const orc_soa* s, int or
 int orc_ul_step(const orc_consts* c, uint32_t n, orc_soa* s, double dt, uint32_t nsteps,
                 int order, uint32_t max_nbh, int threads);
 
+/* The same for a chosen UL action list:
+ * ORC_METHOD_MONAGHAN  rings_monaghan (src/benchmarks/rings.cpp:86-96);
+ * ORC_METHOD_GODUNOV   rings_godunov (src/benchmarks/rings.cpp:151-158):
+ *   quadrature weight, EOS, derive_velocity, Jaumann rate, godunov_continuity,
+ *   godunov_momentum (src/godunov.h:41-189), advection; particle_ul, so no
+ *   artificial stress, viscosity or XSPH.  aux[2..8] are 0 for it.
+ * fcx, fcy (contact forces, may be NULL) enter both momentum equations as
+ * f/m (src/cont_mech.h:44-45, src/godunov.h:148-152). */
+#define ORC_METHOD_MONAGHAN 0
+#define ORC_METHOD_GODUNOV 1
+int orc_ul_derivatives_m(const orc_consts* c, int method, uint32_t n, const orc_soa* s, int order,
+                         uint32_t max_nbh, double* d8, double* aux);
+int orc_ul_step_m(const orc_consts* c, int method, uint32_t n, orc_soa* s, double dt, uint32_t nsteps,
+                  int order, uint32_t max_nbh, int threads);
+
 /* ---------------- Total-Lagrangian (Randles-Libersky) ------------------ */
 
 /* Frozen neighbour lists + corrected kernel values of a TL body.

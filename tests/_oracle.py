@@ -59,6 +59,10 @@ def lib():
                                          dptr, dptr]
         L.orc_ul_step.argtypes = [C.POINTER(Consts), C.c_uint32, C.POINTER(Soa), C.c_double, C.c_uint32,
                                   C.c_int, C.c_uint32, C.c_int]
+        L.orc_ul_derivatives_m.argtypes = [C.POINTER(Consts), C.c_int, C.c_uint32, C.POINTER(Soa), C.c_int,
+                                           C.c_uint32, dptr, dptr]
+        L.orc_ul_step_m.argtypes = [C.POINTER(Consts), C.c_int, C.c_uint32, C.POINTER(Soa), C.c_double,
+                                    C.c_uint32, C.c_int, C.c_uint32, C.c_int]
         L.orc_tl_create.argtypes = [C.c_uint32, dptr, dptr, dptr, dptr, dptr, C.c_double, C.c_int, C.c_int,
                                     C.c_uint32, C.POINTER(C.c_void_p)]
         L.orc_tl_destroy.argtypes = [C.c_void_p]
@@ -101,7 +105,9 @@ def consts_from(params):
                   xsph_eps=params.xsph_eps)
 
 
-def soa_from(ps):
+def soa_from(ps, fc=None):
+    """C view of ps; fc = (fcx, fcy) arrays (idx order) or None.  The
+    arrays must stay alive while the view is used."""
     s = Soa()
     for f in ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy"):
         a = getattr(ps, f)
@@ -109,6 +115,10 @@ def soa_from(ps):
         setattr(s, f, _p(a))
     s.fcx = None
     s.fcy = None
+    if fc is not None:
+        for a in fc:
+            assert a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (ps.n,)
+        s.fcx, s.fcy = _p(fc[0]), _p(fc[1])
     return s
 
 
@@ -169,23 +179,30 @@ def find_neighbors_ref(x, y, h, cap=None):
     return rc, row, col[:row[-1]]
 
 
-def ul_derivatives(ps, params, order=ORDER_STABLE, max_nbh=200):
+METHOD_MONAGHAN = 0
+METHOD_GODUNOV = 1
+
+
+def ul_derivatives(ps, params, order=ORDER_STABLE, max_nbh=200, method=METHOD_MONAGHAN, fc=None):
+    """One UL derivative evaluation (the Gray-Monaghan action list, or the
+    Godunov one with method=METHOD_GODUNOV); fc = (fcx, fcy) contact forces."""
     n = ps.n
     d8 = np.zeros((n, 8))
     aux = np.zeros((n, 16))
     c = consts_from(params)
-    s = soa_from(ps)
-    rc = lib().orc_ul_derivatives(C.byref(c), n, C.byref(s), order, max_nbh, _p(d8), _p(aux))
+    s = soa_from(ps, fc)
+    rc = lib().orc_ul_derivatives_m(C.byref(c), method, n, C.byref(s), order, max_nbh, _p(d8), _p(aux))
     return rc, d8, aux
 
 
-def ul_step(ps, params, nsteps, order=ORDER_STABLE, threads=1, max_nbh=200, dt=None):
+def ul_step(ps, params, nsteps, order=ORDER_STABLE, threads=1, max_nbh=200, dt=None, method=METHOD_MONAGHAN,
+            fc=None):
     """Steps a copy of ps; returns (status, new ParticleSet)."""
     q = ps.copy()
     c = consts_from(params)
-    s = soa_from(q)
-    rc = lib().orc_ul_step(C.byref(c), q.n, C.byref(s), params.dt if dt is None else dt, nsteps, order,
-                           max_nbh, threads)
+    s = soa_from(q, fc)
+    rc = lib().orc_ul_step_m(C.byref(c), method, q.n, C.byref(s), params.dt if dt is None else dt, nsteps,
+                             order, max_nbh, threads)
     return rc, q
 
 

@@ -61,6 +61,16 @@ def test_cfl_time_step():
     assert abs(s.params.dt - 0.3 * 1.3e-3 / (c0 + 59.0)) < 1e-18
 
 
+def test_godunov_is_fp64_fused_only():
+    # the Godunov pair sums exist in FP64 on the fused pipeline; other
+    # combinations are refused at creation (no device needed)
+    for kw in (dict(precision=_capi.PRECISION_MIXED), dict(pipeline=_capi.PIPELINE_REFERENCE)):
+        with pytest.raises(_capi.MfreeError) as e:
+            Solver(method=_capi.METHOD_UL_GODUNOV, **kw)
+        assert e.value.status == _capi.ERR_UNSUPPORTED
+    Solver(method=_capi.METHOD_UL_GODUNOV).close()
+
+
 @pytest.mark.skipif(device_count() > 0, reason="checks the no-device path")
 def test_compute_fails_loudly_without_device():
     s = scenes.make_rings(21)

@@ -32,16 +32,18 @@ def test_reference_drivers_compile_link_and_map_to_pipelines(tmp_path):
     for line in out.splitlines():
         name, *bodies = line.split()
         got[name] = [tuple(int(v) for v in b.split(":")) for b in bodies]
-    UL, TL = 0, 1
-    # the hot path's two pipelines
+    UL, TL, GD = 0, 1, 2
+    # the library's pipelines
     assert [m for m, _ in got["rings_monaghan"]] == [UL] and got["rings_monaghan"][0][1] > 0
     assert [m for m, _ in got["tensile_tl"]] == [TL]
-    # two Monaghan rings as separate bodies (their contact runs in the main
-    # loop, contact.h)
+    assert [m for m, _ in got["rings_godunov"]] == [GD] and got["rings_godunov"][0][1] > 0
+    # two rings as separate bodies (their contact runs in the main loop,
+    # contact.h)
     assert [m for m, _ in got["rings_monaghan_contact"]] == [UL, UL]
-    # methods outside the hot path are refused, not run on the CPU
-    for name in ("rings_godunov", "rings_godunov_contact", "rings_corot_contact", "rings_potential_contact",
-                 "rings_tl_contact", "tensile_godunov", "tensile_corot", "tensile_potential",
+    assert [m for m, _ in got["rings_godunov_contact"]] == [GD, GD]
+    # methods outside the library are refused, not run on the CPU
+    for name in ("rings_corot_contact", "rings_potential_contact",
+                 "rings_tl_contact", "tensile_corot", "tensile_potential",
                  # UL with boundary conditions
-                 "tensile_monaghan"):
+                 "tensile_monaghan", "tensile_godunov"):
         assert all(m == -1 for m, _ in got[name]), name
