@@ -406,6 +406,62 @@ def bench_plate(steps, warmup, log, accuracy_steps=2000):
     return n, out
 
 
+def bench_godunov(steps, warmup, log, cpu_steps=1, nbox=1213):
+    """Godunov (Parshikov) UL SPH: the reference's rings_godunov scene
+    (src/benchmarks/rings.cpp:101-161: h = 1.7 dx, rho0 = 1, +-180 m/s,
+    dt = 1e-7 s) at dx = 6.6e-5 m (nbox = 1213, 1,009,204 particles), FP64.
+    The pair kernel is timed with HIP events on the context's stream; the
+    CPU oracle (one host thread, the reference's sort order) runs cpu_steps
+    steps of the same scene, and the GPU state after as many steps is
+    compared with it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from mfree_iwf_amd import METHOD_UL_GODUNOV, Solver, scenes
+    t0 = time.perf_counter()
+    sc = scenes.reference_rings_godunov(nbox)
+    n = sc.particles.n
+    log(f"godunov: {n} particles (setup {time.perf_counter() - t0:.1f} s)")
+    out = {}
+    with Solver(method=METHOD_UL_GODUNOV) as sv:
+        sv.add_body(sc.particles, sc.params)
+        if cpu_steps > 0:
+            sv.step(cpu_steps)
+            g = sv.state(0)
+        sv.step(max(warmup, 1))
+        sv.profile(True)
+        sv.step(steps)
+        prof = sv.profile_read()
+        sv.profile(False)
+        t0 = time.perf_counter()
+        sv.step(steps)
+        wall_graph = time.perf_counter() - t0
+    pair_ms = prof["pair_ms"] / max(prof["pair_launches"], 1)
+    achieved = ALG_BYTES_PER_PARTICLE_STAGE * n / (pair_ms * 1e-3) / 1e9
+    out["fp64"] = dict(particle_steps_per_s=n * steps / (prof["step_ms"] / 1e3),
+                       particle_steps_per_s_graph=n * steps / wall_graph, ms_per_step=prof["step_ms"] / steps,
+                       pair_kernel_ms=pair_ms,
+                       roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                                     frac=achieved / HBM_PEAK_GBS, kernel="k_ul_pairs64<MFREE_METHOD_UL_GODUNOV>",
+                                     algorithmic_bytes_per_launch=ALG_BYTES_PER_PARTICLE_STAGE * n))
+    log(f"  fp64: {out['fp64']['particle_steps_per_s']:.4e} particle-steps/s ({out['fp64']['ms_per_step']:.3f} "
+        f"ms/step), pair kernel {pair_ms:.3f} ms")
+    if cpu_steps > 0:
+        t0 = time.perf_counter()
+        rc, o = O.ul_step(sc.particles, sc.params, cpu_steps, order=O.ORDER_REFERENCE, threads=1,
+                          method=O.METHOD_GODUNOV)
+        dt = time.perf_counter() - t0
+        if rc != O.OK:
+            raise RuntimeError(f"oracle failed with status {rc}")
+        model, ncpu = cpu_model()
+        out["cpu_baseline"] = dict(value=n * cpu_steps / dt, unit="particle-steps/s", cores=1, kind="port",
+                                   sample=f"{n} particles (the full scene), {cpu_steps} RK4 step(s); SoA oracle "
+                                          f"(g++ -O2) in the reference's sort order on 1 thread",
+                                   cpu_model=model, host_logical_cpus=ncpu, seconds=dt)
+        out["gpu_vs_oracle"] = dict(steps=cpu_steps, **errors(g, o))
+        log(f"  cpu baseline: {cpu_steps} step(s) in {dt:.1f} s; GPU vs oracle: {out['gpu_vs_oracle']}")
+    return n, out
+
+
 def traffic_from_profiles(kernel, particles):
     """The committed rocprof traffic figure of this kernel at this size, or None."""
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
@@ -447,11 +503,11 @@ def main():
     ap.add_argument("--sur-threshold", type=float, default=1e-4)
     ap.add_argument("--sur-steps", type=int, default=0,
                     help="rollout length (default: the accuracy horizon, --accuracy-steps)")
-    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k"])
+    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k", "godunov1m"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
     ap.add_argument("--no-extra-workloads", action="store_true",
-                    help="skip the compact batch256 and plate578k results of the default line")
+                    help="skip the compact batch256, plate578k and godunov1m results of the default line")
     ap.add_argument("--oracle-state", default="",
                     help="npz of the oracle's state of the workload after --accuracy-steps steps: errors of every "
                          "mode against it are added")
@@ -500,6 +556,20 @@ def main():
             "config": {"workload": f"tension plate 0.1 x 0.04 m, hole r = 0.01 m, dx = 8e-5 m, {n} particles, "
                                    "strips 3 dx at +-1 m/s, mixed precision", "particles": n, "mode": "mixed"},
             "accuracy": acc, "modes": out}), flush=True)
+        return
+    if args.workload == "godunov1m":
+        n, out = bench_godunov(min(args.steps, 20), min(args.warmup, 3), log,
+                               cpu_steps=0 if args.no_cpu_baseline else 1)
+        head = out["fp64"]
+        print(json.dumps({
+            "metric": "particle-steps/s, Godunov (Parshikov) UL SPH ring collision",
+            "value": head["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": min(args.steps, 20), "warmup": min(args.warmup, 3), "ms_per_step": head["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"rings_godunov scene, dx = 6.6e-5 m, {n} particles, h = 1.7 dx, FP64",
+                       "particles": n, "mode": "fp64"},
+            "roofline": head["roofline"], "cpu_baseline": out.get("cpu_baseline"),
+            "gpu_vs_oracle": out.get("gpu_vs_oracle"), "modes": out}), flush=True)
         return
     t0 = time.perf_counter()
     scene = scenes.rings_config2()
@@ -585,6 +655,9 @@ def main():
         log("workload plate578k")
         npl, p = bench_plate(min(args.steps, 50), min(args.warmup, 5), log, accuracy_steps=2000)
         extra["plate578k"] = dict(particles=npl, unit="particle-steps/s", **p)
+        log("workload godunov1m")
+        ngd, gd = bench_godunov(20, 3, log, cpu_steps=0 if args.no_cpu_baseline else 1)
+        extra["godunov1m"] = dict(particles=ngd, unit="particle-steps/s", **gd)
 
     line = {
         "metric": METRIC,
