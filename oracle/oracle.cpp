@@ -746,9 +746,11 @@ struct StageCopy {
 // Total-Lagrangian body
 struct orc_tl {
   uint32_t n;
+  int variant = ORC_TL_RL;
   std::vector<uint64_t> rowp;
   std::vector<uint32_t> nbh;  // idx space
-  std::vector<double> w;      // corrected, 3 per pair
+  std::vector<double> w;      // corrected (RL, potential) or plain (co-rotated) kernel, 3 per pair
+  std::vector<double> wji;    // potential bodies: the kernel of the reverse pair, 3 per pair
   std::vector<double> C, B4;
   std::vector<double> qw;     // quad_weight = m/rho at creation (src/benchmarks/tensile.cpp:262)
   std::vector<uint32_t> bc_left, bc_right;
@@ -873,9 +875,17 @@ int orc_ul_step_m(const orc_consts* c, int method, uint32_t n, orc_soa* s, doubl
 int orc_tl_create(uint32_t n, const double* x, const double* y, const double* m, const double* rho,
                   const double* h, double h_search, int search, int contrib_only, uint32_t max_nbh,
                   orc_tl** out) {
+  return orc_tl_create_m(ORC_TL_RL, n, x, y, m, rho, h, h_search, search, contrib_only, max_nbh, out);
+}
+
+int orc_tl_create_m(int variant, uint32_t n, const double* x, const double* y, const double* m, const double* rho,
+                    const double* h, double h_search, int search, int contrib_only, uint32_t max_nbh,
+                    orc_tl** out) {
   if (!out || n == 0 || !x || !y || !m || !rho || !h) return ORC_ERR_ARG;
+  if (variant != ORC_TL_RL && variant != ORC_TL_COROTATIONAL && variant != ORC_TL_POTENTIAL) return ORC_ERR_ARG;
   orc_tl* t = new orc_tl();
   t->n = n;
+  t->variant = variant;
   t->rowp.assign((size_t)n + 1, 0);
   t->C.assign(n, 0.);
   t->B4.assign(4 * (size_t)n, 0.);
@@ -954,8 +964,18 @@ int orc_tl_create(uint32_t n, const double* x, const double* y, const double* m,
       return ORC_ERR_NO_NEIGHBOR;
     }
   }
-  // precomp_randles_libersky, src/precomp_shape_functions.h:231-290
   t->w.assign(3 * t->nbh.size(), 0.);
+  if (variant == ORC_TL_COROTATIONAL) {
+    // precomp_sph, src/precomp_shape_functions.h:210-229 (tensile_corot,
+    // src/benchmarks/tensile.cpp:197-198): the plain kernel at the set-up
+    // positions
+    for (uint32_t i = 0; i < n; i++)
+      for (uint64_t q = t->rowp[i]; q < t->rowp[i + 1]; q++) spline(x[i], y[i], x[t->nbh[q]], y[t->nbh[q]], h[i],
+                                                                    t->w.data() + 3 * q);
+    *out = t;
+    return ORC_OK;
+  }
+  // precomp_randles_libersky, src/precomp_shape_functions.h:231-290
   for (uint32_t i = 0; i < n; i++) {
     double xi = x[i], yi = y[i], hi = h[i];
     double C = 0.;
@@ -992,6 +1012,21 @@ int orc_tl_create(uint32_t n, const double* x, const double* y, const double* m,
       t->w[3 * q + 1] = (w[1] * invB.m[0][0] + w[2] * invB.m[1][0]) * 1. / C;
       t->w[3 * q + 2] = (w[1] * invB.m[0][1] + w[2] * invB.m[1][1]) * 1. / C;
     }
+  }
+  if (variant == ORC_TL_POTENTIAL) {
+    // precomp_randles_libersky_symmetric, src/precomp_shape_functions.h:292-316:
+    // wji of pair (i, j) is j's kernel of its first pair back to i.  A
+    // reverse pair dropped from the lists (q >= 2 with h_j) has a zero kernel.
+    t->wji.assign(3 * t->nbh.size(), 0.);
+    for (uint32_t i = 0; i < n; i++)
+      for (uint64_t q = t->rowp[i]; q < t->rowp[i + 1]; q++) {
+        uint32_t j = t->nbh[q];
+        for (uint64_t k = t->rowp[j]; k < t->rowp[j + 1]; k++)
+          if (t->nbh[k] == i) {
+            for (int c = 0; c < 3; c++) t->wji[3 * q + c] = t->w[3 * k + c];
+            break;
+          }
+      }
   }
   *out = t;
   return ORC_OK;
@@ -1060,11 +1095,18 @@ struct TlWork {
   }
 };
 
+int tl_stage_corot(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, TlWork& wk, int nt, double* d8,
+                   double* aux);
+int tl_stage_potential(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, TlWork& wk, int nt,
+                       double* d8, double* aux);
+
 int tl_stage(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, double time, TlWork& wk, int nt,
              double* d8, double* aux) {
   // src/body.h:110-123: reset, BCs, actions (src/benchmarks/tensile.cpp:276-286)
   for (size_t i = 0; i < 8 * (size_t)n; i++) d8[i] = 0.;
   tl_apply_bc(t, s, time, d8);
+  if (t->variant == ORC_TL_COROTATIONAL) return tl_stage_corot(c, t, n, s, wk, nt, d8, aux);
+  if (t->variant == ORC_TL_POTENTIAL) return tl_stage_potential(c, t, n, s, wk, nt, d8, aux);
   const uint64_t* R = t->rowp.data();
   const uint32_t* NB = t->nbh.data();
   const double* W = t->w.data();
@@ -1207,6 +1249,281 @@ int tl_stage(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, doubl
       a[13] = Pxy_x;
       a[14] = Pyx_y;
       a[15] = Pyy_y;
+    }
+  }
+  return ORC_OK;
+}
+
+// correctors_mghn_artificial_viscosity (src/correctors.h:83-137) of particle
+// i over its TL list, added to d[2], d[3].
+void tl_viscosity(const orc_consts* c, const orc_tl* t, const orc_soa* s, int64_t i, double* d) {
+  const uint64_t* R = t->rowp.data();
+  const uint32_t* NB = t->nbh.data();
+  const double* W = t->w.data();
+  const double K = K_of(c->E, c->nu);
+  const double alpha = c->alpha, beta = c->beta, eta = c->eta;
+  double xi = s->x[i], yi = s->y[i], vxi = s->vx[i], vyi = s->vy[i], hi = s->h[i], rhoi = s->rho[i];
+  double ci = sqrt(K / rhoi);
+  for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+    uint32_t j = NB[q];
+    const double* w = W + 3 * q;
+    double xj = s->x[j], yj = s->y[j], vxj = s->vx[j], vyj = s->vy[j], hj = s->h[j];
+    double mj = s->m[j], rhoj = s->rho[j];
+    double cj = sqrt(K / rhoj);
+    double vijx = vxi - vxj, vijy = vyi - vyj;
+    double xij = xi - xj, yij = yi - yj;
+    double vijposij = vijx * xij + vijy * yij;
+    if (vijposij >= 0.) continue;
+    double cij = 0.5 * (ci + cj);
+    double hij = 0.5 * (hi + hj);
+    double rhoij = 0.5 * (rhoi + rhoj);
+    double r2ij = xij * xij + yij * yij;
+    double muij = (hij * vijposij) / (r2ij + eta * eta * hij * hij);
+    double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+    d[2] += -mj * piij * w[1];
+    d[3] += -mj * piij * w[2];
+  }
+}
+
+// A co-rotated TL body, tensile_corot (src/benchmarks/tensile.cpp:137-211):
+// corotational_anisotropy_tensor (src/corotational.h:32-85),
+// derive_displacement_corotational (src/derivatives.h:202-258),
+// material_compute_stress_cauchy_green (src/material.h:102-130),
+// contmech_cauchy_to_nominal (src/cont_mech.h:154-169; p keeps its initial
+// value 0: no action of this list sets it), derive_stress_corotational
+// (src/derivatives.h:521-588), correctors_mghn_artificial_viscosity,
+// contmech_advection.  No continuity or stress rate: rho_t = S_t = 0, and the
+// stage's S is recomputed from H.
+// aux: Hxx, Hxy, Hyx, Hyy, Rxx, Rxy, Ryx, Ryy, Pxx, Pxy, Pyx, Pyy, Sxx, Sxy, Syy, 0.
+int tl_stage_corot(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, TlWork& wk, int nt, double* d8,
+                   double* aux) {
+  const uint64_t* R = t->rowp.data();
+  const uint32_t* NB = t->nbh.data();
+  const double* W = t->w.data();
+  const double* QW = t->qw.data();
+  const double E = c->E, nu = c->nu, rho0 = c->rho0;
+  // per particle: rotation (stored in Fxx..Fyy), H, S (in vx_x, vx_y, vy_y), P
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    double Xi = s->X[i], Yi = s->Y[i], xi = s->x[i], yi = s->y[i];
+    double Axx = 0., Axy = 0., Ayx = 0., Ayy = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* w = W + 3 * q;
+      double mj = s->m[j];
+      double xji = s->x[j] - xi, yji = s->y[j] - yi;
+      double Xji = s->X[j] - Xi, Yji = s->Y[j] - Yi;
+      Axx += xji * Xji * mj * w[0];
+      Axy += xji * Yji * mj * w[0];
+      Ayx += yji * Xji * mj * w[0];
+      Ayy += yji * Yji * mj * w[0];
+    }
+    double ax = Axx + Ayy;
+    double ay = Ayx - Axy;
+    double theta = atan2(ay, ax);
+    double Rxx = cos(theta), Rxy = -sin(theta), Ryx = sin(theta), Ryy = cos(theta);
+    wk.Fxx[i] = Rxx;
+    wk.Fxy[i] = Rxy;
+    wk.Fyx[i] = Ryx;
+    wk.Fyy[i] = Ryy;
+    // derive_displacement_corotational: R_i^-1 = R_i^T
+    double Rixx_inv = Rxx, Rixy_inv = Ryx, Riyx_inv = Rxy, Riyy_inv = Ryy;
+    double Hxx = 0., Hxy = 0., Hyx = 0., Hyy = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* w = W + 3 * q;
+      double quad_weight = QW[j];
+      double xji = s->x[j] - xi, yji = s->y[j] - yi;
+      double Xji = s->X[j] - Xi, Yji = s->Y[j] - Yi;
+      double uxji = Rixx_inv * xji + Rixy_inv * yji - Xji;
+      double uyji = Riyx_inv * xji + Riyy_inv * yji - Yji;
+      Hxx += uxji * w[1] * quad_weight;
+      Hxy += uxji * w[2] * quad_weight;
+      Hyx += uyji * w[1] * quad_weight;
+      Hyy += uyji * w[2] * quad_weight;
+    }
+    wk.Hxx[i] = Hxx;
+    wk.Hxy[i] = Hxy;
+    wk.Hyx[i] = Hyx;
+    wk.Hyy[i] = Hyy;
+    // material_compute_stress_cauchy_green
+    double D = E / (1 - nu * nu);
+    M2 F = mk(Hxx + 1., Hyx, Hxy, Hyy + 1.);
+    M2 FT = mk(F.m[0][0], F.m[1][0], F.m[0][1], F.m[1][1]);
+    M2 Ecg = sub(smul(0.5, add(F, FT)), mk(1., 0., 0., 1.));
+    double Sxx = D * (Ecg.m[0][0] + nu * Ecg.m[1][1]);
+    double Syy = D * (Ecg.m[0][0] * nu + Ecg.m[1][1]);
+    double Sxy = D * (1 - nu) * Ecg.m[0][1];
+    wk.vx_x[i] = Sxx;
+    wk.vx_y[i] = Sxy;
+    wk.vy_y[i] = Syy;
+    // contmech_cauchy_to_nominal with p = 0
+    double p = 0.;
+    M2 F2 = mk(Hxx + 1, Hyx, Hxy, Hyy + 1);
+    double J = det(F2);
+    M2 sigma = mk(Sxx - p, Sxy, Sxy, Syy - p);
+    M2 Pm = mul(smul(J, inv(F2)), sigma);
+    wk.Pxx[i] = Pm.m[0][0];
+    wk.Pxy[i] = Pm.m[1][0];
+    wk.Pyx[i] = Pm.m[0][1];
+    wk.Pyy[i] = Pm.m[1][1];
+  }
+  // derive_stress_corotational (assigns the accelerations, then adds
+  // f_c/m), viscosity, advection
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    double Pxxi = wk.Pxx[i], Pxyi = wk.Pxy[i], Pyxi = wk.Pyx[i], Pyyi = wk.Pyy[i];
+    double Rxxi = wk.Fxx[i], Rxyi = wk.Fxy[i], Ryxi = wk.Fyx[i], Ryyi = wk.Fyy[i];
+    double rhoi21 = 1. / (rho0 * rho0);
+    double vx_t = 0., vy_t = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* w = W + 3 * q;
+      double Pxxj = wk.Pxx[j], Pxyj = wk.Pxy[j], Pyxj = wk.Pyx[j], Pyyj = wk.Pyy[j];
+      double Rxxj = wk.Fxx[j], Rxyj = wk.Fxy[j], Ryxj = wk.Fyx[j], Ryyj = wk.Fyy[j];
+      double mj = s->m[j];
+      double rhoj21 = 1. / (rho0 * rho0);
+      double aijx = mj * ((Pxxi * rhoi21 + Pxxj * rhoj21) * w[1] + (Pxyi * rhoi21 + Pxyj * rhoj21) * w[2]);
+      double aijy = mj * ((Pyxi * rhoi21 + Pyxj * rhoj21) * w[1] + (Pyyi * rhoi21 + Pyyj * rhoj21) * w[2]);
+      double ajix = -aijx;
+      double ajiy = -aijy;
+      double aijxprime = aijx * Rxxi + aijy * Rxyi;
+      double aijyprime = aijx * Ryxi + aijy * Ryyi;
+      double ajixprime = ajix * Rxxj + ajiy * Rxyj;
+      double ajiyprime = ajix * Ryxj + ajiy * Ryyj;
+      vx_t += 0.5 * (aijxprime - ajixprime);
+      vy_t += 0.5 * (aijyprime - ajiyprime);
+    }
+    double* d = d8 + 8 * (size_t)i;
+    d[2] = vx_t;
+    d[3] = vy_t;
+    double fcx = s->fcx ? s->fcx[i] : 0., fcy = s->fcy ? s->fcy[i] : 0.;
+    d[2] += fcx / s->m[i];
+    d[3] += fcy / s->m[i];
+    tl_viscosity(c, t, s, i, d);
+    d[0] += s->vx[i];
+    d[1] += s->vy[i];
+    if (aux) {
+      double* a = aux + 16 * (size_t)i;
+      a[0] = wk.Hxx[i];
+      a[1] = wk.Hxy[i];
+      a[2] = wk.Hyx[i];
+      a[3] = wk.Hyy[i];
+      a[4] = Rxxi;
+      a[5] = Rxyi;
+      a[6] = Ryxi;
+      a[7] = Ryyi;
+      a[8] = Pxxi;
+      a[9] = Pxyi;
+      a[10] = Pyxi;
+      a[11] = Pyyi;
+      a[12] = wk.vx_x[i];
+      a[13] = wk.vx_y[i];
+      a[14] = wk.vy_y[i];
+      a[15] = 0.;
+    }
+  }
+  return ORC_OK;
+}
+
+// A potential-based TL body, tensile_potential (src/benchmarks/tensile.cpp:291-361):
+// derive_displacement (src/derivatives.h:101-149) with the Randles-Libersky
+// kernel, material_compute_stress_green_stvenant (src/material.h:132-161),
+// potential_comp_elastic_potential (src/potential.h:33-85) with the reverse
+// pairs' kernels, correctors_mghn_artificial_viscosity, contmech_advection.
+// No continuity or stress rate: rho_t = S_t = 0.
+// aux: Hxx, Hxy, Hyx, Hyy, Fe[0][0], Fe[1][0], Fe[0][1], Fe[1][1], 0, 0, 0, 0, Sxx, Sxy, Syy, 0
+// (Fe = -2 V J S, J = F).
+int tl_stage_potential(const orc_consts* c, const orc_tl* t, uint32_t n, orc_soa* s, TlWork& wk, int nt,
+                       double* d8, double* aux) {
+  const uint64_t* R = t->rowp.data();
+  const uint32_t* NB = t->nbh.data();
+  const double* W = t->w.data();
+  const double* WJI = t->wji.data();
+  const double* QW = t->qw.data();
+  const double E = c->E, nu = c->nu;
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    double uxi = s->x[i] - s->X[i], uyi = s->y[i] - s->Y[i];
+    double Hxx = 0., Hxy = 0., Hyx = 0., Hyy = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* w = W + 3 * q;
+      double uxj = s->x[j] - s->X[j], uyj = s->y[j] - s->Y[j];
+      double qw = QW[j];
+      Hxx += (uxj - uxi) * w[1] * qw;
+      Hxy += (uxj - uxi) * w[2] * qw;
+      Hyx += (uyj - uyi) * w[1] * qw;
+      Hyy += (uyj - uyi) * w[2] * qw;
+    }
+    wk.Hxx[i] = Hxx;
+    wk.Hxy[i] = Hxy;
+    wk.Hyx[i] = Hyx;
+    wk.Hyy[i] = Hyy;
+    // material_compute_stress_green_stvenant: C = F^T F, E = (C - I)/2
+    double D = E / (1 - nu * nu);
+    M2 F = mk(Hxx + 1., Hyx, Hxy, Hyy + 1.);
+    M2 FT = mk(F.m[0][0], F.m[1][0], F.m[0][1], F.m[1][1]);
+    M2 Cm = mul(FT, F);
+    M2 Esv = smul(0.5, sub(Cm, mk(1., 0., 0., 1.)));
+    double Sxx = D * (Esv.m[0][0] + nu * Esv.m[1][1]);
+    double Syy = D * (Esv.m[0][0] * nu + Esv.m[1][1]);
+    double Sxy = D * (1 - nu) * Esv.m[1][0];
+    wk.vx_x[i] = Sxx;
+    wk.vx_y[i] = Sxy;
+    wk.vy_y[i] = Syy;
+    // Fe = -2*voli*Ji*Si, evaluated ((-2*voli)*Ji)*Si
+    M2 Ji = mk(Hxx + 1., Hyx, Hxy, Hyy + 1.);
+    M2 Si = mk(Sxx, Sxy, Sxy, Syy);
+    double voli = QW[i];
+    M2 Fe = mul(smul(-2 * voli, Ji), Si);
+    wk.Pxx[i] = Fe.m[0][0];
+    wk.Pxy[i] = Fe.m[1][0];
+    wk.Pyx[i] = Fe.m[0][1];
+    wk.Pyy[i] = Fe.m[1][1];
+  }
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    M2 Fei = mk(wk.Pxx[i], wk.Pyx[i], wk.Pxy[i], wk.Pyy[i]);
+    double mi = s->m[i];
+    double vx_t = 0., vy_t = 0.;
+    for (uint64_t q = R[i]; q < R[i + 1]; q++) {
+      uint32_t j = NB[q];
+      const double* wij = W + 3 * q;
+      const double* wji = WJI + 3 * q;
+      // fij = Fei * grad_wij: (M v)[r] = M[0][r] v.x + M[1][r] v.y
+      double fij0 = Fei.m[0][0] * wij[1] + Fei.m[1][0] * wij[2];
+      double fij1 = Fei.m[0][1] * wij[1] + Fei.m[1][1] * wij[2];
+      M2 Fej = mk(wk.Pxx[j], wk.Pyx[j], wk.Pxy[j], wk.Pyy[j]);
+      double fji0 = Fej.m[0][0] * wji[1] + Fej.m[1][0] * wji[2];
+      double fji1 = Fej.m[0][1] * wji[1] + Fej.m[1][1] * wji[2];
+      vx_t += 0.5 * (fji0 - fij0) / mi;
+      vy_t += 0.5 * (fji1 - fij1) / mi;
+    }
+    double* d = d8 + 8 * (size_t)i;
+    d[2] = vx_t;
+    d[3] = vy_t;
+    double fcx = s->fcx ? s->fcx[i] : 0., fcy = s->fcy ? s->fcy[i] : 0.;
+    d[2] += fcx / mi;
+    d[3] += fcy / mi;
+    tl_viscosity(c, t, s, i, d);
+    d[0] += s->vx[i];
+    d[1] += s->vy[i];
+    if (aux) {
+      double* a = aux + 16 * (size_t)i;
+      a[0] = wk.Hxx[i];
+      a[1] = wk.Hxy[i];
+      a[2] = wk.Hyx[i];
+      a[3] = wk.Hyy[i];
+      a[4] = wk.Pxx[i];
+      a[5] = wk.Pxy[i];
+      a[6] = wk.Pyx[i];
+      a[7] = wk.Pyy[i];
+      a[8] = a[9] = a[10] = a[11] = 0.;
+      a[12] = wk.vx_x[i];
+      a[13] = wk.vx_y[i];
+      a[14] = wk.vy_y[i];
+      a[15] = 0.;
     }
   }
   return ORC_OK;

@@ -145,6 +145,27 @@ typedef struct orc_tl orc_tl;
 int orc_tl_create(uint32_t n, const double* x, const double* y, const double* m,
                   const double* rho, const double* h, double h_search, int search,
                   int contrib_only, uint32_t max_nbh, orc_tl** out);
+/* The same for a chosen TL action list:
+ * ORC_TL_RL            tensile_tl (src/benchmarks/tensile.cpp:213-289);
+ * ORC_TL_COROTATIONAL  tensile_corot (src/benchmarks/tensile.cpp:137-211): the plain
+ *                      kernel (precomp_sph), co-rotated displacement gradient,
+ *                      Cauchy-Green stress, nominal stress, co-rotated stress
+ *                      divergence, viscosity, advection;
+ * ORC_TL_POTENTIAL     tensile_potential (src/benchmarks/tensile.cpp:291-361): the
+ *                      Randles-Libersky kernel and the reverse pairs' kernels
+ *                      (precomp_randles_libersky_symmetric), Green-St Venant stress,
+ *                      elastic-potential forces, viscosity, advection.
+ * Derivatives of the variants (orc_tl_derivatives) carry rho_t = S_t = 0 and
+ * the aux layouts:
+ *   co-rotated: Hxx, Hxy, Hyx, Hyy, Rxx, Rxy, Ryx, Ryy, Pxx, Pxy, Pyx, Pyy, Sxx, Sxy, Syy, 0;
+ *   potential:  Hxx, Hxy, Hyx, Hyy, Fe00, Fe10, Fe01, Fe11, 0, 0, 0, 0, Sxx, Sxy, Syy, 0
+ *               (Fe = -2 V J S). */
+#define ORC_TL_RL 0
+#define ORC_TL_COROTATIONAL 1
+#define ORC_TL_POTENTIAL 2
+int orc_tl_create_m(int variant, uint32_t n, const double* x, const double* y, const double* m,
+                    const double* rho, const double* h, double h_search, int search,
+                    int contrib_only, uint32_t max_nbh, orc_tl** out);
 void orc_tl_destroy(orc_tl* t);
 uint64_t orc_tl_num_pairs(const orc_tl* t);
 /* row_ptr[n+1], col[npairs], w[3*npairs] (w, w_x, w_y corrected). */

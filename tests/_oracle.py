@@ -65,6 +65,8 @@ def lib():
                                     C.c_uint32, C.c_int, C.c_uint32, C.c_int]
         L.orc_tl_create.argtypes = [C.c_uint32, dptr, dptr, dptr, dptr, dptr, C.c_double, C.c_int, C.c_int,
                                     C.c_uint32, C.POINTER(C.c_void_p)]
+        L.orc_tl_create_m.argtypes = [C.c_int, C.c_uint32, dptr, dptr, dptr, dptr, dptr, C.c_double, C.c_int,
+                                      C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]
         L.orc_tl_destroy.argtypes = [C.c_void_p]
         L.orc_tl_num_pairs.argtypes = [C.c_void_p]
         L.orc_tl_num_pairs.restype = C.c_uint64
@@ -206,14 +208,22 @@ def ul_step(ps, params, nsteps, order=ORDER_STABLE, threads=1, max_nbh=200, dt=N
     return rc, q
 
 
-class TL:
-    """Frozen TL neighbour lists + Randles-Libersky kernels of the oracle."""
+TL_RL = 0
+TL_COROTATIONAL = 1
+TL_POTENTIAL = 2
 
-    def __init__(self, ps, h_search, search=1, contrib_only=1, max_nbh=1 << 30):
+
+class TL:
+    """Frozen TL neighbour lists + kernels of the oracle: Randles-Libersky
+    (tensile_tl, variant TL_RL), the plain kernel of a co-rotated body
+    (TL_COROTATIONAL) or Randles-Libersky with the reverse pairs' kernels of
+    a potential body (TL_POTENTIAL)."""
+
+    def __init__(self, ps, h_search, search=1, contrib_only=1, max_nbh=1 << 30, variant=TL_RL):
         self.n = ps.n
         h = C.c_void_p()
-        rc = lib().orc_tl_create(ps.n, _p(ps.x), _p(ps.y), _p(ps.m), _p(ps.rho), _p(ps.h), h_search, search,
-                                 contrib_only, max_nbh, C.byref(h))
+        rc = lib().orc_tl_create_m(variant, ps.n, _p(ps.x), _p(ps.y), _p(ps.m), _p(ps.rho), _p(ps.h), h_search,
+                                   search, contrib_only, max_nbh, C.byref(h))
         self.rc = rc
         self.h = h if rc == OK else None
 
@@ -245,12 +255,12 @@ class TL:
         return lib().orc_tl_set_bc(self.h, left.ctypes.data_as(u32p), len(left), right.ctypes.data_as(u32p),
                                    len(right), v_bc)
 
-    def derivatives(self, ps, params, time=0.0):
+    def derivatives(self, ps, params, time=0.0, fc=None):
         q = ps.copy()
         d8 = np.zeros((self.n, 8))
         aux = np.zeros((self.n, 16))
         c = consts_from(params)
-        s = soa_from(q)
+        s = soa_from(q, fc)
         rc = lib().orc_tl_derivatives(C.byref(c), self.h, self.n, C.byref(s), time, _p(d8), _p(aux))
         return rc, d8, aux, q
 
