@@ -7,6 +7,8 @@
 //   Gray-Monaghan UL  (src/benchmarks/rings.cpp:86-96)   -> particle_monaghan
 //   Total-Lagrangian  (src/benchmarks/tensile.cpp:276-286) -> particle_tl
 //   Godunov UL        (src/benchmarks/rings.cpp:151-158)  -> particle_ul (FP64)
+//   co-rotated TL     (src/benchmarks/tensile.cpp:202-208) -> particle_corotational (FP64)
+//   potential TL      (src/benchmarks/tensile.cpp:354-358) -> particle_potential (FP64)
 // and the body is moved to the GPU (SoA, idx order).  Any other action
 // list, particle type or boundary function is rejected with a message
 // (MFREE_ERR_PIPELINE): there is no CPU execution of the actions here.
@@ -120,7 +122,7 @@ class body {
     const int32_t method = pipeline();
     if (method < 0) return "the action list is not one of the supported pipelines (MFREE_ERR_PIPELINE)";
     if (!m_bc.list().empty()) {
-      if (method != MFREE_METHOD_TL_RL) return "boundary conditions on a UL body are not supported";
+      if (!tl(method)) return "boundary conditions on a UL body are not supported";
       double speed = -1.;
       for (const auto& e : m_bc.list()) {
         double a = 0.;
@@ -143,6 +145,11 @@ class body {
   boundary<T>* m_boundary = nullptr;
   std::shared_ptr<mfree_adapter::Gpu> m_gpu;
   std::shared_ptr<bool> m_stale = std::make_shared<bool>(false);
+
+  static bool tl(int32_t method) {
+    return method == MFREE_METHOD_TL_RL || method == MFREE_METHOD_TL_COROTATIONAL ||
+           method == MFREE_METHOD_TL_POTENTIAL;
+  }
 
   int32_t pipeline() const {
     typedef void (*A)(body<T>&);
@@ -176,9 +183,25 @@ class body {
                                &godunov_continuity<T>,
                                &godunov_momentum<T>,
                                &contmech_advection<T>};
+    // src/benchmarks/tensile.cpp:202-208
+    const std::vector<A> corot = {&corotational_anisotropy_tensor<T>,
+                                  &derive_displacement_corotational<T>,
+                                  &material_compute_stress_cauchy_green<T>,
+                                  &contmech_cauchy_to_nominal<T>,
+                                  &derive_stress_corotational<T>,
+                                  &correctors_mghn_artificial_viscosity<T>,
+                                  &contmech_advection<T>};
+    // src/benchmarks/tensile.cpp:354-358
+    const std::vector<A> pot = {&derive_displacement<T>,
+                                &material_compute_stress_green_stvenant<T>,
+                                &potential_comp_elastic_potential<T>,
+                                &correctors_mghn_artificial_viscosity<T>,
+                                &contmech_advection<T>};
     if (m_actions == ul && std::is_base_of<particle_monaghan, T>::value) return MFREE_METHOD_UL_MONAGHAN;
     if (m_actions == tl && std::is_base_of<particle_tl, T>::value) return MFREE_METHOD_TL_RL;
     if (m_actions == gd && std::is_base_of<particle_ul, T>::value) return MFREE_METHOD_UL_GODUNOV;
+    if (m_actions == corot && std::is_base_of<particle_corotational, T>::value) return MFREE_METHOD_TL_COROTATIONAL;
+    if (m_actions == pot && std::is_base_of<particle_potential, T>::value) return MFREE_METHOD_TL_POTENTIAL;
     return -1;
   }
 
@@ -213,12 +236,24 @@ class body {
     mfree_options opt;
     mfree_default_options(&opt);
     opt.method = method;
-    // the Godunov pair sums run in FP64 only
-    opt.precision = method == MFREE_METHOD_UL_GODUNOV ? MFREE_PRECISION_FP64 : mfree_adapter::precision();
-    if (method == MFREE_METHOD_TL_RL) {
+    // the Godunov, co-rotated and potential lists run in FP64 only
+    opt.precision = (method == MFREE_METHOD_UL_MONAGHAN || method == MFREE_METHOD_TL_RL) ? mfree_adapter::precision()
+                                                                                       : MFREE_PRECISION_FP64;
+    if (tl(method)) {
+      // the set-up the driver called on the particles before the body
+      // existed (find_neighbors + the list's kernel precomputation)
       auto it = mfree_adapter::tl_requests().find((const void*)m_particles);
-      if (it == mfree_adapter::tl_requests().end() || !it->second.rl)
-        mfree_adapter::die("TL body: find_neighbors and precomp_randles_libersky must be called on its particles");
+      const bool found = it != mfree_adapter::tl_requests().end();
+      const bool ok = found && (method == MFREE_METHOD_TL_RL            ? it->second.rl
+                                : method == MFREE_METHOD_TL_COROTATIONAL ? it->second.sph
+                                                                         : it->second.rl_sym);
+      if (!ok)
+        mfree_adapter::die(method == MFREE_METHOD_TL_RL
+                               ? "TL body: find_neighbors and precomp_randles_libersky must be called on its particles"
+                           : method == MFREE_METHOD_TL_COROTATIONAL
+                               ? "co-rotated body: find_neighbors and precomp_sph must be called on its particles"
+                               : "potential body: find_neighbors and precomp_randles_libersky_symmetric must be "
+                                 "called on its particles");
       opt.tl_h_search = it->second.h;
     }
     auto gpu = std::make_shared<mfree_adapter::Gpu>();
@@ -265,7 +300,7 @@ class body {
     if (mfree_add_body(gpu->ctx, &prm, &hp, n, &bid) != MFREE_OK)
       mfree_adapter::die(std::string("add_body: ") + mfree_last_error(gpu->ctx));
     if (!m_bc.list().empty()) {
-      if (method != MFREE_METHOD_TL_RL) mfree_adapter::die("boundary conditions on a UL body are not supported");
+      if (!tl(method)) mfree_adapter::die("boundary conditions on a UL body are not supported");
       std::vector<uint32_t> left, right;
       double speed = -1.;
       for (const auto& e : m_bc.list()) {
@@ -279,7 +314,7 @@ class body {
                              (uint32_t)right.size(), speed) != MFREE_OK)
         mfree_adapter::die(std::string("set_strip_bc: ") + mfree_last_error(gpu->ctx));
     }
-    if (method == MFREE_METHOD_TL_RL && mfree_tl_precompute(gpu->ctx) != MFREE_OK)
+    if (tl(method) && mfree_tl_precompute(gpu->ctx) != MFREE_OK)
       mfree_adapter::die(std::string("TL precompute: ") + mfree_last_error(gpu->ctx));
     m_gpu = gpu;
   }

@@ -1,9 +1,11 @@
 // src/precomp_shape_functions.h of the reference: the kernel precomputations
 // drivers call on a particle array before the body exists.  They are
-// recorded here and carried out on the GPU by the body's first step
-// (precomp_randles_libersky, src/precomp_shape_functions.h:231-290, for the
-// Total-Lagrangian pipeline); the variants of the other methods are accepted
-// and left to those methods' bodies, which step() refuses.
+// recorded here and carried out on the GPU by the body's first step:
+// precomp_randles_libersky (src/precomp_shape_functions.h:231-290) for the
+// Total-Lagrangian pipeline, precomp_sph (:210-229) for co-rotated bodies and
+// precomp_randles_libersky_symmetric (:292-316) for potential bodies; the
+// RKPM variants are accepted and left to the weak-form bodies, which step()
+// refuses.
 #ifndef MFREE_PRECOMP_SHAPE_FUNCTIONS_H_
 #define MFREE_PRECOMP_SHAPE_FUNCTIONS_H_
 
@@ -16,7 +18,9 @@ namespace mfree_adapter {
 // particle array.
 struct TlRequest {
   double h = 0.;
-  bool rl = false;
+  bool rl = false;      // precomp_randles_libersky
+  bool sph = false;     // precomp_sph
+  bool rl_sym = false;  // precomp_randles_libersky_symmetric
 };
 inline std::map<const void*, TlRequest>& tl_requests() {
   static std::map<const void*, TlRequest> r;
@@ -32,18 +36,21 @@ void precomp_randles_libersky(T** particles, unsigned int np) {
   mfree_adapter::tl_requests()[(const void*)particles].rl = true;
 }
 
-// src/precomp_shape_functions.h:210-229 (co-rotated bodies),
-// :292-... (potential bodies), :175-208 (RKPM, weak forms).
+// src/precomp_shape_functions.h:210-229 (co-rotated bodies; UL bodies
+// recompute it every stage): recorded, executed on the GPU.
 template <typename T>
 void precomp_sph(T** particles, unsigned int n) {
-  (void)particles;
   (void)n;
+  mfree_adapter::tl_requests()[(const void*)particles].sph = true;
 }
+// src/precomp_shape_functions.h:292-316 (potential bodies): recorded,
+// executed on the GPU.
 template <typename T>
 void precomp_randles_libersky_symmetric(T** particles, unsigned int np) {
-  (void)particles;
   (void)np;
+  mfree_adapter::tl_requests()[(const void*)particles].rl_sym = true;
 }
+// :175-208 (RKPM, weak forms).
 template <typename T>
 void precomp_rkpm(T** particles, unsigned int n) {
   (void)particles;

@@ -64,6 +64,16 @@ typedef int32_t mfree_status;
 #define MFREE_METHOD_UL_GODUNOV 2  /* particle_ul, rings_godunov action list (src/benchmarks/rings.cpp:151-158):
                                       Parshikov's Godunov-type continuity and momentum (src/godunov.h:41-189);
                                       FP64 fused pipeline only */
+#define MFREE_METHOD_TL_COROTATIONAL 3 /* particle_corotational, tensile_corot action list
+                                          (src/benchmarks/tensile.cpp:202-208): co-rotated displacement
+                                          gradient, Cauchy-Green stress, co-rotated stress divergence
+                                          (src/corotational.h:32-85, src/derivatives.h:202-258,521-588);
+                                          plain kernel frozen at set-up (precomp_sph); FP64 only */
+#define MFREE_METHOD_TL_POTENTIAL 4    /* particle_potential, tensile_potential action list
+                                          (src/benchmarks/tensile.cpp:354-358): Green-St Venant stress,
+                                          elastic-potential forces (src/potential.h:33-85); Randles-Libersky
+                                          kernel of both pair directions frozen at set-up
+                                          (precomp_randles_libersky_symmetric); FP64 only */
 
 /* Pipelines of a UL FP64 context. */
 #define MFREE_PIPELINE_FUSED 0     /* fused per-stage kernels (default) */
@@ -242,7 +252,12 @@ mfree_status mfree_step(mfree_ctx* ctx, uint32_t nsteps);
  * UL: quad_weight, p, Rxx, Rxy, Ryy, Sxx_x, Sxy_y, Sxy_x, Syy_y, vx_x, vx_y,
  *     vy_x, vy_y, num_nbh, 0, 0
  * TL: Hxx, Hxy, Hyx, Hyy, Fdotxx, Fdotxy, Fdotyx, Fdotyy, Pxx, Pxy, Pyx, Pyy,
- *     Pxx_x, Pxy_x, Pyx_y, Pyy_y */
+ *     Pxx_x, Pxy_x, Pyx_y, Pyy_y
+ * TL co-rotated: Hxx, Hxy, Hyx, Hyy, Rxx, Rxy, Ryx, Ryy, Pxx, Pxy, Pyx, Pyy, Sxx, Sxy, Syy, 0
+ * TL potential:  Hxx, Hxy, Hyx, Hyy, Fe00, Fe10, Fe01, Fe11 (Fe = -2 V F S, glm [column][row]),
+ *                0, 0, 0, 0, Sxx, Sxy, Syy, 0
+ * (the stage's stresses of the co-rotated and potential lists, recomputed from H; those lists do
+ * not integrate S or rho, so the state's S and rho keep their initial values) */
 mfree_status mfree_eval_derivatives(mfree_ctx* ctx, double* d8, double* aux);
 
 /* The Verlet lists the reference would build for a body's current state

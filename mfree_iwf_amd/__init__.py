@@ -4,7 +4,8 @@ pair sums, Randles-Libersky corrected Total-Lagrangian SPH and RK4 -- behind
 the C ABI of include/mfree_iwf_amd.h.
 """
 from . import _capi  # noqa: F401
-from ._capi import (METHOD_TL_RL, METHOD_UL_GODUNOV, METHOD_UL_MONAGHAN, PRECISION_FP64, PRECISION_MIXED, MfreeError,  # noqa: F401
+from ._capi import (METHOD_TL_COROTATIONAL, METHOD_TL_POTENTIAL, METHOD_TL_RL, METHOD_UL_GODUNOV,  # noqa: F401
+                    METHOD_UL_MONAGHAN, PRECISION_FP64, PRECISION_MIXED, MfreeError,  # noqa: F401
                     Params)
 from .scenes import ParticleSet, cubic_spline, make_plate, make_rings  # noqa: F401
 from .solver import Solver, device_count  # noqa: F401

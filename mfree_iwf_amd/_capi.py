@@ -26,6 +26,8 @@ ERR_SKIN = -12
 METHOD_UL_MONAGHAN = 0
 METHOD_TL_RL = 1
 METHOD_UL_GODUNOV = 2
+METHOD_TL_COROTATIONAL = 3
+METHOD_TL_POTENTIAL = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PIPELINE_FUSED = 0

@@ -34,6 +34,8 @@ void scene_consts(const mfree_params& p, uint32_t offset, uint32_t n, SceneConst
   double c0 = sqrt(K / p.rho0);
   memset(&s, 0, sizeof(s));
   s.rho0 = p.rho0;
+  s.E = p.E;
+  s.nu = p.nu;
   s.K = K;
   s.G2 = 2 * G;
   s.c0sq = c0 * c0;
@@ -124,7 +126,7 @@ mfree_status commit(mfree_ctx* c) {
     return fail(c, MFREE_ERR_HIP, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t N = c->N;
-  const bool tl = c->opt.method == MFREE_METHOD_TL_RL;
+  const bool tl = tl_method(c->opt.method);
   for (int b = 0; b < 2; b++) {
     if (b == 1 && tl) break;  // TL storage is never reordered
     HIPCHK(c, alloc8(c, c->y[b], N));
@@ -210,7 +212,8 @@ mfree_status commit(mfree_ctx* c) {
   c->tile_grid = (uint32_t)std::max(8, prop.multiProcessorCount) * 2;
   c->tile_grid = (c->tile_grid + kXcds - 1) / kXcds * kXcds;
   if (tl) {
-    mfree_status st = tl_alloc(c->tl, (uint32_t)N, c->hX.data(), c->hY.data(), c->bc_left, c->bc_right, c->off);
+    mfree_status st = tl_alloc(c->tl, tl_variant(c->opt.method), (uint32_t)N, c->hX.data(), c->hY.data(),
+                               c->bc_left, c->bc_right, c->off);
     if (st != MFREE_OK) return fail(c, st, "TL allocation failed");
   }
   if (c->opt.pipeline == MFREE_PIPELINE_REFERENCE) {
@@ -260,7 +263,7 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.cell_start = c->cell_start;
   a.perm = c->perm;
   a.err = c->errb;
-  a.berr = c->opt.method == MFREE_METHOD_TL_RL ? nullptr : c->berr;
+  a.berr = tl_method(c->opt.method) ? nullptr : c->berr;
   a.tile_cells = c->tile_cells;
   a.tile_off = c->tile_cells ? c->tile_off : nullptr;
   return a;
@@ -572,7 +575,7 @@ void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>
 }
 
 mfree_status run_steps(mfree_ctx* c, uint32_t nsteps) {
-  const bool tl = c->opt.method == MFREE_METHOD_TL_RL;
+  const bool tl = tl_method(c->opt.method);
   if (tl) {
     mfree_status st = tl_ensure_precomputed(c->tl, tl_shared(c));
     if (st != MFREE_OK) return fail(c, st, "TL precompute failed: " + c->tl.msg);
@@ -754,10 +757,16 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     c->opt = *opt;
   else
     mfree_default_options(&c->opt);
-  if (c->opt.method != MFREE_METHOD_UL_MONAGHAN && c->opt.method != MFREE_METHOD_TL_RL &&
-      c->opt.method != MFREE_METHOD_UL_GODUNOV) {
+  if (c->opt.method != MFREE_METHOD_UL_MONAGHAN && c->opt.method != MFREE_METHOD_UL_GODUNOV &&
+      !tl_method(c->opt.method)) {
     delete c;
     return MFREE_ERR_ARG;
+  }
+  // the co-rotated and potential TL lists exist in FP64
+  if ((c->opt.method == MFREE_METHOD_TL_COROTATIONAL || c->opt.method == MFREE_METHOD_TL_POTENTIAL) &&
+      c->opt.precision != MFREE_PRECISION_FP64) {
+    delete c;
+    return MFREE_ERR_UNSUPPORTED;
   }
   // the Godunov pair sums exist in FP64 on the fused pipeline
   if (c->opt.method == MFREE_METHOD_UL_GODUNOV &&
@@ -794,7 +803,7 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     delete c;
     return MFREE_ERR_ARG;
   }
-  if (c->opt.isolate_bodies && (c->opt.method == MFREE_METHOD_TL_RL || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
+  if (c->opt.isolate_bodies && (tl_method(c->opt.method) || c->opt.pipeline != MFREE_PIPELINE_FUSED)) {
     delete c;
     return MFREE_ERR_UNSUPPORTED;
   }
@@ -823,7 +832,7 @@ mfree_status mfree_add_body(mfree_ctx* c, const mfree_params* p, const mfree_par
                             uint32_t* body_id) {
   if (!c || !p) return MFREE_ERR_ARG;
   if (c->committed) return fail(c, MFREE_ERR_ARG, "bodies must be added before the first compute call");
-  const bool tl = c->opt.method == MFREE_METHOD_TL_RL;
+  const bool tl = tl_method(c->opt.method);
   if (!valid_particles(host, tl)) return fail(c, MFREE_ERR_ARG, "missing particle arrays");
   if (n < 2) return fail(c, MFREE_ERR_ARG, "a body needs at least 2 particles");
   // the mixed pair kernel packs sorted positions into 28 bits
@@ -861,7 +870,7 @@ mfree_status mfree_add_body(mfree_ctx* c, const mfree_params* p, const mfree_par
 mfree_status mfree_set_strip_bc(mfree_ctx* c, uint32_t body, const uint32_t* idx_left, uint32_t nl,
                                 const uint32_t* idx_right, uint32_t nr, double v_bc) {
   if (!c || body >= c->prm.size()) return MFREE_ERR_ARG;
-  if (c->opt.method != MFREE_METHOD_TL_RL) return fail(c, MFREE_ERR_UNSUPPORTED, "strip BCs are TL only");
+  if (!tl_method(c->opt.method)) return fail(c, MFREE_ERR_UNSUPPORTED, "strip BCs are TL only");
   if (c->committed) return fail(c, MFREE_ERR_ARG, "BCs must be set before the first compute call");
   uint32_t n = c->off[body + 1] - c->off[body];
   for (uint32_t i = 0; i < nl; i++)
@@ -876,7 +885,7 @@ mfree_status mfree_set_strip_bc(mfree_ctx* c, uint32_t body, const uint32_t* idx
 
 mfree_status mfree_tl_precompute(mfree_ctx* c) {
   if (!c) return MFREE_ERR_ARG;
-  if (c->opt.method != MFREE_METHOD_TL_RL) return fail(c, MFREE_ERR_UNSUPPORTED, "not a TL context");
+  if (!tl_method(c->opt.method)) return fail(c, MFREE_ERR_UNSUPPORTED, "not a TL context");
   mfree_status st = commit(c);
   if (st) return st;
   st = tl_ensure_precomputed(c->tl, tl_shared(c));
@@ -886,7 +895,7 @@ mfree_status mfree_tl_precompute(mfree_ctx* c) {
 
 mfree_status mfree_set_state(mfree_ctx* c, uint32_t body, const mfree_particles* host) {
   if (!c || body >= c->prm.size()) return MFREE_ERR_ARG;
-  const bool tl = c->opt.method == MFREE_METHOD_TL_RL;
+  const bool tl = tl_method(c->opt.method);
   if (!valid_particles(host, false)) return fail(c, MFREE_ERR_ARG, "missing particle arrays");
   mfree_status st = commit(c);
   if (st) return st;
@@ -981,7 +990,7 @@ mfree_status mfree_eval_derivatives(mfree_ctx* c, double* d8, double* aux) {
     hipFree(dd);
     return fail(c, MFREE_ERR_HIP, "hipMalloc failed");
   }
-  if (c->opt.method == MFREE_METHOD_TL_RL) {
+  if (tl_method(c->opt.method)) {
     st = tl_ensure_precomputed(c->tl, tl_shared(c));
     if (st == MFREE_OK)
       tl_enqueue_eval(c->tl, tl_shared(c), dd, da);
@@ -1015,7 +1024,7 @@ mfree_status mfree_get_pairs(mfree_ctx* c, uint32_t body, uint32_t* row_ptr, uin
   if (st) return st;
   HIPCHK(c, hipSetDevice(c->opt.device));
   const uint32_t o = c->off[body], n = c->off[body + 1] - o;
-  if (c->opt.method == MFREE_METHOD_TL_RL) {
+  if (tl_method(c->opt.method)) {
     st = tl_ensure_precomputed(c->tl, tl_shared(c));
     if (st != MFREE_OK) return fail(c, st, "TL precompute failed: " + c->tl.msg);
     uint64_t tot = 0;

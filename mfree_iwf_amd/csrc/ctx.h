@@ -102,6 +102,14 @@ struct mfree_ctx {
 
 namespace mfree {
 
+// Total-Lagrangian methods (frozen neighbour lists, tl.hip) and their action list
+inline bool tl_method(int32_t m) {
+  return m == MFREE_METHOD_TL_RL || m == MFREE_METHOD_TL_COROTATIONAL || m == MFREE_METHOD_TL_POTENTIAL;
+}
+inline int tl_variant(int32_t m) {
+  return m == MFREE_METHOD_TL_COROTATIONAL ? kTlCorot : m == MFREE_METHOD_TL_POTENTIAL ? kTlPotential : kTlRl;
+}
+
 inline mfree_status fail(mfree_ctx* c, mfree_status s, const std::string& msg) {
   if (c) c->err = msg;
   return s;

@@ -78,6 +78,7 @@ __device__ inline void raise_err(uint32_t* err, uint32_t* berr, uint32_t body, u
 // the host with the reference's formulas (src/simulation_data.cpp:32-47).
 struct SceneConst {
   double rho0, K, G2, c0sq;        // G2 = 2*G, c0sq = c0*c0 with c0 = sqrt(K/rho0)
+  double E, nu;                    // the stress models of the co-rotated and potential TL lists
   double wdeltap, corr_exp, mghn_eps;
   double alpha, beta, eta, xsph_eps;
   double dt, half_dt, dt6;         // dt6 = dt*1./6.

@@ -32,8 +32,14 @@ struct TlShared {
   double h_search;  // <= 0: per-particle h
 };
 
+// Action lists of TL bodies: tensile_tl (Randles-Libersky), tensile_corot
+// (co-rotated, plain kernel) and tensile_potential (elastic potential,
+// Randles-Libersky kernel of both pair directions).
+enum TlVariant : int { kTlRl = 0, kTlCorot = 1, kTlPotential = 2 };
+
 struct TlState {
   bool ready = false;
+  int variant = kTlRl;
   std::string msg;
   uint32_t N = 0;
   double *X = nullptr, *Y = nullptr;  // reference configuration
@@ -46,7 +52,10 @@ struct TlState {
   double* qw = nullptr;               // quadrature weight m/rho at precompute
   double* rec = nullptr;              // pass-1 records for pass 2 (96 B per particle)
   double* part8 = nullptr;            // pass-1 derivative parts (4N: rho_t, S_t)
-  double* diag = nullptr;             // H, Fdot, P per particle (12N), for diagnostics
+  double* diag = nullptr;             // per-particle diagnostics (16N): H, Fdot, P (RL); H, R, P, S
+                                      // (co-rotated); H, Fe, -, S (potential)
+  double2* rot = nullptr;             // co-rotated: (cos t, sin t) of each particle's rotation
+  uint32_t* rev = nullptr;            // potential: index of the reverse pair (j, i) of pair q, or ~0u
   Fields8 Kb{};                       // second stage buffer: pass 1 reads the
                                       // stage state, pass 2 writes the next one
   std::vector<void*> allocs;
@@ -57,7 +66,7 @@ struct TlState {
 using EventPairs = std::vector<std::pair<hipEvent_t, hipEvent_t>>;
 using TakeEvent = std::function<hipEvent_t()>;
 
-mfree_status tl_alloc(TlState& t, uint32_t N, const double* X, const double* Y,
+mfree_status tl_alloc(TlState& t, int variant, uint32_t N, const double* X, const double* Y,
                       const std::vector<std::vector<uint32_t>>& left,
                       const std::vector<std::vector<uint32_t>>& right, const std::vector<uint32_t>& off);
 void tl_free(TlState& t);

@@ -16,6 +16,20 @@
 //     (src/correctors.h:83-137), momentum (src/cont_mech.h:70-87), advection,
 //     RK4 stage update; strip boundary conditions (src/benchmarks/tensile.cpp:20-42)
 //     act on the stage copies and after the combine, at the step-start time.
+// The same two passes carry the other TL action lists (FP64):
+//   co-rotated (tensile_corot, src/benchmarks/tensile.cpp:137-211): set-up
+//     precomp_sph (plain kernel); pass 1 corotational_anisotropy_tensor
+//     (src/corotational.h:32-85), derive_displacement_corotational
+//     (src/derivatives.h:202-258), Cauchy-Green stress (src/material.h:102-130),
+//     P = J F^-1 sigma with p = 0 (no action of the list sets p); pass 2
+//     derive_stress_corotational (src/derivatives.h:521-588), then viscosity;
+//   potential (tensile_potential, src/benchmarks/tensile.cpp:291-361): set-up
+//     precomp_randles_libersky_symmetric (the reverse pair's kernel, found
+//     by index); pass 1 derive_displacement, Green-St Venant stress
+//     (src/material.h:132-161), Fe = -2 V F S; pass 2
+//     potential_comp_elastic_potential (src/potential.h:33-85), then viscosity.
+//   Neither list integrates rho or S (rho_t = S_t = 0); the stage's S is
+//   recomputed from H.
 // Storage of TL bodies stays in idx order (frozen lists index it directly).
 #include <math.h>
 #include <string.h>
@@ -43,6 +57,8 @@ struct TlArgs {
   Fields8 src;
   void* rec;  // TlRec64 / TlRec32 per particle, written by pass 1
   double *part8, *diag;
+  double2* rot;         // co-rotated: (cos t, sin t) per particle, written by pass 1
+  const uint32_t* rev;  // potential: reverse pair of each pair (~0u: none)
   int stage;
   Fields8 y, K, acc;
   double* d8;
@@ -131,7 +147,147 @@ __device__ inline void load_w(const TlArgs& a, uint32_t q, typename TlTypes<MIXE
   }
 }
 
-template <bool MIXED>
+// Pass 1 of a co-rotated body (FP64): rotation, co-rotated displacement
+// gradient, Cauchy-Green stress, nominal stress; writes the pass-2 record
+// (P), the rotation and the diagnostics.
+__device__ inline void tl_pass1_corot(const TlArgs& a, const SceneConst* __restrict__ c, uint32_t i, double t,
+                                      double xi, double yi, double vxi, double vyi) {
+  const double Xi = a.X[i], Yi = a.Y[i];
+  // corotational_anisotropy_tensor
+  double Axx = 0., Axy = 0., Ayx = 0., Ayy = 0.;
+#pragma unroll 4
+  for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+    const uint32_t j = a.col[q];
+    const double w0 = a.w64[3 * (size_t)q];
+    double xj, yj, vxj, vyj;
+    tl_xv(a, j, t, xj, yj, vxj, vyj);
+    const double mj = a.m[j];
+    const double xji = xj - xi, yji = yj - yi;
+    const double Xji = a.X[j] - Xi, Yji = a.Y[j] - Yi;
+    Axx += xji * Xji * mj * w0;
+    Axy += xji * Yji * mj * w0;
+    Ayx += yji * Xji * mj * w0;
+    Ayy += yji * Yji * mj * w0;
+  }
+  const double theta = atan2(Ayx - Axy, Axx + Ayy);
+  const double cs = cos(theta), sn = sin(theta);
+  const double Rxx = cs, Rxy = -sn, Ryx = sn, Ryy = cs;
+  // derive_displacement_corotational with R^-1 = R^T
+  const double Rixx_inv = Rxx, Rixy_inv = Ryx, Riyx_inv = Rxy, Riyy_inv = Ryy;
+  double Hxx = 0., Hxy = 0., Hyx = 0., Hyy = 0.;
+#pragma unroll 4
+  for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+    const uint32_t j = a.col[q];
+    const double w1 = a.w64[3 * (size_t)q + 1], w2 = a.w64[3 * (size_t)q + 2];
+    double xj, yj, vxj, vyj;
+    tl_xv(a, j, t, xj, yj, vxj, vyj);
+    const double qw = a.qw[j];
+    const double xji = xj - xi, yji = yj - yi;
+    const double Xji = a.X[j] - Xi, Yji = a.Y[j] - Yi;
+    const double uxji = Rixx_inv * xji + Rixy_inv * yji - Xji;
+    const double uyji = Riyx_inv * xji + Riyy_inv * yji - Yji;
+    Hxx += uxji * w1 * qw;
+    Hxy += uxji * w2 * qw;
+    Hyx += uyji * w1 * qw;
+    Hyy += uyji * w2 * qw;
+  }
+  // material_compute_stress_cauchy_green: E = 0.5 (F + F^T) - I
+  const double E = c->E, nu = c->nu;
+  const double D = E / (1 - nu * nu);
+  const M2 F = m2(Hxx + 1., Hyx, Hxy, Hyy + 1.);
+  const double e00 = 0.5 * (F.m[0][0] + F.m[0][0]) - 1.;
+  const double e01 = 0.5 * (F.m[0][1] + F.m[1][0]) - 0.;
+  const double e11 = 0.5 * (F.m[1][1] + F.m[1][1]) - 1.;
+  const double Sxx = D * (e00 + nu * e11);
+  const double Syy = D * (e00 * nu + e11);
+  const double Sxy = D * (1 - nu) * e01;
+  // contmech_cauchy_to_nominal, p = 0
+  const double p = 0.;
+  const M2 F2 = m2(Hxx + 1, Hyx, Hxy, Hyy + 1);
+  const double J = m2_det(F2);
+  const M2 sigma = m2(Sxx - p, Sxy, Sxy, Syy - p);
+  const M2 P = m2_mul(m2_scale(J, m2_inv(F2)), sigma);
+  const double rho = a.src.f[4][i];
+  TlRec64 r;
+  r.x = xi;
+  r.y = yi;
+  r.vx = vxi;
+  r.vy = vyi;
+  r.Pxx = P.m[0][0];
+  r.Pxy = P.m[1][0];
+  r.Pyx = P.m[0][1];
+  r.Pyy = P.m[1][1];
+  r.m = a.m[i];
+  r.c = sqrt(c->K / rho);
+  r.h = a.h[i];
+  r.rho = rho;
+  ((TlRec64*)a.rec)[i] = r;
+  a.rot[i] = make_double2(cs, sn);
+  if (a.stage == 0 && a.aux) {
+    double* dg = a.diag + 16 * (size_t)i;
+    const double v[16] = {Hxx, Hxy, Hyx, Hyy, Rxx, Rxy, Ryx, Ryy, r.Pxx, r.Pxy, r.Pyx, r.Pyy, Sxx, Sxy, Syy, 0.};
+#pragma unroll
+    for (int k = 0; k < 16; k++) dg[k] = v[k];
+  }
+}
+
+// Pass 1 of a potential-based body (FP64): displacement gradient with the
+// Randles-Libersky kernel, Green-St Venant stress, Fe = -2 V F S (stored in
+// the record's P slots, same index mapping).
+__device__ inline void tl_pass1_potential(const TlArgs& a, const SceneConst* __restrict__ c, uint32_t i, double t,
+                                          double xi, double yi, double vxi, double vyi) {
+  const double uxi = xi - a.X[i], uyi = yi - a.Y[i];
+  double Hxx = 0., Hxy = 0., Hyx = 0., Hyy = 0.;
+#pragma unroll 8
+  for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+    const uint32_t j = a.col[q];
+    const double w1 = a.w64[3 * (size_t)q + 1], w2 = a.w64[3 * (size_t)q + 2];
+    double xj, yj, vxj, vyj;
+    tl_xv(a, j, t, xj, yj, vxj, vyj);
+    const double uxj = xj - a.X[j], uyj = yj - a.Y[j];
+    const double qw = a.qw[j];
+    Hxx += (uxj - uxi) * w1 * qw;
+    Hxy += (uxj - uxi) * w2 * qw;
+    Hyx += (uyj - uyi) * w1 * qw;
+    Hyy += (uyj - uyi) * w2 * qw;
+  }
+  // material_compute_stress_green_stvenant: C = F^T F, E = 0.5 (C - I)
+  const double E = c->E, nu = c->nu;
+  const double D = E / (1 - nu * nu);
+  const M2 F = m2(Hxx + 1., Hyx, Hxy, Hyy + 1.);
+  const M2 FT = m2(F.m[0][0], F.m[1][0], F.m[0][1], F.m[1][1]);
+  const M2 C = m2_mul(FT, F);
+  const double e00 = 0.5 * (C.m[0][0] - 1.), e10 = 0.5 * (C.m[1][0] - 0.), e11 = 0.5 * (C.m[1][1] - 1.);
+  const double Sxx = D * (e00 + nu * e11);
+  const double Syy = D * (e00 * nu + e11);
+  const double Sxy = D * (1 - nu) * e10;
+  // Fe = -2*voli*Ji*Si, evaluated ((-2 voli) J) S
+  const double voli = a.qw[i];
+  const M2 Fe = m2_mul(m2_scale(-2 * voli, m2(Hxx + 1., Hyx, Hxy, Hyy + 1.)), m2(Sxx, Sxy, Sxy, Syy));
+  const double rho = a.src.f[4][i];
+  TlRec64 r;
+  r.x = xi;
+  r.y = yi;
+  r.vx = vxi;
+  r.vy = vyi;
+  r.Pxx = Fe.m[0][0];
+  r.Pxy = Fe.m[1][0];
+  r.Pyx = Fe.m[0][1];
+  r.Pyy = Fe.m[1][1];
+  r.m = a.m[i];
+  r.c = sqrt(c->K / rho);
+  r.h = a.h[i];
+  r.rho = rho;
+  ((TlRec64*)a.rec)[i] = r;
+  if (a.stage == 0 && a.aux) {
+    double* dg = a.diag + 16 * (size_t)i;
+    const double v[16] = {Hxx, Hxy, Hyx, Hyy, r.Pxx, r.Pxy, r.Pyx, r.Pyy, 0., 0., 0., 0., Sxx, Sxy, Syy, 0.};
+#pragma unroll
+    for (int k = 0; k < 16; k++) dg[k] = v[k];
+  }
+}
+
+template <bool MIXED, int V>
 __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   using T = typename TlTypes<MIXED>::T;
   using A = typename TlTypes<MIXED>::A;
@@ -141,6 +297,16 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   const double t = c->time;
   double xi, yi, vxi, vyi;
   tl_xv(a, i, t, xi, yi, vxi, vyi);
+  if constexpr (V != kTlRl) {
+    // no continuity, EOS or stress rate in these lists: rho_t = S_t = 0
+    double* pp = a.part8 + 4 * (size_t)i;
+    pp[0] = pp[1] = pp[2] = pp[3] = 0.;
+    if constexpr (V == kTlCorot)
+      tl_pass1_corot(a, c, i, t, xi, yi, vxi, vyi);
+    else
+      tl_pass1_potential(a, c, i, t, xi, yi, vxi, vyi);
+    return;
+  }
   const double uxi = xi - a.X[i], uyi = yi - a.Y[i];
   A Hxx, Hxy, Hyx, Hyy, Fxx, Fxy, Fyx, Fyy;
   // unrolled: the next neighbours' loads are issued with the current one's
@@ -216,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   pp[2] = 0. + t01;
   pp[3] = 0. + t11;
   if (a.stage == 0 && a.aux) {
-    double* dg = a.diag + 12 * (size_t)i;
+    double* dg = a.diag + 16 * (size_t)i;
     dg[8] = P.m[0][0];
     dg[9] = P.m[1][0];
     dg[10] = P.m[0][1];
@@ -232,7 +398,85 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass1(TlArgs a) {
   }
 }
 
-template <bool MIXED>
+// The variants' pair force sums of particle i (FP64), in the reference's
+// order: the force action assigns the sum, adds f_c/m (zero), then the
+// viscosity adds its pair terms.
+template <int V>
+__device__ inline void tl_pass2_forces(const TlArgs& a, const SceneConst* __restrict__ c, uint32_t i,
+                                       const TlRec64& ri, double& ax, double& ay) {
+  const TlRec64* __restrict__ rec = (const TlRec64*)a.rec;
+  const double rho0 = c->rho0;
+  double vx_t = 0., vy_t = 0.;
+  if constexpr (V == kTlCorot) {
+    // derive_stress_corotational
+    const double2 roti = a.rot[i];
+    const double Rxxi = roti.x, Rxyi = -roti.y, Ryxi = roti.y, Ryyi = roti.x;
+    const double rhoi21 = 1. / (rho0 * rho0);
+#pragma unroll 4
+    for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+      const uint32_t j = a.col[q];
+      const double w1 = a.w64[3 * (size_t)q + 1], w2 = a.w64[3 * (size_t)q + 2];
+      const TlRec64 rj = rec[j];
+      const double2 rotj = a.rot[j];
+      const double Rxxj = rotj.x, Rxyj = -rotj.y, Ryxj = rotj.y, Ryyj = rotj.x;
+      const double mj = rj.m;
+      const double rhoj21 = 1. / (rho0 * rho0);
+      const double aijx = mj * ((ri.Pxx * rhoi21 + rj.Pxx * rhoj21) * w1 + (ri.Pxy * rhoi21 + rj.Pxy * rhoj21) * w2);
+      const double aijy = mj * ((ri.Pyx * rhoi21 + rj.Pyx * rhoj21) * w1 + (ri.Pyy * rhoi21 + rj.Pyy * rhoj21) * w2);
+      const double ajix = -aijx, ajiy = -aijy;
+      const double aijxprime = aijx * Rxxi + aijy * Rxyi;
+      const double aijyprime = aijx * Ryxi + aijy * Ryyi;
+      const double ajixprime = ajix * Rxxj + ajiy * Rxyj;
+      const double ajiyprime = ajix * Ryxj + ajiy * Ryyj;
+      vx_t += 0.5 * (aijxprime - ajixprime);
+      vy_t += 0.5 * (aijyprime - ajiyprime);
+    }
+  } else {
+    // potential_comp_elastic_potential: Fe in the P slots, (M v)[r] = M[0][r] v.x + M[1][r] v.y
+    const double mi = ri.m;
+#pragma unroll 4
+    for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+      const uint32_t j = a.col[q];
+      const double wij1 = a.w64[3 * (size_t)q + 1], wij2 = a.w64[3 * (size_t)q + 2];
+      const uint32_t qr = a.rev[q];
+      const double wji1 = qr != ~0u ? a.w64[3 * (size_t)qr + 1] : 0.;
+      const double wji2 = qr != ~0u ? a.w64[3 * (size_t)qr + 2] : 0.;
+      const TlRec64 rj = rec[j];
+      const double fij0 = ri.Pxx * wij1 + ri.Pxy * wij2;
+      const double fij1 = ri.Pyx * wij1 + ri.Pyy * wij2;
+      const double fji0 = rj.Pxx * wji1 + rj.Pxy * wji2;
+      const double fji1 = rj.Pyx * wji1 + rj.Pyy * wji2;
+      vx_t += 0.5 * (fji0 - fij0) / mi;
+      vy_t += 0.5 * (fji1 - fij1) / mi;
+    }
+  }
+  vx_t += 0. / ri.m;
+  vy_t += 0. / ri.m;
+  // correctors_mghn_artificial_viscosity
+  const double alpha = c->alpha, beta = c->beta, eta = c->eta;
+#pragma unroll 4
+  for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+    const uint32_t j = a.col[q];
+    const double w1 = a.w64[3 * (size_t)q + 1], w2 = a.w64[3 * (size_t)q + 2];
+    const TlRec64 rj = rec[j];
+    const double vijx = ri.vx - rj.vx, vijy = ri.vy - rj.vy;
+    const double xij = ri.x - rj.x, yij = ri.y - rj.y;
+    const double vp = vijx * xij + vijy * yij;
+    if (vp >= 0.) continue;
+    const double cij = 0.5 * (ri.c + rj.c);
+    const double hij = 0.5 * (ri.h + rj.h);
+    const double rhoij = 0.5 * (ri.rho + rj.rho);
+    const double r2 = xij * xij + yij * yij;
+    const double muij = (hij * vp) / (r2 + eta * eta * hij * hij);
+    const double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+    vx_t += -rj.m * piij * w1;
+    vy_t += -rj.m * piij * w2;
+  }
+  ax = vx_t;
+  ay = vy_t;
+}
+
+template <bool MIXED, int V>
 __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   using T = typename TlTypes<MIXED>::T;
   using A = typename TlTypes<MIXED>::A;
@@ -251,9 +495,11 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   const T hi = ri.h, rhoi = ri.rho, ci = ri.c;
   const T alpha = (T)c->alpha, beta = (T)c->beta, eta = (T)c->eta;
   A Pxx_x, Pyx_y, Pxy_x, Pyy_y, ax, ay;
+  double vx_t = 0., vy_t = 0.;
+  if constexpr (V != kTlRl) tl_pass2_forces<V>(a, c, i, ri, vx_t, vy_t);
   constexpr int kUnroll = MIXED ? 16 : 8;
 #pragma unroll kUnroll
-  for (uint32_t q = a.rowp[i]; q < a.rowp[i + 1]; q++) {
+  for (uint32_t q = a.rowp[i]; q < (V == kTlRl ? a.rowp[i + 1] : 0u); q++) {
     const uint32_t j = a.col[q];
     T w[3];
     load_w<MIXED>(a, q, w);
@@ -284,8 +530,13 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   double d[8];
   d[0] = 0. + vxi;  // contmech_advection
   d[1] = 0. + vyi;
-  d[2] = ax.val() + (1. / rho0 * (pxx_x + pyx_y) + 0. / a.m[i]);  // contmech_momentum_tl
-  d[3] = ay.val() + (1. / rho0 * (pxy_x + pyy_y) + 0. / a.m[i]);
+  if constexpr (V == kTlRl) {
+    d[2] = ax.val() + (1. / rho0 * (pxx_x + pyx_y) + 0. / a.m[i]);  // contmech_momentum_tl
+    d[3] = ay.val() + (1. / rho0 * (pxy_x + pyy_y) + 0. / a.m[i]);
+  } else {
+    d[2] = vx_t;
+    d[3] = vy_t;
+  }
   d[4] = pp[0];
   d[5] = pp[1];
   d[6] = pp[2];
@@ -300,13 +551,18 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
     for (int k = 0; k < 8; k++) o[k] = d[k];
     if (a.aux) {
       double* x = a.aux + 16 * (size_t)i;
-      const double* dg = a.diag + 12 * (size_t)i;
+      const double* dg = a.diag + 16 * (size_t)i;
+      if constexpr (V == kTlRl) {
 #pragma unroll
-      for (int k = 0; k < 12; k++) x[k] = dg[k];
-      x[12] = pxx_x;
-      x[13] = pxy_x;
-      x[14] = pyx_y;
-      x[15] = pyy_y;
+        for (int k = 0; k < 12; k++) x[k] = dg[k];
+        x[12] = pxx_x;
+        x[13] = pxy_x;
+        x[14] = pyx_y;
+        x[15] = pyy_y;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) x[k] = dg[k];
+      }
     }
     return;
   }
@@ -484,6 +740,50 @@ __global__ __launch_bounds__(kBlock) void k_tl_rl(uint32_t n, const uint32_t* __
   }
 }
 
+// precomp_sph (src/precomp_shape_functions.h:210-229) on the frozen lists:
+// the plain kernel of each pair at the set-up positions (co-rotated bodies).
+__global__ __launch_bounds__(kBlock) void k_tl_sph(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                   const uint32_t* __restrict__ col, const double* __restrict__ x,
+                                                   const double* __restrict__ y, const double* __restrict__ h,
+                                                   const double* __restrict__ m, const double* __restrict__ rho,
+                                                   double* __restrict__ w64, float* __restrict__ w32,
+                                                   double* __restrict__ qw) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  qw[i] = m[i] / rho[i];
+  for (uint32_t q = rowp[i]; q < rowp[i + 1]; q++) {
+    double w[3];
+    spline64(x[i], y[i], x[col[q]], y[col[q]], h[i], w);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      w64[3 * (size_t)q + k] = w[k];
+      w32[3 * (size_t)q + k] = (float)w[k];
+    }
+  }
+}
+
+// precomp_randles_libersky_symmetric (src/precomp_shape_functions.h:292-316):
+// for pair q = (i, j), the index of pair (j, i) in j's row (rows are sorted
+// by idx, so a binary search finds the reference's first match), or ~0u when
+// the lists hold no such pair (its kernel is zero: q >= 2 with h_j).
+__global__ __launch_bounds__(kBlock) void k_tl_rev(uint32_t n, const uint32_t* __restrict__ rowp,
+                                                   const uint32_t* __restrict__ col, uint32_t* __restrict__ rev) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (uint32_t q = rowp[i]; q < rowp[i + 1]; q++) {
+    const uint32_t j = col[q];
+    uint32_t lo = rowp[j], hi = rowp[j + 1];
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (col[mid] < i)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    rev[q] = (lo < rowp[j + 1] && col[lo] == i) ? lo : ~0u;
+  }
+}
+
 inline uint32_t nblk(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 template <typename T>
@@ -519,6 +819,8 @@ TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stag
   a.rec = t.rec;
   a.part8 = t.part8;
   a.diag = t.diag;
+  a.rot = t.rot;
+  a.rev = t.rev;
   a.stage = stage;
   a.y = s.y;
   a.K = dst;
@@ -527,13 +829,20 @@ TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stag
   return a;
 }
 
-void launch_stage(const TlShared& s, const TlArgs& a) {
-  if (s.precision == MFREE_PRECISION_MIXED) {
-    hipLaunchKernelGGL(k_tl_pass1<true>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
-    hipLaunchKernelGGL(k_tl_pass2<true>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
+void launch_stage(const TlState& t, const TlShared& s, const TlArgs& a) {
+  const dim3 g(xcd_grid(nblk(a.n))), b(kBlock);
+  if (t.variant == kTlCorot) {
+    hipLaunchKernelGGL((k_tl_pass1<false, kTlCorot>), g, b, 0, s.st, a);
+    hipLaunchKernelGGL((k_tl_pass2<false, kTlCorot>), g, b, 0, s.st, a);
+  } else if (t.variant == kTlPotential) {
+    hipLaunchKernelGGL((k_tl_pass1<false, kTlPotential>), g, b, 0, s.st, a);
+    hipLaunchKernelGGL((k_tl_pass2<false, kTlPotential>), g, b, 0, s.st, a);
+  } else if (s.precision == MFREE_PRECISION_MIXED) {
+    hipLaunchKernelGGL((k_tl_pass1<true, kTlRl>), g, b, 0, s.st, a);
+    hipLaunchKernelGGL((k_tl_pass2<true, kTlRl>), g, b, 0, s.st, a);
   } else {
-    hipLaunchKernelGGL(k_tl_pass1<false>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
-    hipLaunchKernelGGL(k_tl_pass2<false>, dim3(xcd_grid(nblk(a.n))), dim3(kBlock), 0, s.st, a);
+    hipLaunchKernelGGL((k_tl_pass1<false, kTlRl>), g, b, 0, s.st, a);
+    hipLaunchKernelGGL((k_tl_pass2<false, kTlRl>), g, b, 0, s.st, a);
   }
 }
 
@@ -651,10 +960,11 @@ double host_max_h(const TlShared& s) {
 
 }  // namespace
 
-mfree_status tl_alloc(TlState& t, uint32_t N, const double* X, const double* Y,
+mfree_status tl_alloc(TlState& t, int variant, uint32_t N, const double* X, const double* Y,
                       const std::vector<std::vector<uint32_t>>& left, const std::vector<std::vector<uint32_t>>& right,
                       const std::vector<uint32_t>& off) {
   t.N = N;
+  t.variant = variant;
   t.hbc.assign(N, 0);
   for (size_t b = 0; b < left.size(); b++) {
     for (uint32_t k : left[b]) t.hbc[off[b] + k] = 1;
@@ -662,7 +972,7 @@ mfree_status tl_alloc(TlState& t, uint32_t N, const double* X, const double* Y,
   }
   bool bad = talloc(t, &t.X, N) || talloc(t, &t.Y, N) || talloc(t, &t.bc, N) || talloc(t, &t.qw, N) ||
              talloc(t, &t.rec, 12 * (size_t)N) || talloc(t, &t.part8, 4 * (size_t)N) ||
-             talloc(t, &t.diag, 12 * (size_t)N);
+             talloc(t, &t.diag, 16 * (size_t)N) || (variant == kTlCorot && talloc(t, &t.rot, N));
   for (int k = 0; k < 8 && !bad; k++) bad = talloc(t, &t.Kb.f[k], N) != hipSuccess;
   if (bad) {
     t.msg = "hipMalloc failed";
@@ -699,12 +1009,20 @@ mfree_status tl_ensure_precomputed(TlState& t, const TlShared& s) {
   t.rowp = rowp;
   t.col = col;
   t.npairs = np;
-  if (talloc(t, &t.w64, 3 * np) || talloc(t, &t.w32, 3 * np)) {
+  if (talloc(t, &t.w64, 3 * np) || talloc(t, &t.w32, 3 * np) ||
+      (t.variant == kTlPotential && talloc(t, &t.rev, np))) {
     t.msg = "hipMalloc failed";
     return MFREE_ERR_HIP;
   }
-  hipLaunchKernelGGL(k_tl_rl, dim3(nblk(s.N)), dim3(kBlock), 0, s.st, s.N, t.rowp, t.col, s.y.f[0], s.y.f[1], s.h,
-                     s.m, s.y.f[4], t.w64, t.w32, t.qw);
+  if (t.variant == kTlCorot) {
+    hipLaunchKernelGGL(k_tl_sph, dim3(nblk(s.N)), dim3(kBlock), 0, s.st, s.N, t.rowp, t.col, s.y.f[0], s.y.f[1],
+                       s.h, s.m, s.y.f[4], t.w64, t.w32, t.qw);
+  } else {
+    hipLaunchKernelGGL(k_tl_rl, dim3(nblk(s.N)), dim3(kBlock), 0, s.st, s.N, t.rowp, t.col, s.y.f[0], s.y.f[1],
+                       s.h, s.m, s.y.f[4], t.w64, t.w32, t.qw);
+    if (t.variant == kTlPotential)
+      hipLaunchKernelGGL(k_tl_rev, dim3(nblk(s.N)), dim3(kBlock), 0, s.st, s.N, t.rowp, t.col, t.rev);
+  }
   if (hipStreamSynchronize(s.st) != hipSuccess) {
     t.msg = "RL precompute failed";
     return MFREE_ERR_HIP;
@@ -725,7 +1043,7 @@ void tl_enqueue_step(TlState& t, const TlShared& s, EventPairs* evs, const TakeE
       e1 = take();
       if (e0) hipEventRecord(e0, s.st);
     }
-    launch_stage(s, a);
+    launch_stage(t, s, a);
     if (evs && e0 && e1) {
       hipEventRecord(e1, s.st);
       evs->push_back({e0, e1});
@@ -737,7 +1055,7 @@ void tl_enqueue_eval(TlState& t, const TlShared& s, double* d8, double* aux) {
   TlArgs a = tl_args(t, s, s.y, s.K, 0);
   a.d8 = d8;
   a.aux = aux;
-  launch_stage(s, a);
+  launch_stage(t, s, a);
 }
 
 mfree_status tl_get_pairs(TlState& t, const TlShared& s, uint32_t body, uint32_t o, uint32_t n, uint32_t* row_ptr,

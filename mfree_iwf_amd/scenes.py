@@ -189,6 +189,31 @@ def make_plate(dx=8e-5, L=0.1, W=0.04, r_hole=0.01, hdx=1.0, rho0=7850.0, E=2e9,
     return PlateScene(ps, prm, left[:nl.value], right[:nr.value], v_bc, dx)
 
 
+def reference_tensile(nbox):
+    """The tension drivers' own scene (tensile_tl, tensile_corot,
+    tensile_potential: src/benchmarks/tensile.cpp:137-361): a unit square of
+    nbox x nbox particles at (i dx, j dx), dx = 1/(nbox - 1), created with
+    idx = i nbox + j, rho0 = 1, h = dx, m = dx^2, E = 1e7, nu = 0.4,
+    alpha = beta = 1, eta = 0.1, dt = 1e-6 s, the columns i = 0 and
+    i = nbox - 1 pulled apart at 10 m/s."""
+    dx = 1.0 / (nbox - 1)
+    n = nbox * nbox
+    ps = ParticleSet.zeros(n)
+    ii, jj = np.meshgrid(np.arange(nbox), np.arange(nbox), indexing="ij")
+    ps.x[:] = (ii * dx).ravel()
+    ps.y[:] = (jj * dx).ravel()
+    ps.X[:] = ps.x
+    ps.Y[:] = ps.y
+    ps.rho[:] = 1.0
+    ps.h[:] = dx
+    ps.m[:] = dx * dx
+    left = np.nonzero(ii.ravel() == 0)[0].astype(np.uint32)
+    right = np.nonzero(ii.ravel() == nbox - 1)[0].astype(np.uint32)
+    prm = Params(E=1e7, nu=0.4, rho0=1.0, wdeltap=0.0, corr_exp=0.0, mghn_eps=0.0, alpha=1.0, beta=1.0, eta=0.1,
+                 xsph_eps=0.0, dt=1e-6, time=0.0)
+    return PlateScene(ps, prm, left, right, 10.0, dx)
+
+
 def plate_config3():
     """Tension plate 0.1 x 0.04 m, hole r = 0.01 m, dx = 8e-5 m (~578k particles)."""
     return make_plate()

@@ -32,7 +32,7 @@ def test_reference_drivers_compile_link_and_map_to_pipelines(tmp_path):
     for line in out.splitlines():
         name, *bodies = line.split()
         got[name] = [tuple(int(v) for v in b.split(":")) for b in bodies]
-    UL, TL, GD = 0, 1, 2
+    UL, TL, GD, CR, PT = 0, 1, 2, 3, 4
     # the library's pipelines
     assert [m for m, _ in got["rings_monaghan"]] == [UL] and got["rings_monaghan"][0][1] > 0
     assert [m for m, _ in got["tensile_tl"]] == [TL]
@@ -41,9 +41,12 @@ def test_reference_drivers_compile_link_and_map_to_pipelines(tmp_path):
     # contact.h)
     assert [m for m, _ in got["rings_monaghan_contact"]] == [UL, UL]
     assert [m for m, _ in got["rings_godunov_contact"]] == [GD, GD]
+    assert [m for m, _ in got["tensile_corot"]] == [CR]
+    assert [m for m, _ in got["tensile_potential"]] == [PT]
+    assert [m for m, _ in got["rings_corot_contact"]] == [CR, CR]
+    assert [m for m, _ in got["rings_potential_contact"]] == [PT, PT]
     # methods outside the library are refused, not run on the CPU
-    for name in ("rings_corot_contact", "rings_potential_contact",
-                 "rings_tl_contact", "tensile_corot", "tensile_potential",
+    for name in ("rings_tl_contact",
                  # UL with boundary conditions
                  "tensile_monaghan", "tensile_godunov"):
         assert all(m == -1 for m, _ in got[name]), name
