@@ -20,6 +20,7 @@
 
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1572,3 +1573,113 @@ int orc_tl_step(const orc_consts* c, orc_tl* t, uint32_t n, orc_soa* s, double d
 }
 
 }  // extern "C"
+
+// ==========================================================================
+// Penalty contact (src/contact.h, src/boundary.h, src/convex_hull.h)
+
+extern "C" {
+
+uint32_t orc_convex_hull(const double* x, const double* y, uint32_t n, uint32_t* idx_out) {
+  // src/convex_hull.h:30-95: points as (x, y, index), sorted by leftlower
+  // (x, then y); ccw(p1, p2, p3) = (p2 - p1) x (p3 - p1)
+  struct P {
+    double x, y;
+    uint32_t i;
+  };
+  std::vector<P> pts(n);
+  for (uint32_t i = 0; i < n; i++) pts[i] = P{x[i], y[i], i};
+  std::sort(pts.begin(), pts.end(), [](const P& a, const P& b) {
+    if (a.x < b.x) return true;
+    if (a.x > b.x) return false;
+    if (a.y < b.y) return true;
+    if (a.y > b.y) return false;
+    return false;
+  });
+  auto ccw = [](const P& p1, const P& p2, const P& p3) {
+    return (p2.x - p1.x) * (p3.y - p1.y) - (p2.y - p1.y) * (p3.x - p1.x);
+  };
+  std::vector<P> hull(2 * (size_t)n + 1);
+  int k = 0;
+  for (int i = 0; i < (int)n; ++i) {
+    while (k >= 2 && ccw(hull[k - 2], hull[k - 1], pts[i]) < 0) --k;
+    hull[k++] = pts[i];
+  }
+  for (int i = (int)n - 2, t = k + 1; i >= 0; --i) {
+    while (k >= t && ccw(hull[k - 2], hull[k - 1], pts[i]) < 0) --k;
+    hull[k++] = pts[i];
+  }
+  uint32_t m = 0;
+  for (int i = 0; i < k - 1; i++) idx_out[m++] = hull[i].i;
+  return m;
+}
+
+void orc_boundary_update(uint32_t nb, const uint32_t* idx, const double* x, const double* y, double* pts) {
+  // src/body.h:152-165: positions first, then src/boundary.h:47-73
+  for (uint32_t k = 0; k < nb; k++) {
+    pts[4 * (size_t)k] = x[idx[k]];
+    pts[4 * (size_t)k + 1] = y[idx[k]];
+  }
+  auto normalize = [](double vx, double vy, double* o) {
+    // glm::normalize(v) = v * inversesqrt(dot(v, v)), inversesqrt(d) = 1/sqrt(d)
+    double is = 1. / sqrt(vx * vx + vy * vy);
+    o[0] = vx * is;
+    o[1] = vy * is;
+  };
+  for (uint32_t i = 0; i < nb; i++) {
+    int left = ((int)i) - 1;
+    int right = i + 1;
+    if (left < 0) left = nb - 1;
+    if (right >= (int)nb) right = 0;
+    double px = pts[4 * (size_t)i], py = pts[4 * (size_t)i + 1];
+    double tlx = pts[4 * (size_t)left] - px, tly = pts[4 * (size_t)left + 1] - py;
+    double trx = pts[4 * (size_t)right] - px, try_ = pts[4 * (size_t)right + 1] - py;
+    double nl[2], nr[2];
+    normalize(-tly, tlx, nl);
+    normalize(try_, -trx, nr);
+    pts[4 * (size_t)i + 2] = 0.5 * (nl[0] + nr[0]);
+    pts[4 * (size_t)i + 3] = 0.5 * (nl[1] + nr[1]);
+  }
+}
+
+void orc_contact(uint32_t nb, const double* pts, uint32_t ns, const double* x, const double* y, const double* m,
+                 double dt, double* fcx, double* fcy) {
+  // src/contact.h:36-80
+  const double dt2 = dt * dt;
+  const double alpha = 0.1;
+  for (uint32_t i = 0; i < ns; i++) {
+    fcx[i] = 0.;
+    fcy[i] = 0.;
+  }
+  for (uint32_t i = 0; i < ns; i++) {
+    double qx = x[i], qy = y[i];
+    // kd_tree::find_closest_point (src/kd_tree.cpp:55-106) returns a point of
+    // least (qx - px)^2 + (qy - py)^2
+    double best = DBL_MAX;
+    uint32_t bk = 0;
+    for (uint32_t k = 0; k < nb; k++) {
+      double dx = qx - pts[4 * (size_t)k];
+      double dy = qy - pts[4 * (size_t)k + 1];
+      double dist2 = dx * dx + dy * dy;
+      if (dist2 < best) {
+        best = dist2;
+        bk = k;
+      }
+    }
+    const double* c = pts + 4 * (size_t)bk;
+    double dx = qx - c[0];
+    double dy = qy - c[1];
+    double pdotn = dx * c[2] + dy * c[3];
+    bool inside = pdotn <= 0.;
+    if (!inside) continue;
+    double gN_nx = c[2] * pdotn;
+    double gN_ny = c[3] * pdotn;
+    double ms = m[i];
+    double fx = ms * gN_nx / dt2 * alpha;
+    double fy = ms * gN_ny / dt2 * alpha;
+    fcx[i] = -fx;
+    fcy[i] = -fy;
+  }
+}
+
+}  // extern "C"
+

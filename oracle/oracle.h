@@ -195,6 +195,31 @@ int orc_tl_derivatives(const orc_consts* c, orc_tl* t, uint32_t n, orc_soa* s,
 int orc_tl_step(const orc_consts* c, orc_tl* t, uint32_t n, orc_soa* s, double dt,
                 uint32_t nsteps, double* time, int threads);
 
+/* ---------------- Penalty contact between bodies ------------------------ */
+
+/* convex_hull (src/convex_hull.h:62-95): Andrew's monotone chain over the
+ * points sorted by (x, y), turning test ccw, popping clockwise turns only
+ * (collinear points stay); writes the hull's point indexes (counter-clockwise
+ * from the lowest-leftmost point, the first not repeated) and returns their
+ * count.  idx_out holds up to 2n entries. */
+uint32_t orc_convex_hull(const double* x, const double* y, uint32_t n, uint32_t* idx_out);
+
+/* body::update_boundary + boundary::update_normals (src/body.h:152-165,
+ * src/boundary.h:47-73): boundary point k takes the position of particle
+ * idx[k]; its normal is the mean of the outward normals of the edges to
+ * its neighbours k-1 and k+1 (cyclic), each normalised as glm::normalize.
+ * pts: 4 per point (x, y, nx, ny). */
+void orc_boundary_update(uint32_t nb, const uint32_t* idx, const double* x, const double* y, double* pts);
+
+/* contact_apply_body_to_boundary (src/contact.h:36-80) of one slave body
+ * against one master outline: every slave particle's contact force is
+ * zeroed, then set to -m (n (p - c).n) alpha / dt^2 (alpha = 0.1) where the
+ * nearest master boundary point c with normal n has (p - c).n <= 0.  The
+ * nearest point is found by exhaustive search; of points at exactly the
+ * same squared distance the first in boundary order is taken. */
+void orc_contact(uint32_t nb, const double* pts, uint32_t ns, const double* x, const double* y, const double* m,
+                 double dt, double* fcx, double* fcy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,10 @@
-// Harness around the reference's own neighbour-search and kernel sources.
+// Harness around the reference's own neighbour-search, kernel, convex-hull
+// and k-d tree sources.
 //
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) from
 // the reference tree as it lies on disk (src/kernel.cpp, src/particle.cpp,
-// src/grid.h, src/benchmarks/neighbor_search.h) into oracle/_ref/, and used
+// src/kd_tree.cpp, src/grid.h, src/convex_hull.h,
+// src/benchmarks/neighbor_search.h) into oracle/_ref/, and used
 // only to pin the oracle and to write the golden vectors under tests/golden.
 // The reference's physics headers (derivatives.h, correctors.h, material.h,
 // cont_mech.h, precomp_shape_functions.h, body.h, rk4.h) include glm, which
@@ -15,7 +17,9 @@
 #include <type_traits>
 #include <vector>
 
+#include "convex_hull.h"
 #include "grid.h"
+#include "kd_tree.h"
 #include "kernel.h"
 #include "particle.h"
 #include "benchmarks/neighbor_search.h"
@@ -97,6 +101,36 @@ int ref_find_neighbors(uint32_t n, const double* x, const double* y, double h, u
   }
   for (auto* p : ps) delete p;
   return rc;
+}
+
+// src/convex_hull.h:62-95 on particles at (x, y); returns the hull's
+// particle indexes and their count (idx_out holds up to n entries).
+uint32_t ref_convex_hull(const double* x, const double* y, uint32_t n, uint32_t* idx_out) {
+  std::vector<particle*> ps(n);
+  for (uint32_t i = 0; i < n; i++) {
+    ps[i] = new particle(i);
+    ps[i]->x = x[i];
+    ps[i]->y = y[i];
+  }
+  std::vector<unsigned int> h = convex_hull(ps.data(), n);
+  for (size_t k = 0; k < h.size() && k < n; k++) idx_out[k] = h[k];
+  for (auto* p : ps) delete p;
+  return (uint32_t)h.size();
+}
+
+// kd_tree (src/kd_tree.cpp:20-116) over points pts (x, y, nx, ny per point):
+// the closest point of each query, written as (x, y, nx, ny).
+void ref_kd_nearest(const double* pts, uint32_t nb, const double* qx, const double* qy, uint32_t nq, double* out) {
+  std::vector<kd_point> p(nb);
+  for (uint32_t k = 0; k < nb; k++) p[k] = kd_point{pts[4 * k], pts[4 * k + 1], pts[4 * k + 2], pts[4 * k + 3]};
+  kd_tree tree(p.data(), nb);
+  for (uint32_t i = 0; i < nq; i++) {
+    kd_point c = tree.find_closest_point(qx[i], qy[i]);
+    out[4 * i] = c.x;
+    out[4 * i + 1] = c.y;
+    out[4 * i + 2] = c.nx;
+    out[4 * i + 3] = c.ny;
+  }
 }
 
 }  // extern "C"

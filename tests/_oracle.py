@@ -77,6 +77,10 @@ def lib():
                                          dptr, dptr]
         L.orc_tl_step.argtypes = [C.POINTER(Consts), C.c_void_p, C.c_uint32, C.POINTER(Soa), C.c_double,
                                   C.c_uint32, dptr, C.c_int]
+        L.orc_convex_hull.argtypes = [dptr, dptr, C.c_uint32, u32p]
+        L.orc_convex_hull.restype = C.c_uint32
+        L.orc_boundary_update.argtypes = [C.c_uint32, u32p, dptr, dptr, dptr]
+        L.orc_contact.argtypes = [C.c_uint32, dptr, C.c_uint32, dptr, dptr, dptr, C.c_double, dptr, dptr]
         L.orc_G.restype = C.c_double
         L.orc_K.restype = C.c_double
         L.orc_G.argtypes = [C.c_double, C.c_double]
@@ -93,6 +97,9 @@ def ref():
         R.ref_cubic_spline.argtypes = [C.c_double] * 5 + [dptr]
         R.ref_verlet_pairs.argtypes = [C.c_uint32, dptr, dptr, dptr, u32p, u32p, C.c_uint64, dptr]
         R.ref_find_neighbors.argtypes = [C.c_uint32, dptr, dptr, C.c_double, u32p, u32p, C.c_uint64]
+        R.ref_convex_hull.argtypes = [dptr, dptr, C.c_uint32, u32p]
+        R.ref_convex_hull.restype = C.c_uint32
+        R.ref_kd_nearest.argtypes = [dptr, C.c_uint32, dptr, dptr, C.c_uint32, dptr]
         _ref = R
     return _ref
 
@@ -279,3 +286,51 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64).ravel()
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+# ---- penalty contact -------------------------------------------------------
+
+def convex_hull(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(2 * len(x) + 1, dtype=np.uint32)
+    k = lib().orc_convex_hull(_p(x), _p(y), len(x), out.ctypes.data_as(u32p))
+    return out[:k].copy()
+
+
+def convex_hull_ref(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(len(x), dtype=np.uint32)
+    k = ref().ref_convex_hull(_p(x), _p(y), len(x), out.ctypes.data_as(u32p))
+    return out[:k].copy()
+
+
+def boundary_update(idx, x, y):
+    """(nb, 4) array of boundary points (x, y, nx, ny)."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    pts = np.zeros((len(idx), 4))
+    lib().orc_boundary_update(len(idx), idx.ctypes.data_as(u32p), _p(np.ascontiguousarray(x)),
+                              _p(np.ascontiguousarray(y)), _p(pts))
+    return pts
+
+
+def contact(pts, x, y, m, dt):
+    """Contact forces of slave particles (x, y, m) against master boundary
+    points pts; returns (fcx, fcy)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n = len(x)
+    fcx, fcy = np.zeros(n), np.zeros(n)
+    lib().orc_contact(len(pts), _p(pts), n, _p(np.ascontiguousarray(x)), _p(np.ascontiguousarray(y)),
+                      _p(np.ascontiguousarray(m)), dt, _p(fcx), _p(fcy))
+    return fcx, fcy
+
+
+def kd_nearest_ref(pts, qx, qy):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    qx = np.ascontiguousarray(qx, dtype=np.float64)
+    qy = np.ascontiguousarray(qy, dtype=np.float64)
+    out = np.zeros((len(qx), 4))
+    ref().ref_kd_nearest(_p(pts), len(pts), _p(qx), _p(qy), len(qx), _p(out))
+    return out
+
