@@ -132,7 +132,7 @@ typedef struct mfree_options {
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
  * (reference configuration) are read for TL bodies; fcx, fcy (contact
- * forces) may be NULL and must otherwise be zero. */
+ * forces) may be NULL (zero); mfree_get_state fills them when given. */
 typedef struct mfree_particles {
   double *x, *y, *X, *Y;
   double *m, *rho, *h;
@@ -239,6 +239,47 @@ mfree_status mfree_vtk_wait(mfree_ctx* ctx);
  * Needs no device. */
 mfree_status mfree_vtk_write_host(const char* path, uint32_t nbodies, const mfree_particles* bodies,
                                   const uint32_t* counts);
+
+/* ---------------------------------------------------------------------
+ * Penalty contact between bodies (src/contact.h:36-80), run by the main
+ * loop after every step (src/mfree_iwf.cpp:84-97): each body steps, its
+ * outline follows its particles, then every ordered pair of bodies
+ * (master i, slave j != i) sets the slave's contact forces, which its
+ * next step reads as f_c / m in the momentum equations.
+ */
+
+/* Replaces boundary<T>(indexes, particles) + body<T>::set_boundary
+ * (src/boundary.h:31-45, src/body.h:200-202): the outline of a body, its
+ * particle idx in order around the body (the drivers take the convex hull,
+ * mfree_convex_hull).  n = 0 removes it.  Forces become part of every step
+ * of the context from then on. */
+mfree_status mfree_set_boundary(mfree_ctx* ctx, uint32_t body, const uint32_t* idx, uint32_t n);
+/* Replaces body<T>::update_boundary of every body with an outline
+ * (src/body.h:152-165, src/boundary.h:47-73): outline points take their
+ * particles' current positions; each normal is the mean of the outward
+ * normals of the edges to its two neighbours (glm::normalize each). */
+mfree_status mfree_update_boundaries(mfree_ctx* ctx);
+/* The body's outline points as (x, y, nx, ny) (cap points); *n = their count
+ * (pts = NULL: only the count). */
+mfree_status mfree_get_boundary(mfree_ctx* ctx, uint32_t body, double* pts, uint32_t cap, uint32_t* n);
+/* Replaces contact_apply_body_to_boundary(master, slave) (src/contact.h:36-80):
+ * unless one of the bodies has no outline, every particle of the slave body
+ * gets f_c = -m n ((p - c).n) alpha / dt^2 (alpha = 0.1, dt the slave's)
+ * where c is the master outline point closest to it and (p - c).n <= 0,
+ * and f_c = 0 otherwise.  The contexts may differ (on one device); the
+ * master's outline must be current (mfree_update_boundaries).  The closest
+ * point is exact; of points at exactly the same squared distance the first
+ * in outline order is taken (the reference's k-d tree takes the first it
+ * visits). */
+mfree_status mfree_contact(mfree_ctx* master, uint32_t master_body, mfree_ctx* slave, uint32_t slave_body);
+/* The main loop's contact pass over the bodies of one context
+ * (src/mfree_iwf.cpp:91-97): mfree_contact(i, j) for i = 0.., j = 0.., i != j. */
+mfree_status mfree_contact_all(mfree_ctx* ctx);
+/* Replaces convex_hull (src/convex_hull.h:62-95), host only: the idx of the
+ * points' convex hull, counter-clockwise from the lowest-leftmost point,
+ * collinear points on the hull kept; returns their count (idx_out holds up
+ * to 2n entries). */
+uint32_t mfree_convex_hull(const double* x, const double* y, uint32_t n, uint32_t* idx_out);
 
 /* Replaces body<T>::step() (src/body.h:223-225 -> src/rk4.h:71-108) for
  * every body: nsteps RK4 steps.  Each body advances its own time by its dt

@@ -8,5 +8,5 @@ from ._capi import (METHOD_TL_COROTATIONAL, METHOD_TL_POTENTIAL, METHOD_TL_RL, M
                     METHOD_UL_MONAGHAN, PRECISION_FP64, PRECISION_MIXED, MfreeError,  # noqa: F401
                     Params)
 from .scenes import ParticleSet, cubic_spline, make_plate, make_rings  # noqa: F401
-from .solver import Solver, device_count  # noqa: F401
+from .solver import Solver, convex_hull, device_count  # noqa: F401
 from .surrogate import Surrogate  # noqa: F401

@@ -148,6 +148,12 @@ SIGNATURES = {
                                           C.POINTER(C.c_uint64)]),
     "mfree_mpnn_forward_backward": (C.c_int32, [C.c_int32, C.c_uint32, u32ptr, u32ptr, fptr, fptr, fptr, fptr,
                                                 fptr, fptr, fptr]),
+    "mfree_set_boundary": (C.c_int32, [C.c_void_p, C.c_uint32, u32ptr, C.c_uint32]),
+    "mfree_update_boundaries": (C.c_int32, [C.c_void_p]),
+    "mfree_get_boundary": (C.c_int32, [C.c_void_p, C.c_uint32, dptr, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "mfree_contact": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "mfree_contact_all": (C.c_int32, [C.c_void_p]),
+    "mfree_convex_hull": (C.c_uint32, [dptr, dptr, C.c_uint32, u32ptr]),
 }
 
 _lib = None

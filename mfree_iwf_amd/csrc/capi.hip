@@ -209,6 +209,10 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, hipMemcpy(c->scene[0], sc.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->gid[0], gid.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice));
   c->cur = 0;
+  {
+    mfree_status st = contact_alloc(c);
+    if (st != MFREE_OK) return st;
+  }
   c->tile_grid = (uint32_t)std::max(8, prop.multiProcessorCount) * 2;
   c->tile_grid = (c->tile_grid + kXcds - 1) / kXcds * kXcds;
   if (tl) {
@@ -300,6 +304,8 @@ UlArgs ul_args(mfree_ctx* c, int b) {
   u.tile_off = c->tile_cells ? c->tile_off : nullptr;
   u.tile_grid = c->tile_grid;
   u.xcd_chunk = c->opt.xcd_chunk ? c->opt.xcd_chunk : kXcdChunkDefault;
+  u.fcx = c->fc_active ? c->fcx : nullptr;
+  u.fcy = c->fc_active ? c->fcy : nullptr;
   return u;
 }
 
@@ -331,6 +337,8 @@ TlShared tl_shared(mfree_ctx* c) {
   t.perm = c->perm;
   t.err = c->errb;
   t.h_search = c->opt.tl_h_search;
+  t.fcx = c->fc_active ? c->fcx : nullptr;
+  t.fcy = c->fc_active ? c->fcy : nullptr;
   return t;
 }
 
@@ -838,12 +846,25 @@ mfree_status mfree_add_body(mfree_ctx* c, const mfree_params* p, const mfree_par
   // the mixed pair kernel packs sorted positions into 28 bits
   if ((uint64_t)c->off.back() + n >= (1ull << 28)) return fail(c, MFREE_ERR_ARG, "too many particles (max 2^28)");
   if (!(p->dt > 0) || !(p->rho0 > 0)) return fail(c, MFREE_ERR_ARG, "dt and rho0 must be positive");
+  bool fc = false;
   for (uint32_t i = 0; i < n; i++) {
-    if ((host->fcx && host->fcx[i] != 0.) || (host->fcy && host->fcy[i] != 0.))
-      return fail(c, MFREE_ERR_UNSUPPORTED, "contact forces (fcx, fcy) must be zero");
+    fc = fc || (host->fcx && host->fcx[i] != 0.) || (host->fcy && host->fcy[i] != 0.);
     if (!(host->h[i] > 0) || !(host->rho[i] > 0) || !isfinite(host->x[i]) || !isfinite(host->y[i]))
       return fail(c, MFREE_ERR_ARG, "h and rho must be positive, positions finite");
   }
+  // contact forces given with the body (src/particle.h:57): kept for commit
+  if (fc && c->opt.pipeline == MFREE_PIPELINE_REFERENCE)
+    return fail(c, MFREE_ERR_UNSUPPORTED, "contact forces on the reference-structured pipeline");
+  const size_t before = c->off.back();
+  if (fc && c->hfcx.empty()) {
+    c->hfcx.assign(before, 0.);
+    c->hfcy.assign(before, 0.);
+  }
+  if (!c->hfcx.empty())
+    for (uint32_t i = 0; i < n; i++) {
+      c->hfcx.push_back(host->fcx ? host->fcx[i] : 0.);
+      c->hfcy.push_back(host->fcy ? host->fcy[i] : 0.);
+    }
   auto app = [n](std::vector<double>& v, const double* s) { v.insert(v.end(), s, s + n); };
   app(c->hx, host->x);
   app(c->hy, host->y);
@@ -913,6 +934,13 @@ mfree_status mfree_set_state(mfree_ctx* c, uint32_t body, const mfree_particles*
   const int b = c->cur;
   for (int k = 0; k < 8; k++)
     HIPCHK(c, hipMemcpy(c->y[b].f[k] + o, src[k], n * sizeof(double), hipMemcpyHostToDevice));
+  if (host->fcx || host->fcy) {
+    bool nz = false;
+    for (uint32_t i = 0; i < n; i++) nz = nz || (host->fcx && host->fcx[i] != 0.) || (host->fcy && host->fcy[i] != 0.);
+    if (nz && (st = contact_activate(c))) return st;
+    if (host->fcx) HIPCHK(c, hipMemcpy(c->fcx + o, host->fcx, n * sizeof(double), hipMemcpyHostToDevice));
+    if (host->fcy) HIPCHK(c, hipMemcpy(c->fcy + o, host->fcy, n * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (!tl) {
     HIPCHK(c, hipMemcpy(c->m[b] + o, host->m, n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->h[b] + o, host->h, n * sizeof(double), hipMemcpyHostToDevice));
@@ -951,6 +979,9 @@ mfree_status mfree_get_state(mfree_ctx* c, uint32_t body, mfree_particles* host)
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < 10; k++)
     if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], tmp[k] + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  // contact forces are kept by global idx
+  if (host->fcx) HIPCHK(c, hipMemcpy(host->fcx, c->fcx + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (host->fcy) HIPCHK(c, hipMemcpy(host->fcy, c->fcy + o, n * sizeof(double), hipMemcpyDeviceToHost));
   if (host->X)
     for (uint32_t i = 0; i < n; i++) host->X[i] = c->hX[o + i];
   if (host->Y)

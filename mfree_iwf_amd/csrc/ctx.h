@@ -77,6 +77,19 @@ struct mfree_ctx {
   PairRec32* rec32b = nullptr;
   double* vpart = nullptr;       // max |v|^2 per binning chunk
 
+  // penalty contact between bodies (contact.hip)
+  std::vector<std::vector<uint32_t>> bnd;  // per body: outline particle idx (body-local), in outline order
+  std::vector<double> hfcx, hfcy;          // contact forces given with the bodies (idx order), or empty
+  bool fc_active = false;                  // kernels read fcx / fcy (set once forces can be non-zero)
+  double *fcx = nullptr, *fcy = nullptr;   // contact forces by global idx
+  uint32_t* inv_gid = nullptr;             // global idx -> storage index (refreshed by the boundary update)
+  std::vector<uint32_t> bnd_off;           // first outline point of each body (S + 1)
+  uint32_t bnd_cap = 0;                    // outline points the device arrays hold
+  bool bnd_dirty = false;                  // outlines changed since the last upload
+  uint32_t* bnd_gid = nullptr;             // outline point -> global idx
+  uint2* bnd_rng = nullptr;                // outline point -> (first point of its body, point count)
+  double4* bnd_pts = nullptr;              // outline points: x, y, nx, ny
+
   TlState tl;  // Total-Lagrangian frozen lists (method TL)
   mfree::RefPipe* ref = nullptr;  // pipeline MFREE_PIPELINE_REFERENCE
   mfree::VtkState* vtk = nullptr;  // VTK writer (created by the first write)
@@ -168,6 +181,13 @@ struct RkPlan {
   std::function<void(UlArgs&)> stage1;
 };
 void enqueue_ul_rk(mfree_ctx* c, const RkPlan& p, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs);
+
+// Penalty contact (contact.hip): device arrays at commit, and the switch
+// that makes the stepping kernels read the contact forces (drops captured
+// graphs, whose kernel arguments do not read them).
+mfree_status contact_alloc(mfree_ctx* c);
+mfree_status contact_activate(mfree_ctx* c);
+void drop_graphs(mfree_ctx* c);
 
 // Reference-structured FP64 pipeline (baseline.hip).
 mfree_status ref_alloc(mfree_ctx* c);

@@ -30,6 +30,7 @@ struct TlShared {
   uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
   uint32_t* err;
   double h_search;  // <= 0: per-particle h
+  const double *fcx, *fcy;  // contact forces by global idx (= storage index), or null
 };
 
 // Action lists of TL bodies: tensile_tl (Randles-Libersky), tensile_corot

@@ -59,6 +59,7 @@ struct TlArgs {
   double *part8, *diag;
   double2* rot;         // co-rotated: (cos t, sin t) per particle, written by pass 1
   const uint32_t* rev;  // potential: reverse pair of each pair (~0u: none)
+  const double *fcx, *fcy;  // contact forces (idx order), or null
   int stage;
   Fields8 y, K, acc;
   double* d8;
@@ -450,8 +451,8 @@ __device__ inline void tl_pass2_forces(const TlArgs& a, const SceneConst* __rest
       vy_t += 0.5 * (fji1 - fij1) / mi;
     }
   }
-  vx_t += 0. / ri.m;
-  vy_t += 0. / ri.m;
+  vx_t += (a.fcx ? a.fcx[i] : 0.) / ri.m;
+  vy_t += (a.fcy ? a.fcy[i] : 0.) / ri.m;
   // correctors_mghn_artificial_viscosity
   const double alpha = c->alpha, beta = c->beta, eta = c->eta;
 #pragma unroll 4
@@ -531,8 +532,8 @@ __global__ __launch_bounds__(kBlock) void k_tl_pass2(TlArgs a) {
   d[0] = 0. + vxi;  // contmech_advection
   d[1] = 0. + vyi;
   if constexpr (V == kTlRl) {
-    d[2] = ax.val() + (1. / rho0 * (pxx_x + pyx_y) + 0. / a.m[i]);  // contmech_momentum_tl
-    d[3] = ay.val() + (1. / rho0 * (pxy_x + pyy_y) + 0. / a.m[i]);
+    d[2] = ax.val() + (1. / rho0 * (pxx_x + pyx_y) + (a.fcx ? a.fcx[i] : 0.) / a.m[i]);  // contmech_momentum_tl
+    d[3] = ay.val() + (1. / rho0 * (pxy_x + pyy_y) + (a.fcy ? a.fcy[i] : 0.) / a.m[i]);
   } else {
     d[2] = vx_t;
     d[3] = vy_t;
@@ -821,6 +822,8 @@ TlArgs tl_args(TlState& t, const TlShared& s, Fields8 src, Fields8 dst, int stag
   a.diag = t.diag;
   a.rot = t.rot;
   a.rev = t.rev;
+  a.fcx = s.fcx;
+  a.fcy = s.fcy;
   a.stage = stage;
   a.y = s.y;
   a.K = dst;

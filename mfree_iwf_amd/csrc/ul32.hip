@@ -188,6 +188,11 @@ __device__ inline void finish(const UlArgs& a, const SceneConst* __restrict__ c,
   d[1] = acc_val(S.ys) + vyi;
   d[2] = acc_val(S.ax) + 1. / rhoi * (dSxx_x + dSxy_y);
   d[3] = acc_val(S.ay) + 1. / rhoi * (dSxy_x + dSyy_y);
+  if (a.fcx) {  // contact forces (src/cont_mech.h:44-45)
+    const uint32_t g = a.gid[home];
+    d[2] = acc_val(S.ax) + (1. / rhoi * (dSxx_x + dSxy_y) + a.fcx[g] / a.m[home]);
+    d[3] = acc_val(S.ay) + (1. / rhoi * (dSxy_x + dSyy_y) + a.fcy[g] / a.m[home]);
+  }
   d[4] = -rhoi * (gvx_x + gvy_y);
   d[5] = t00;
   d[6] = t01;
@@ -595,6 +600,11 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
   d[1] = v[11] + vyi;
   d[2] = v[8] + 1. / rhoi * (dSxx_x + dSxy_y);
   d[3] = v[9] + 1. / rhoi * (dSxy_x + dSyy_y);
+  if (a.fcx) {  // contact forces (src/cont_mech.h:44-45)
+    const uint32_t g = a.gid[i];
+    d[2] = v[8] + (1. / rhoi * (dSxx_x + dSxy_y) + a.fcx[g] / a.m[i]);
+    d[3] = v[9] + (1. / rhoi * (dSxy_x + dSyy_y) + a.fcy[g] / a.m[i]);
+  }
   d[4] = -rhoi * (v[4] + v[7]);
   d[5] = t00;
   d[6] = t01;

@@ -135,6 +135,45 @@ class Solver:
         self._chk(self._L.mfree_energy(self._h, body, e.ctypes.data_as(_capi.dptr)), "mfree_energy")
         return float(e[0]), float(e[1]), float(e[2])
 
+    # -- penalty contact between bodies (src/contact.h)
+    def set_boundary(self, body, idx):
+        """The body's outline: particle idx in order around it (the drivers
+        take the convex hull, :func:`convex_hull`)."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        self._chk(self._L.mfree_set_boundary(self._h, body, idx.ctypes.data_as(_capi.u32ptr), len(idx)),
+                  "mfree_set_boundary")
+
+    def update_boundaries(self):
+        self._chk(self._L.mfree_update_boundaries(self._h), "mfree_update_boundaries")
+
+    def boundary(self, body=0):
+        """(n, 4) array of the body's outline points: x, y, nx, ny."""
+        n = C.c_uint32(0)
+        self._chk(self._L.mfree_get_boundary(self._h, body, None, 0, C.byref(n)), "mfree_get_boundary")
+        pts = np.zeros((n.value, 4))
+        self._chk(self._L.mfree_get_boundary(self._h, body, pts.ctypes.data_as(_capi.dptr), n.value, C.byref(n)),
+                  "mfree_get_boundary")
+        return pts
+
+    def contact(self, master_body, slave_body, master=None):
+        """contact_apply_body_to_boundary(master, slave); master is another
+        Solver (on the same device) or this one."""
+        m = self if master is None else master
+        self._chk(self._L.mfree_contact(m._h, master_body, self._h, slave_body), "mfree_contact")
+
+    def contact_all(self):
+        self._chk(self._L.mfree_contact_all(self._h), "mfree_contact_all")
+
+    def contact_forces(self, body=0):
+        """(fcx, fcy) of the body's particles, idx order."""
+        n = self.sizes[body]
+        fx, fy = np.zeros(n), np.zeros(n)
+        p = _capi.Particles()
+        p.fcx = fx.ctypes.data_as(_capi.dptr)
+        p.fcy = fy.ctypes.data_as(_capi.dptr)
+        self._chk(self._L.mfree_get_state(self._h, body, C.byref(p)), "mfree_get_state")
+        return fx, fy
+
     # -- parity helpers
     def derivatives(self, with_aux=True):
         n = self.n_total
@@ -181,3 +220,15 @@ class Solver:
 
 def device_count():
     return int(_capi.lib().mfree_device_count())
+
+
+def convex_hull(x, y):
+    """convex_hull (src/convex_hull.h:62-95) by the library's host code: the
+    idx of the hull of the points, counter-clockwise from the lowest-leftmost
+    point, collinear points kept."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(2 * len(x) + 1, dtype=np.uint32)
+    k = _capi.lib().mfree_convex_hull(x.ctypes.data_as(_capi.dptr), y.ctypes.data_as(_capi.dptr), len(x),
+                                      out.ctypes.data_as(_capi.u32ptr))
+    return out[:k].copy()

@@ -271,10 +271,10 @@ class TL:
         rc = lib().orc_tl_derivatives(C.byref(c), self.h, self.n, C.byref(s), time, _p(d8), _p(aux))
         return rc, d8, aux, q
 
-    def step(self, ps, params, nsteps, time=0.0, threads=1, dt=None):
+    def step(self, ps, params, nsteps, time=0.0, threads=1, dt=None, fc=None):
         q = ps.copy()
         c = consts_from(params)
-        s = soa_from(q)
+        s = soa_from(q, fc)
         t = C.c_double(time)
         rc = lib().orc_tl_step(C.byref(c), self.h, self.n, C.byref(s), params.dt if dt is None else dt, nsteps,
                                C.byref(t), threads)

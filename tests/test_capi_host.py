@@ -61,6 +61,20 @@ def test_cfl_time_step():
     assert abs(s.params.dt - 0.3 * 1.3e-3 / (c0 + 59.0)) < 1e-18
 
 
+def test_convex_hull_equals_the_oracle():
+    # the library's host hull (the drivers' outline) against the oracle's,
+    # which is pinned to the reference's own convex_hull.h
+    import _oracle as O
+    from mfree_iwf_amd import convex_hull
+    rng = np.random.default_rng(2)
+    g = np.arange(11, dtype=np.float64)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    for s in scenes.reference_rings_contact(41) + scenes.reference_rings_contact(21, "potential"):
+        assert np.array_equal(convex_hull(s.particles.x, s.particles.y), O.convex_hull(s.particles.x, s.particles.y))
+    for x, y in ((X.ravel(), Y.ravel()), (rng.uniform(size=300), rng.uniform(size=300))):
+        assert np.array_equal(convex_hull(x, y), O.convex_hull(x, y))
+
+
 def test_godunov_is_fp64_fused_only():
     # the Godunov pair sums exist in FP64 on the fused pipeline; other
     # combinations are refused at creation (no device needed)
