@@ -87,9 +87,20 @@ class body {
 
   void add_action(void (*action)(body<T>& b)) { m_actions.push_back(action); }
 
-  // src/body.h:196-212: the outline used by the penalty contact
-  void set_boundary(boundary<T>* b) { m_boundary = b; }
-  boundary<T>* get_boundary() const { return m_boundary; }
+  // src/body.h:196-212: the outline used by the penalty contact.  Its
+  // points and normals are kept on the GPU (update_boundary); the host
+  // object is refreshed from the particles when it is read.
+  void set_boundary(boundary<T>* b) {
+    m_boundary = b;
+    *m_bnd_sent = false;
+  }
+  boundary<T>* get_boundary() const {
+    if (m_boundary && *m_bnd_stale) {
+      m_boundary->update(get_particles());
+      *m_bnd_stale = false;
+    }
+    return m_boundary;
+  }
 
   // src/body.h:223-225 -> src/rk4.h:71-108
   void step() {
@@ -108,10 +119,24 @@ class body {
   unsigned int get_num_part() const { return m_num_part; }
   simulation_data get_sim_data() const { return m_data; }
   bool has_boundary() const { return m_boundary != nullptr; }
+  // src/body.h:152-165: the outline follows the particles (on the GPU)
   void update_boundary() {
-    if (m_boundary) m_boundary->update(get_particles());
+    if (!m_boundary) return;
+    if (!m_gpu) setup();
+    if (!*m_bnd_sent) {
+      const std::vector<unsigned int>& ix = m_boundary->indexes();
+      std::vector<uint32_t> idx(ix.begin(), ix.end());
+      if (mfree_set_boundary(m_gpu->ctx, 0, idx.data(), (uint32_t)idx.size()) != MFREE_OK)
+        mfree_adapter::die(std::string("set_boundary: ") + mfree_last_error(m_gpu->ctx));
+      *m_bnd_sent = true;
+    }
+    if (mfree_update_boundaries(m_gpu->ctx) != MFREE_OK)
+      mfree_adapter::die(std::string("update_boundary: ") + mfree_last_error(m_gpu->ctx));
+    *m_bnd_stale = true;
   }
   mfree_ctx* context() const { return m_gpu ? m_gpu->ctx : nullptr; }
+  // the GPU changed the body (contact forces): host particles are re-read on access
+  void mark_stale() { *m_stale = true; }
 
   // The library method the registered actions, particle type and boundary
   // conditions map to (MFREE_METHOD_*), or -1 when step() would refuse the
@@ -145,6 +170,8 @@ class body {
   boundary<T>* m_boundary = nullptr;
   std::shared_ptr<mfree_adapter::Gpu> m_gpu;
   std::shared_ptr<bool> m_stale = std::make_shared<bool>(false);
+  std::shared_ptr<bool> m_bnd_sent = std::make_shared<bool>(false);   // outline indexes given to the library
+  std::shared_ptr<bool> m_bnd_stale = std::make_shared<bool>(false);  // host outline behind the GPU's
 
   static bool tl(int32_t method) {
     return method == MFREE_METHOD_TL_RL || method == MFREE_METHOD_TL_COROTATIONAL ||
@@ -322,10 +349,10 @@ class body {
   void sync() const {
     if (!m_gpu || !*m_stale) return;
     const unsigned int n = m_num_part;
-    std::vector<double> f[8];
+    std::vector<double> f[10];
     for (auto& v : f) v.resize(n);
-    mfree_particles hp = {f[0].data(), f[1].data(), nullptr, nullptr, nullptr, f[4].data(), nullptr,
-                          f[2].data(), f[3].data(), f[5].data(), f[6].data(), f[7].data(), nullptr, nullptr};
+    mfree_particles hp = {f[0].data(), f[1].data(), nullptr,     nullptr,     nullptr,     f[4].data(), nullptr,
+                          f[2].data(), f[3].data(), f[5].data(), f[6].data(), f[7].data(), f[8].data(), f[9].data()};
     if (mfree_get_state(m_gpu->ctx, 0, &hp) != MFREE_OK)
       mfree_adapter::die(std::string("get_state: ") + mfree_last_error(m_gpu->ctx));
     for (unsigned int i = 0; i < n; i++) {
@@ -338,6 +365,8 @@ class body {
       p->Sxx = f[5][i];
       p->Sxy = f[6][i];
       p->Syy = f[7][i];
+      p->fcx = f[8][i];
+      p->fcy = f[9][i];
     }
     *m_stale = false;
   }

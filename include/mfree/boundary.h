@@ -38,14 +38,16 @@ class boundary {
     m_ny.assign(n, 0.);
     for (size_t k = 0; k < n && n > 1; k++) {
       const size_t l = (k + n - 1) % n, r = (k + 1) % n;
-      // counter-clockwise outline: the outward normal of edge a -> b is (dy, -dx)
-      double ax = m_y[k] - m_y[l], ay = -(m_x[k] - m_x[l]);
-      double bx = m_y[r] - m_y[k], by = -(m_x[r] - m_x[k]);
-      const double la = sqrt(ax * ax + ay * ay), lb = sqrt(bx * bx + by * by);
-      if (la > 0.) ax /= la, ay /= la;
-      if (lb > 0.) bx /= lb, by /= lb;
-      m_nx[k] = 0.5 * (ax + bx);
-      m_ny[k] = 0.5 * (ay + by);
+      // counter-clockwise outline: the outward normal of the edge from the
+      // left neighbour is (-tl.y, tl.x) with tl = left - p, that of the edge to
+      // the right neighbour (tr.y, -tr.x) with tr = right - p; each scaled by
+      // 1/sqrt(|v|^2) (glm::normalize, as the GPU does)
+      const double tlx = m_x[l] - m_x[k], tly = m_y[l] - m_y[k];
+      const double trx = m_x[r] - m_x[k], try_ = m_y[r] - m_y[k];
+      const double ax = -tly, ay = tlx, bx = try_, by = -trx;
+      const double ia = 1. / sqrt(ax * ax + ay * ay), ib = 1. / sqrt(bx * bx + by * by);
+      m_nx[k] = 0.5 * (ax * ia + bx * ib);
+      m_ny[k] = 0.5 * (ay * ia + by * ib);
     }
   }
 

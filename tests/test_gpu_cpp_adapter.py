@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+from _contact import OracleBodies
 from mfree_iwf_amd import Params, scenes
 from mfree_iwf_amd.scenes import ParticleSet
 
@@ -24,6 +25,42 @@ def run_driver(*args):
     n, t = lines[0].split()
     d = np.array([list(map(float, ln.split())) for ln in lines[1:]])
     return int(n), float(t), d
+
+
+def run_driver_bodies(*args):
+    """The driver's output for every body: [(n, time, rows)]."""
+    assert os.path.exists(EXE), "build tests/cpp first (make -C tests/cpp)"
+    print(f"[cpp driver] {' '.join(args)}", flush=True)
+    out = subprocess.run([EXE, *args], check=True, capture_output=True, text=True, timeout=120).stdout
+    lines = out.strip().splitlines()
+    bodies, k = [], 0
+    while k < len(lines):
+        n, t = lines[k].split()
+        n = int(n)
+        bodies.append((n, float(t), np.array([list(map(float, ln.split())) for ln in lines[k + 1:k + 1 + n]])))
+        k += 1 + n
+    return bodies
+
+
+def test_rings_contact_driver_main_loop_vs_oracle():
+    # rings_monaghan_contact through the reference main loop's cadence in
+    # C++ (step + update_boundary per body, contact_apply_body_to_boundary
+    # for every ordered pair) against the oracle's loop
+    out = run_driver_bodies("rings_contact", "41", "30")
+    orc = OracleBodies(scenes.reference_rings_contact(41), "monaghan")
+    for _ in range(30):
+        orc.step()
+    assert len(out) == 2
+    for k, (n, t, d) in enumerate(out):
+        o = orc.ps[k]
+        assert n == o.n
+        ep = O.rel_l2(np.concatenate([d[:, 1], d[:, 2]]), np.concatenate([o.x, o.y]))
+        ev = O.rel_l2(np.concatenate([d[:, 3], d[:, 4]]), np.concatenate([o.vx, o.vy]))
+        ef = O.rel_l2(np.concatenate([d[:, 9], d[:, 10]]), np.concatenate(orc.fc[k]))
+        assert ep <= 1e-10 and ev <= 1e-10 and ef <= 1e-8, (k, ep, ev, ef)
+        assert np.count_nonzero(d[:, 9]) > 0
+        # every body's step advances the global time (src/rk4.h:107)
+        assert abs(t - 60e-6) < 1e-15
 
 
 def test_rings_monaghan_driver_vs_oracle():
