@@ -462,6 +462,88 @@ def bench_godunov(steps, warmup, log, cpu_steps=1, nbox=1213):
     return n, out
 
 
+def bench_contact(steps, warmup, log, nbox=1213):
+    """The two-ring contact driver (rings_monaghan_contact,
+    src/benchmarks/rings.cpp:522-620) as two bodies of 504,602 particles
+    each (nbox = 1213, dx = 6.6e-5 m), mixed precision, stepped with the
+    reference main loop's cadence: one RK4 step of both bodies, the
+    outline update, the penalty contact of both ordered pairs
+    (src/mfree_iwf.cpp:84-97).  The driver's dt (1e-6 s, sized for dx =
+    1e-3 m) is replaced by the CFL-0.3 step of this resolution, and the
+    rings start overlapping by one dx, closing at 360 m/s: the timed steps
+    run in contact.  Reports the loop's rate and the time of the outline update
+    and contact kernels per step (the loop's time less that of plain
+    steps)."""
+    from mfree_iwf_amd import PRECISION_MIXED, Solver, convex_hull, scenes
+    t0 = time.perf_counter()
+    dx = 0.08 / (nbox - 1)
+    dt = scenes.cfl_dt(0.3, 1.7 * dx, 1e7, 0.4, 1.0, 180.0)
+    bodies = scenes.reference_rings_contact(nbox, dt=dt)
+    gap = bodies[1].particles.x.min() - bodies[0].particles.x.max()
+    for k, b in enumerate(bodies):
+        b.particles.x += (0.5 * (gap + b.dx)) * (1 if k == 0 else -1)
+    n = sum(b.particles.n for b in bodies)
+    hulls = [convex_hull(b.particles.x, b.particles.y) for b in bodies]
+    log(f"contact: 2 bodies, {n} particles, outlines {len(hulls[0])} + {len(hulls[1])} points "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    with Solver(precision=PRECISION_MIXED) as sv:
+        for k, b in enumerate(bodies):
+            sv.add_body(b.particles, b.params)
+            sv.set_boundary(k, hulls[k])
+
+        def loop(k):
+            for _ in range(k):
+                sv.step(1)
+                sv.update_boundaries()
+                sv.contact_all()
+        loop(max(warmup, 1))
+        sv.state(0)  # synchronise
+        t0 = time.perf_counter()
+        loop(steps)
+        sv.state(0)
+        wall = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sv.step(steps)
+        sv.state(0)
+        wall_step = time.perf_counter() - t0
+        fx, _ = sv.contact_forces(0)
+    out = dict(particles=n, particle_steps_per_s=n * steps / wall, ms_per_step=wall / steps * 1e3,
+               ms_per_step_without_contact=wall_step / steps * 1e3,
+               contact_ms_per_step=(wall - wall_step) / steps * 1e3,
+               outline_points=[int(len(h)) for h in hulls], particles_in_contact=int(np.count_nonzero(fx)))
+    log(f"  {out['particle_steps_per_s']:.4e} particle-steps/s ({out['ms_per_step']:.3f} ms/step, contact "
+        f"{out['contact_ms_per_step']:.3f} ms), {out['particles_in_contact']} particles of body 0 in contact")
+    return n, out
+
+
+def bench_tl_variants(steps, warmup, log):
+    """The co-rotated and potential TL action lists (tensile_corot,
+    tensile_potential) on the tension plate with a hole (0.1 x 0.04 m,
+    r = 0.01 m, dx = 8e-5 m, 577,695 particles), FP64.  The potential list's
+    forces are the gradient of its potential divided by a particle volume,
+    which at this resolution (V = 6.4e-9 m^2) needs a time step far below
+    the CFL step: it runs at 1e-6 of it (the rate does not depend on dt)."""
+    from mfree_iwf_amd import METHOD_TL_COROTATIONAL, METHOD_TL_POTENTIAL, Solver, scenes
+    out = {}
+    for name, method in (("corot", METHOD_TL_COROTATIONAL), ("potential", METHOD_TL_POTENTIAL)):
+        pl = scenes.plate_config3()
+        if method == METHOD_TL_POTENTIAL:
+            pl.params.dt = pl.params.dt * 1e-6
+        n = pl.particles.n
+        with Solver(method=method) as sv:
+            sv.add_body(pl.particles, pl.params)
+            sv.set_strip_bc(0, pl.left, pl.right, pl.v_bc)
+            sv.step(max(warmup, 1))
+            sv.profile(True)
+            sv.step(steps)
+            prof = sv.profile_read()
+            sv.profile(False)
+        out[name] = dict(particles=n, particle_steps_per_s=n * steps / (prof["step_ms"] / 1e3),
+                         ms_per_step=prof["step_ms"] / steps)
+        log(f"  {name}: {out[name]['particle_steps_per_s']:.4e} particle-steps/s ({out[name]['ms_per_step']:.3f} ms/step)")
+    return out
+
+
 def traffic_from_profiles(kernel, particles):
     """The committed rocprof traffic figure of this kernel at this size, or None."""
     p = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
@@ -503,7 +585,8 @@ def main():
     ap.add_argument("--sur-threshold", type=float, default=1e-4)
     ap.add_argument("--sur-steps", type=int, default=0,
                     help="rollout length (default: the accuracy horizon, --accuracy-steps)")
-    ap.add_argument("--workload", default="rings1m", choices=["rings1m", "batch256", "plate578k", "godunov1m"])
+    ap.add_argument("--workload", default="rings1m",
+                    choices=["rings1m", "batch256", "plate578k", "godunov1m", "contact1m", "tl_variants578k"])
     ap.add_argument("--validate-fp64-steps", type=int, default=0,
                     help="only run the FP64-vs-oracle gate over this many steps and print its JSON")
     ap.add_argument("--no-extra-workloads", action="store_true",
@@ -556,6 +639,23 @@ def main():
             "config": {"workload": f"tension plate 0.1 x 0.04 m, hole r = 0.01 m, dx = 8e-5 m, {n} particles, "
                                    "strips 3 dx at +-1 m/s, mixed precision", "particles": n, "mode": "mixed"},
             "accuracy": acc, "modes": out}), flush=True)
+        return
+    if args.workload in ("contact1m", "tl_variants578k"):
+        if args.workload == "contact1m":
+            n, out = bench_contact(min(args.steps, 20), min(args.warmup, 3), log)
+            head, metric, cfg = out, "particle-steps/s, two-ring contact (penalty contact between bodies)", dict(
+                workload=f"rings_monaghan_contact scene, 2 bodies, {n} particles, mixed precision, main-loop "
+                         "cadence (step, outline update, contact)", particles=n, mode="mixed")
+        else:
+            out = bench_tl_variants(min(args.steps, 20), min(args.warmup, 3), log)
+            head, metric, cfg = out["corot"], "particle-steps/s, co-rotated TL tension plate", dict(
+                workload="tension plate 0.1 x 0.04 m, hole r = 0.01 m, dx = 8e-5 m, co-rotated and potential "
+                         "TL lists, FP64", particles=out["corot"]["particles"], mode="fp64")
+        print(json.dumps({"metric": metric, "value": head["particle_steps_per_s"], "unit": "particle-steps/s",
+                          "n_gpus": 1, "steps": min(args.steps, 20), "warmup": min(args.warmup, 3),
+                          "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32+f64" if args.workload == "contact1m" else "f64",
+                          "data": "synthetic", "config": cfg, "modes": out}), flush=True)
         return
     if args.workload == "godunov1m":
         n, out = bench_godunov(min(args.steps, 20), min(args.warmup, 3), log,
@@ -658,6 +758,11 @@ def main():
         log("workload godunov1m")
         ngd, gd = bench_godunov(20, 3, log, cpu_steps=0 if args.no_cpu_baseline else 1)
         extra["godunov1m"] = dict(particles=ngd, unit="particle-steps/s", **gd)
+        log("workload contact1m")
+        _, ct = bench_contact(20, 3, log)
+        extra["contact1m"] = dict(unit="particle-steps/s", **ct)
+        log("workload tl_variants578k")
+        extra["tl_variants578k"] = dict(unit="particle-steps/s", **bench_tl_variants(20, 3, log))
 
     line = {
         "metric": METRIC,

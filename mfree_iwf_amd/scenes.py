@@ -156,7 +156,7 @@ def reference_rings_godunov(nbox, **kw):
                       eta=0.0, art_eps=0.0, corr_exp=0.0, xsph_eps=0.0, **kw)
 
 
-def reference_rings_contact(nbox, kind="monaghan"):
+def reference_rings_contact(nbox, kind="monaghan", dt=None):
     """The two-body contact drivers' scenes (src/benchmarks/rings.cpp:
     rings_monaghan_contact :522-620, rings_godunov_contact :164-243,
     rings_corot_contact :245-335, rings_potential_contact :431-520): the
@@ -165,11 +165,12 @@ def reference_rings_contact(nbox, kind="monaghan"):
     (1.3 dx and rings of radii 0.75 / 1.0 m for the potential driver);
     dt = 1e-6 s for Monaghan, 1e-7 s otherwise; Monaghan's corrections for
     Monaghan bodies, viscosity (1, 1, 0.1) for the TL bodies, none for
-    Godunov.  Returns [left, right]; the drivers' outline of each is the
-    convex hull of its particles."""
+    Godunov.  dt overrides the driver's time step (the drivers' steps are
+    sized for nbox = 80).  Returns [left, right]; the drivers' outline of
+    each is the convex hull of its particles."""
     ri, ro, hdx = (0.75, 1.0, 1.3) if kind == "potential" else (0.03, 0.04, 1.7)
     kw = dict(ri=ri, ro=ro, centre=ro + 0.001, hdx=hdx, rho0=1.0, v0=180.0, E=1e7, nu=0.4,
-              dt=1e-6 if kind == "monaghan" else 1e-7)
+              dt=dt if dt is not None else (1e-6 if kind == "monaghan" else 1e-7))
     if kind == "godunov":
         kw.update(alpha=0.0, beta=0.0, eta=0.0, art_eps=0.0, corr_exp=0.0, xsph_eps=0.0)
     elif kind in ("corot", "potential"):
