@@ -20,11 +20,14 @@ def write_host(path, particle_sets):
                 what="mfree_vtk_write_host")
 
 
-def run(solver, nsteps, freq=10, outdir="results", out=sys.stdout, body=0):
+def run(solver, nsteps, freq=10, outdir="results", out=sys.stdout, body=0, contact=False):
     """The reference's main loop for nsteps steps: after step s (counted from
     0) the state is written to outdir/out_%06d.vtk when s % freq == 0, and
-    "s: time" is printed.  Files are written asynchronously while the next
-    steps run; returns the paths written."""
+    "s: time" is printed.  With contact, every step is followed by the
+    outline update and the penalty contact of every ordered pair of bodies
+    (src/mfree_iwf.cpp:84-97; the outlines must have been set).  Files are
+    written asynchronously while the next steps run; returns the paths
+    written."""
     os.makedirs(outdir, exist_ok=True)
     written = []
     t = solver.time(body)
@@ -33,7 +36,13 @@ def run(solver, nsteps, freq=10, outdir="results", out=sys.stdout, body=0):
     while s < nsteps:
         # steps up to the next write (or the end) in one call
         nxt = s if s % freq == 0 else min(nsteps - 1, (s // freq + 1) * freq)
-        solver.step(nxt - s + 1)
+        if contact:
+            for _ in range(nxt - s + 1):
+                solver.step(1)
+                solver.update_boundaries()
+                solver.contact_all()
+        else:
+            solver.step(nxt - s + 1)
         for k in range(s, nxt + 1):
             t = t + dt
             if out is not None:
