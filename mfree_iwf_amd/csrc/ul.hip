@@ -268,11 +268,14 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
       // single precision (its entries are R, S rounded to float); sigma has
       // S_yy without the pressure; SRijstar[1] reads sigma_j twice -- all as
       // the reference computes them.
-      const double Rx = pj.x - pi.x;
-      const double Ry = pj.y - pi.y;
+      // R = x_j - x_i = -x_ij (IEEE subtraction is antisymmetric), so
+      // |R|^2 = |S|^2 = r2 and the reference's three square roots of it and
+      // its two 1/sqrt are rij and one reciprocal: the same values
+      const double Rx = -xij;
+      const double Ry = -yij;
       const double Sx = Ry, Sy = -Rx;
-      const double iR = 1. / sqrt(Rx * Rx + Ry * Ry);
-      const double iS = 1. / sqrt(Sx * Sx + Sy * Sy);
+      const double iR = 1. / rij;
+      const double iS = iR;
       const double Rnx = Rx * iR, Rny = Ry * iR;
       const double Snx = Sx * iS, Sny = Sy * iS;
       const double l00 = (double)(float)Rnx, l01 = (double)(float)Rny;
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs64(UlArgs a) {
       // glm::transpose(l) * SR: [r] = l[r][0] SR0 + l[r][1] SR1
       const double SRij0 = l00 * SR0 + l01 * SR1;
       const double SRij1 = l10 * SR0 + l11 * SR1;
-      const double mag_r = sqrt(Rx * Rx + Ry * Ry);
+      const double mag_r = rij;
       if (fabs(Rx) > 0.) ax += 2. / ri.rho * wx * mag_r / Rx * SRij0 * rj.qw;
       if (fabs(Ry) > 0.) ay += 2. / ri.rho * wy * mag_r / Ry * SRij1 * rj.qw;
       return;

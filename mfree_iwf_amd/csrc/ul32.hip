@@ -587,11 +587,18 @@ constexpr uint32_t kListMask = 0x0FFFFFFFu;
 // index i: the stage-1 prepass moved storage into sorted order), the RK4
 // stage update, and the record of the next stage.  The L lanes of the
 // particle hold the same sums; lane q stores the fields f with f % L == q.
+template <class T>
+__device__ inline T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 template <int L>
 __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict__ c, const SceneGeom& g,
                                    uint32_t key, uint32_t i, uint32_t q, const double* v) {
-  const double vxi = a.src.f[2][i], vyi = a.src.f[3][i], rhoi = a.src.f[4][i];
-  const double Sxx = a.src.f[5][i], Sxy = a.src.f[6][i], Syy = a.src.f[7][i];
+  // state read and written once per stage: streamed past the caches so that
+  // they keep the positions the neighbour gathers re-read
+  const double vxi = ld_nt(&a.src.f[2][i]), vyi = ld_nt(&a.src.f[3][i]), rhoi = ld_nt(&a.src.f[4][i]);
+  const double Sxx = ld_nt(&a.src.f[5][i]), Sxy = ld_nt(&a.src.f[6][i]), Syy = ld_nt(&a.src.f[7][i]);
   const double dSxx_x = v[0] * rhoi, dSxy_y = v[1] * rhoi, dSxy_x = v[2] * rhoi, dSyy_y = v[3] * rhoi;
   double t00, t01, t11;
   jaumann64(c->G2, v[4], v[5], v[6], v[7], Sxx, Sxy, Syy, t00, t01, t11);
@@ -623,10 +630,10 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
   double kn[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    kn[k] = a.y.f[k][i] + d[k] * dts;
+    kn[k] = ld_nt(&a.y.f[k][i]) + d[k] * dts;
     if (k % L == (int)q) {
-      a.acc.f[k][i] = a.stage == 1 ? d[k] : a.acc.f[k][i] + 2 * d[k];
-      if (k >= 2) a.K.f[k][i] = kn[k];  // the positions of a stage live in its records
+      st_nt(&a.acc.f[k][i], a.stage == 1 ? d[k] : ld_nt(&a.acc.f[k][i]) + 2 * d[k]);
+      if (k >= 2) st_nt(&a.K.f[k][i], kn[k]);  // the positions of a stage live in its records
     }
   }
   const double ux = d[0] * dts, uy = d[1] * dts, half = 0.5 * g.skin;
@@ -685,7 +692,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_list32(UlArgs a) {
 #pragma unroll
       for (int t = 0; t < kScanBatch; t++) {
         const uint32_t k = k0 + t * L;
-        e[t] = col[(size_t)(k < end ? k : base) * a.n];
+        e[t] = ld_nt(&col[(size_t)(k < end ? k : base) * a.n]);  // each list entry is read once
       }
 #pragma unroll
       for (int t = 0; t < kScanBatch; t++) pj[t] = a.pos32[e[t] & kListMask];
