@@ -128,6 +128,13 @@ typedef struct mfree_options {
                                mfree_body_status reports the body's error; mfree_set_state on the
                                body thaws it.  0 (default): the first failure fails the step, as
                                the reference's assertions stop its run */
+  int32_t list_staging;     /* neighbour-list stages with one lane per particle: 0 = default (on),
+                               1 = on: each workgroup of 256 sorted particles copies the three
+                               contiguous storage runs that hold its candidates into LDS and
+                               evaluates its lists from there (workgroups whose runs exceed 1024
+                               particles or that span two bodies gather from global memory);
+                               -1 = off: every neighbour gathered from global memory.  Both give
+                               the same sums in the same order */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y

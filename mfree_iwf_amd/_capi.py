@@ -50,7 +50,8 @@ class Options(C.Structure):
                 ("max_nbh", C.c_uint32), ("use_graph", C.c_int32), ("deterministic", C.c_int32),
                 ("tl_h_search", C.c_double), ("tile_cells", C.c_int32), ("pipeline", C.c_int32),
                 ("xcd_chunk", C.c_uint32), ("cell_scale", C.c_double), ("neighbor_list", C.c_int32),
-                ("list_lanes", C.c_int32), ("isolate_bodies", C.c_int32)]
+                ("list_lanes", C.c_int32), ("isolate_bodies", C.c_int32),
+                ("list_staging", C.c_int32)]
 
 
 PARTICLE_FIELDS = ("x", "y", "X", "Y", "m", "rho", "h", "vx", "vy", "Sxx", "Sxy", "Syy", "fcx", "fcy")

@@ -182,6 +182,7 @@ mfree_status commit(mfree_ctx* c) {
       if (c->use_list) {
         c->nl_cap = kListCap;
         c->list_lanes = c->opt.list_lanes ? c->opt.list_lanes : 1;
+        c->list_staged = c->opt.list_staging >= 0;
         HIPCHK(c, dalloc(c, &c->nl_col, (size_t)kListCap * N));
         HIPCHK(c, dalloc(c, &c->nl_deg, N));
         HIPCHK(c, dalloc(c, &c->pos32b, N));
@@ -566,7 +567,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
     if (k == 1)
       launch_ul_list_first(st, p);
     else
-      launch_ul_list(st, p, c->list_lanes);
+      launch_ul_list(st, p, c->list_lanes, c->list_staged);
     if (evs && e0 && e1) {
       hipEventRecord(e1, st);
       evs->push_back({e0, e1});
@@ -754,6 +755,7 @@ void mfree_default_options(mfree_options* o) {
   o->cell_scale = 0.;
   o->neighbor_list = 0;
   o->list_lanes = 0;
+  o->list_staging = 0;
   o->isolate_bodies = 0;
 }
 
@@ -790,7 +792,7 @@ mfree_status mfree_create(const mfree_options* opt, mfree_ctx** out) {
     delete c;
     return MFREE_ERR_ARG;
   }
-  if (c->opt.neighbor_list < -1 || c->opt.neighbor_list > 1 ||
+  if (c->opt.neighbor_list < -1 || c->opt.neighbor_list > 1 || c->opt.list_staging < -1 || c->opt.list_staging > 1 ||
       !(c->opt.list_lanes == 0 || c->opt.list_lanes == 1 || c->opt.list_lanes == 2 || c->opt.list_lanes == 4)) {
     delete c;
     return MFREE_ERR_ARG;

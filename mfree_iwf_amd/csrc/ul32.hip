@@ -23,7 +23,10 @@
 //  k_ul_pairs32  the same per particle straight from global memory, with the
 //                candidates inside the cutoff compacted in LDS first.
 // The autotuner times both (and several tile sizes) on the actual bodies.
-// Compiled with FMA contraction enabled (FP32 math only in this file).
+// The neighbour-list path (one list per RK4 step) has its own kernels below.
+// Compiled with -ffp-contract=on: a multiply and an add of one expression
+// may fuse, never across statements, so every kernel variant of a pass (LDS
+// staged or not) computes the same bits.
 #include "kernels.h"
 
 namespace mfree {
@@ -592,13 +595,48 @@ __device__ inline T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
 template <class T>
 __device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
+// What the epilogue of sorted particle i reads from HBM: the stage's state,
+// the step's start state (stage 4: the fields of lane q only), the weighted
+// rates so far (stages 2-4, the fields of lane q), mass and smoothing length.
+// Each is read and written once per stage: streamed past the caches so that
+// they keep the positions and records the neighbour loads re-read.
+struct EpiIn {
+  double vx, vy, rho, Sxx, Sxy, Syy;
+  double y[8], acc[8];
+  double m, h;
+};
+
 template <int L>
+__device__ inline EpiIn epi_load(const UlArgs& a, uint32_t i, uint32_t q) {
+  EpiIn e;
+  e.vx = ld_nt(&a.src.f[2][i]);
+  e.vy = ld_nt(&a.src.f[3][i]);
+  e.rho = ld_nt(&a.src.f[4][i]);
+  e.Sxx = ld_nt(&a.src.f[5][i]);
+  e.Sxy = ld_nt(&a.src.f[6][i]);
+  e.Syy = ld_nt(&a.src.f[7][i]);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const bool mine = k % L == (int)q;
+    e.y[k] = a.stage != 4 || mine ? ld_nt(&a.y.f[k][i]) : 0.;
+    e.acc[k] = a.stage != 1 && mine ? ld_nt(&a.acc.f[k][i]) : 0.;
+  }
+  e.m = a.m[i];
+  e.h = a.h[i];
+  return e;
+}
+
+// PRE: the inputs were loaded ahead into e (epi_load); otherwise they are
+// loaded here, next to their use, which keeps fewer registers live.
+template <int L, bool PRE>
 __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict__ c, const SceneGeom& g,
-                                   uint32_t key, uint32_t i, uint32_t q, const double* v) {
-  // state read and written once per stage: streamed past the caches so that
-  // they keep the positions the neighbour gathers re-read
-  const double vxi = ld_nt(&a.src.f[2][i]), vyi = ld_nt(&a.src.f[3][i]), rhoi = ld_nt(&a.src.f[4][i]);
-  const double Sxx = ld_nt(&a.src.f[5][i]), Sxy = ld_nt(&a.src.f[6][i]), Syy = ld_nt(&a.src.f[7][i]);
+                                   uint32_t key, uint32_t i, uint32_t q, const double* v, const EpiIn& e) {
+  const double vxi = PRE ? e.vx : ld_nt(&a.src.f[2][i]), vyi = PRE ? e.vy : ld_nt(&a.src.f[3][i]);
+  const double rhoi = PRE ? e.rho : ld_nt(&a.src.f[4][i]);
+  const double Sxx = PRE ? e.Sxx : ld_nt(&a.src.f[5][i]), Sxy = PRE ? e.Sxy : ld_nt(&a.src.f[6][i]);
+  const double Syy = PRE ? e.Syy : ld_nt(&a.src.f[7][i]);
+  auto y_k = [&](int k) { return PRE ? e.y[k] : ld_nt(&a.y.f[k][i]); };
+  auto acc_k = [&](int k) { return PRE ? e.acc[k] : ld_nt(&a.acc.f[k][i]); };
   const double dSxx_x = v[0] * rhoi, dSxy_y = v[1] * rhoi, dSxy_x = v[2] * rhoi, dSyy_y = v[3] * rhoi;
   double t00, t01, t11;
   jaumann64(c->G2, v[4], v[5], v[6], v[7], Sxx, Sxy, Syy, t00, t01, t11);
@@ -623,16 +661,16 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
   if (a.stage == 4) {
 #pragma unroll
     for (int k = 0; k < 8; k++)
-      if (k % L == (int)q) a.y.f[k][i] = a.y.f[k][i] + c->dt6 * (a.acc.f[k][i] + d[k]);
+      if (k % L == (int)q) st_nt(&a.y.f[k][i], y_k(k) + c->dt6 * (acc_k(k) + d[k]));
     return;
   }
   const double dts = a.stage == 3 ? c->dt : c->half_dt;
   double kn[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    kn[k] = ld_nt(&a.y.f[k][i]) + d[k] * dts;
+    kn[k] = y_k(k) + d[k] * dts;
     if (k % L == (int)q) {
-      st_nt(&a.acc.f[k][i], a.stage == 1 ? d[k] : ld_nt(&a.acc.f[k][i]) + 2 * d[k]);
+      st_nt(&a.acc.f[k][i], a.stage == 1 ? d[k] : acc_k(k) + 2 * d[k]);
       if (k >= 2) st_nt(&a.K.f[k][i], kn[k]);  // the positions of a stage live in its records
     }
   }
@@ -643,7 +681,7 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
   float2 pos;
   PairRec32 r;
   make_rec32(c, g.bbmin_x + (double)ix * g.cell, g.bbmin_y + (double)iy * g.cell, kn[0], kn[1], kn[2], kn[3], kn[4],
-             kn[5], kn[6], kn[7], a.m[i], a.h[i], pos, r);
+             kn[5], kn[6], kn[7], PRE ? e.m : a.m[i], PRE ? e.h : a.h[i], pos, r);
   if (q == 0) a.pos32_out[i] = pos;
   const float4* rv = reinterpret_cast<const float4*>(&r);
   float4* out = reinterpret_cast<float4*>(a.rec32_out + i);
@@ -662,14 +700,10 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
 // FP64 with cross-lane shuffles.
 constexpr int kListBatch = 32;
 
+// Sorted particle i, lane q of L, every neighbour gathered from global
+// memory; my = this thread's compaction buffer (kListBatch words, stride kBlock).
 template <int L>
-__global__ __launch_bounds__(kBlock) void k_ul_list32(UlArgs a) {
-  static_assert(L == 1 || L == 2 || L == 4, "lanes per particle");
-  __shared__ uint32_t nb_lds[kListBatch * kBlock];
-  uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * (kBlock / L) + threadIdx.x / L;
-  const uint32_t q = threadIdx.x % L;
-  if (i >= a.n) return;  // the L lanes of a particle leave together
+__device__ inline void list_stage_global(const UlArgs& a, uint32_t i, uint32_t q, uint32_t* my) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
   const SceneGeom g = a.geom[mt.y];
@@ -728,7 +762,177 @@ __global__ __launch_bounds__(kBlock) void k_ul_list32(UlArgs a) {
   for (int o = 1; o < L; o <<= 1)
 #pragma unroll
     for (int t = 0; t < 12; t++) v[t] += __shfl_xor(v[t], o, 64);
-  finish_list<L>(a, c, g, mt.x, i, q, v);
+  finish_list<L, false>(a, c, g, mt.x, i, q, v, EpiIn{});
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_ul_list32(UlArgs a) {
+  static_assert(L == 1 || L == 2 || L == 4, "lanes per particle");
+  __shared__ uint32_t nb_lds[kListBatch * kBlock];
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * (kBlock / L) + threadIdx.x / L;
+  if (i >= a.n) return;  // the L lanes of a particle leave together
+  list_stage_global<L>(a, i, threadIdx.x % L, nb_lds + threadIdx.x);
+}
+
+// The same stage with the neighbours staged in LDS.  A workgroup takes 256
+// consecutive sorted particles, whose cell keys span [k0, k1] of one body.
+// Every list entry of tag tx (neighbour cell column = own column + 1 - tx)
+// lies in the key range [k0 + (1 - tx) ny - 1, k1 + (1 - tx) ny + 1]: three
+// contiguous runs of sorted storage (bands), at most kBandCap candidates in
+// all for the bodies this is built for (about 3.3 per particle).  The bands'
+// positions and records are copied into LDS once with coalesced loads, and
+// each particle's list is evaluated from LDS: the scattered 64-byte record
+// gathers of the global path become LDS reads.  The pair sums run over the
+// same entries in the same order with the same values, so the result is the
+// global path's bit for bit.  A workgroup whose particles span two bodies or
+// whose bands exceed kBandCap takes the global path.
+constexpr int kBandCap = 1024;  // staged candidates per workgroup (10-bit LDS slot)
+
+struct BandLds {
+  float px[kBandCap], py[kBandCap];
+  float vx[kBandCap], vy[kBandCap], rho[kBandCap], qw[kBandCap], m[kBandCap], h[kBandCap], c[kBandCap];
+  float sxx[kBandCap], sxy[kBandCap], syy[kBandCap], Rxx[kBandCap], Rxy[kBandCap], Ryy[kBandCap];
+};
+static_assert(sizeof(BandLds) >= sizeof(uint32_t) * kListBatch * kBlock, "global path buffer aliases the bands");
+
+__device__ inline Nb nb_from_lds(const BandLds& s, uint32_t k) {
+  Nb n;
+  n.vx = s.vx[k];
+  n.vy = s.vy[k];
+  n.rho = s.rho[k];
+  n.qw = s.qw[k];
+  n.m = s.m[k];
+  n.h = s.h[k];
+  n.c = s.c[k];
+  n.sxx = s.sxx[k];
+  n.sxy = s.sxy[k];
+  n.syy = s.syy[k];
+  n.Rxx = s.Rxx[k];
+  n.Rxy = s.Rxy[k];
+  n.Ryy = s.Ryy[k];
+  return n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ul_list_lds32(UlArgs a) {
+  __shared__ BandLds s;
+  __shared__ uint16_t s_nb[kListBatch * kBlock];
+  __shared__ uint32_t s_base[3], s_end[3];  // slot = j + s_base[tx]; band tx ends at slot s_end[tx]
+  __shared__ int32_t s_tot;                 // staged candidates, -1: global path
+  const uint32_t i0 = xcd_tile_chunked(a.xcd_chunk) * kBlock;
+  if (i0 >= a.n) return;  // the whole workgroup
+  if (threadIdx.x == 0) {
+    const uint32_t i1 = min(a.n, i0 + kBlock) - 1;
+    const uint2 m0 = a.meta[i0], m1 = a.meta[i1];
+    int32_t tot = -1;
+    if (m0.y == m1.y) {
+      const SceneConst* __restrict__ c = a.sc + m0.y;
+      const SceneGeom g = a.geom[m0.y];
+      const int64_t kb = c->cell_offset, ke = kb + (int64_t)g.num_cell - 1, ny = g.ny;
+      uint32_t t = 0;
+      for (int b = 0; b < 3; b++) {
+        const int64_t ka = max(kb, (int64_t)m0.x + (1 - b) * ny - 1);
+        const int64_t kz = min(ke, (int64_t)m1.x + (1 - b) * ny + 1);
+        const uint32_t lo = ka <= kz ? a.cell_start[ka] : 0u;
+        const uint32_t len = ka <= kz ? a.cell_start[kz + 1] - lo : 0u;
+        s_base[b] = t - lo;
+        t += len;
+        s_end[b] = t;
+      }
+      tot = t <= (uint32_t)kBandCap ? (int32_t)t : -1;
+    }
+    s_tot = tot;
+  }
+  __syncthreads();
+  const int32_t tot = s_tot;
+  const uint32_t i = i0 + threadIdx.x;
+  // the epilogue's state loads are issued first: they complete while the
+  // bands are staged and the pairs evaluated
+  EpiIn ein;
+  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+  for (int32_t k = threadIdx.x; k < tot; k += kBlock) {
+    const uint32_t b = (uint32_t)k < s_end[0] ? 0 : ((uint32_t)k < s_end[1] ? 1 : 2);
+    const uint32_t j = (uint32_t)k - s_base[b];
+    const float2 p = a.pos32[j];
+    const PairRec32 r = a.rec32[j];
+    s.px[k] = p.x;
+    s.py[k] = p.y;
+    s.vx[k] = r.vx;
+    s.vy[k] = r.vy;
+    s.rho[k] = r.rho;
+    s.qw[k] = r.qw;
+    s.m[k] = r.m;
+    s.h[k] = r.h;
+    s.c[k] = r.c;
+    s.sxx[k] = r.sxx_r;
+    s.sxy[k] = r.sxy_r;
+    s.syy[k] = r.syy_r;
+    s.Rxx[k] = r.Rxx;
+    s.Rxy[k] = r.Rxy;
+    s.Ryy[k] = r.Ryy;
+  }
+  __syncthreads();  // the last barrier: threads past the end leave below
+  if (i >= a.n) return;
+  if (tot < 0) {
+    list_stage_global<1>(a, i, 0, reinterpret_cast<uint32_t*>(&s) + threadIdx.x);
+    return;
+  }
+  const uint32_t base0 = s_base[0], base1 = s_base[1], base2 = s_base[2];
+  uint16_t* const my = s_nb + threadIdx.x;
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const float cellf = (float)g.cell;
+  const uint32_t li = i + base1;  // the particle's own cell lies in band 1
+  const float pix = s.px[li], piy = s.py[li];
+  const Me me = make_me(c, nb_from_lds(s, li));
+  const uint32_t deg = a.nl_deg[i];
+  const uint32_t* __restrict__ col = a.nl_col + i;
+  Sums S;
+  for (uint32_t base = 0; base < deg; base += kListBatch) {
+    const uint32_t end = min(deg, base + kListBatch);
+    int nb = 0;
+    for (uint32_t k0 = base; k0 < end; k0 += kScanBatch) {
+      uint32_t e[kScanBatch], sl[kScanBatch];
+#pragma unroll
+      for (int t = 0; t < kScanBatch; t++) {
+        const uint32_t k = k0 + t;
+        e[t] = ld_nt(&col[(size_t)(k < end ? k : base) * a.n]);  // each list entry is read once
+        const uint32_t tx = (e[t] >> 28) & 3u;
+        sl[t] = ((e[t] & kListMask) + (tx == 0 ? base0 : tx == 1 ? base1 : base2)) & (kBandCap - 1);
+      }
+#pragma unroll
+      for (int t = 0; t < kScanBatch; t++) {
+        const float xij = (pix - s.px[sl[t]]) + (float)((int)((e[t] >> 28) & 3u) - 1) * cellf;
+        const float yij = (piy - s.py[sl[t]]) + (float)((int)(e[t] >> 30) - 1) * cellf;
+        if (k0 + t < end && xij * xij + yij * yij <= me.cut2)
+          my[nb++ * kBlock] = (uint16_t)(sl[t] | ((e[t] >> 18) & 0x3C00u));  // slot | tx << 10 | ty << 12
+      }
+    }
+    // the next entry's fields and the word after it are read while the
+    // current one is evaluated (the last reads repeat the last entry)
+    const int last = max(nb - 1, 0);  // with no entry the reads stay inside the buffer, unused
+    uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
+    float pxn = s.px[wn & (kBandCap - 1)], pyn = s.py[wn & (kBandCap - 1)];
+    Nb nn = nb_from_lds(s, wn & (kBandCap - 1));
+    for (int t = 0; t < nb; t++) {
+      const uint32_t w = wn;
+      const float pxj = pxn, pyj = pyn;
+      const Nb nj = nn;
+      wn = wnn;
+      wnn = my[min(t + 2, last) * kBlock];
+      const uint32_t kn = wn & (kBandCap - 1);
+      pxn = s.px[kn];
+      pyn = s.py[kn];
+      nn = nb_from_lds(s, kn);
+      const float xij = (pix - pxj) + (float)((int)((w >> 10) & 3u) - 1) * cellf;
+      const float yij = (piy - pyj) + (float)((int)(w >> 12) - 1) * cellf;
+      pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
+    }
+  }
+  const double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
+                        acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
+                        acc_val(S.ax),    acc_val(S.ay),    acc_val(S.xs),    acc_val(S.ys)};
+  finish_list<1, true>(a, c, g, mt.x, i, 0, v, ein);
 }
 
 // Stage 1 of a list step in one pass: the 3x3 block of the step's cells is
@@ -817,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
   const double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
                         acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
                         acc_val(S.ax),    acc_val(S.ay),    acc_val(S.xs),    acc_val(S.ys)};
-  finish_list<1>(a, c, g, mt.x, i, 0, v);
+  finish_list<1, false>(a, c, g, mt.x, i, 0, v, EpiIn{});
 }
 
 void launch_ul_list_first(hipStream_t st, const UlArgs& a) {
@@ -825,10 +1029,12 @@ void launch_ul_list_first(hipStream_t st, const UlArgs& a) {
   hipLaunchKernelGGL(k_ul_list_first32, grid, dim3(kBlock), 0, st, a);
 }
 
-void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes) {
+void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes, bool staged) {
   const uint32_t per = kBlock / (uint32_t)lanes;
   const dim3 grid(xcd_grid((a.n + per - 1) / per));
-  if (lanes == 4)
+  if (lanes == 1 && staged)
+    hipLaunchKernelGGL(k_ul_list_lds32, grid, dim3(kBlock), 0, st, a);
+  else if (lanes == 4)
     hipLaunchKernelGGL(k_ul_list32<4>, grid, dim3(kBlock), 0, st, a);
   else if (lanes == 2)
     hipLaunchKernelGGL(k_ul_list32<2>, grid, dim3(kBlock), 0, st, a);

@@ -15,7 +15,7 @@ class Solver:
     def __init__(self, method=_capi.METHOD_UL_MONAGHAN, precision=_capi.PRECISION_FP64, use_graph=True,
                  deterministic=True, max_nbh=200, device=0, tl_h_search=0.0, tile_cells=0,
                  pipeline=_capi.PIPELINE_FUSED, xcd_chunk=0, cell_scale=0.0, neighbor_list=0, list_lanes=0,
-                 isolate_bodies=False):
+                 isolate_bodies=False, list_staging=0):
         L = _capi.lib()
         opt = _capi.Options()
         L.mfree_default_options(C.byref(opt))
@@ -33,6 +33,7 @@ class Solver:
         opt.neighbor_list = neighbor_list
         opt.list_lanes = list_lanes
         opt.isolate_bodies = 1 if isolate_bodies else 0
+        opt.list_staging = list_staging
         h = C.c_void_p()
         _capi.check(L.mfree_create(C.byref(opt), C.byref(h)), what="mfree_create")
         self._h = h
