@@ -565,7 +565,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
       if (e0) hipEventRecord(e0, st);
     }
     if (k == 1)
-      launch_ul_list_first(st, p);
+      launch_ul_list_first(st, p, c->list_staged);
     else
       launch_ul_list(st, p, c->list_lanes, c->list_staged);
     if (evs && e0 && e1) {

@@ -71,7 +71,7 @@ struct mfree_ctx {
   // neighbour-list path (mixed UL RK4 steps)
   bool use_list = false;
   int list_lanes = 1;
-  bool list_staged = true;  // list stages: neighbours staged in LDS (one lane per particle)
+  bool list_staged = true;  // list steps: neighbours staged in LDS (stage 1, and stages with one lane)
   uint32_t nl_cap = 0;
   uint32_t *nl_col = nullptr, *nl_deg = nullptr;
   float2* pos32b = nullptr;      // second record buffer (records of the next stage)

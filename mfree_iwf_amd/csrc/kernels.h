@@ -223,9 +223,9 @@ void launch_ul_prepass(hipStream_t st, const UlArgs& a);
 void launch_ul_pairs(hipStream_t st, const UlArgs& a);
 // Neighbour-list path (after the stage-1 binning and prepass): stage 1,
 // which also builds the step's list, and the stages 2-4 from the list with
-// `lanes` lanes per particle (one lane: neighbours staged in LDS per
-// workgroup when `staged`).
-void launch_ul_list_first(hipStream_t st, const UlArgs& a);
+// `lanes` lanes per particle (stage 1, and the stages with one lane:
+// neighbours staged in LDS per workgroup when `staged`).
+void launch_ul_list_first(hipStream_t st, const UlArgs& a, bool staged);
 void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes, bool staged);
 
 // Verlet-list emission in the reference's order (parity helper).

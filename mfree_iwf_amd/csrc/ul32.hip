@@ -6,8 +6,7 @@
 //  - positions are cell-local (x minus the particle's own cell corner,
 //    subtracted in FP64), so x_ij keeps ~1e-11 m resolution instead of the
 //    ~1e-8 m of absolute FP32 coordinates;
-//  - every pair sum is accumulated with Kahan compensation into a
-//    (sum, compensation) pair;
+//  - every FP32 pair term is accumulated into an FP64 sum;
 //  - the per-particle epilogue (Jaumann rate, continuity, momentum,
 //    advection) and the RK4 state update stay FP64.
 //
@@ -34,19 +33,16 @@ namespace mfree {
 namespace {
 
 struct Acc {
-  float s = 0.f, c = 0.f;
+  double s = 0.;
 };
 
-// Kahan compensated summation: c carries the negated low-order part lost
-// by each addition (Kahan 1965).
-__device__ inline void acc_add(Acc& a, float x) {
-  const float y = x - a.c;
-  const float t = a.s + y;
-  a.c = (t - a.s) - y;
-  a.s = t;
-}
+// Each FP32 pair term is added to an FP64 sum: exact conversion, one
+// rounding at 2^-53 per term.  On gfx950 an FP64 add issues at the FP32
+// rate, so this costs two instructions per term where a compensated FP32
+// (Kahan) sum costs four, and it is the more accurate of the two.
+__device__ inline void acc_add(Acc& a, float x) { a.s += (double)x; }
 
-__device__ inline double acc_val(const Acc& a) { return (double)a.s - (double)a.c; }
+__device__ inline double acc_val(const Acc& a) { return a.s; }
 
 struct Sums {
   Acc Sxx_x, Sxy_y, Sxy_x, Syy_y;
@@ -115,8 +111,7 @@ __device__ inline Nb nb_from_rec(const PairRec32& r) {
 // All four pair sums of one neighbour at separation (xij, yij), r2 = |x_ij|^2.
 // Square roots and reciprocals are the hardware's single instructions
 // (v_sqrt_f32, v_rcp_f32: about 1 ulp) rather than the correctly rounded
-// sequences; each term then carries about one rounding more, well inside the
-// compensated sums' error.
+// sequences; each term then carries about one FP32 rounding more.
 __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
   const float rij = __builtin_amdgcn_sqrtf(r2);
   const float q = rij * me.h1;
@@ -696,8 +691,8 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
 // compacted in LDS (list word and position only), then evaluated with the
 // next one's position and record loaded ahead, so the expensive pair terms
 // run with every lane busy (about 70 % of the entries are inside the
-// cutoff).  The compensated partial sums of the L lanes are combined in
-// FP64 with cross-lane shuffles.
+// cutoff).  The FP64 partial sums of the L lanes are combined with
+// cross-lane shuffles.
 constexpr int kListBatch = 32;
 
 // Sorted particle i, lane q of L, every neighbour gathered from global
@@ -813,45 +808,61 @@ __device__ inline Nb nb_from_lds(const BandLds& s, uint32_t k) {
   return n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ul_list_lds32(UlArgs a) {
-  __shared__ BandLds s;
-  __shared__ uint16_t s_nb[kListBatch * kBlock];
-  __shared__ uint32_t s_base[3], s_end[3];  // slot = j + s_base[tx]; band tx ends at slot s_end[tx]
-  __shared__ int32_t s_tot;                 // staged candidates, -1: global path
-  const uint32_t i0 = xcd_tile_chunked(a.xcd_chunk) * kBlock;
-  if (i0 >= a.n) return;  // the whole workgroup
-  if (threadIdx.x == 0) {
-    const uint32_t i1 = min(a.n, i0 + kBlock) - 1;
-    const uint2 m0 = a.meta[i0], m1 = a.meta[i1];
-    int32_t tot = -1;
-    if (m0.y == m1.y) {
-      const SceneConst* __restrict__ c = a.sc + m0.y;
-      const SceneGeom g = a.geom[m0.y];
-      const int64_t kb = c->cell_offset, ke = kb + (int64_t)g.num_cell - 1, ny = g.ny;
-      uint32_t t = 0;
-      for (int b = 0; b < 3; b++) {
-        const int64_t ka = max(kb, (int64_t)m0.x + (1 - b) * ny - 1);
-        const int64_t kz = min(ke, (int64_t)m1.x + (1 - b) * ny + 1);
-        const uint32_t lo = ka <= kz ? a.cell_start[ka] : 0u;
-        const uint32_t len = ka <= kz ? a.cell_start[kz + 1] - lo : 0u;
-        s_base[b] = t - lo;
-        t += len;
-        s_end[b] = t;
-      }
-      tot = t <= (uint32_t)kBandCap ? (int32_t)t : -1;
-    }
-    s_tot = tot;
+// Pair terms of the nb compacted entries my[0], my[kBlock], ... (LDS slot |
+// tx << 10 | ty << 12), in order.  The next entry's fields and the word after
+// it are read while the current one is evaluated (reads past the end repeat
+// the last entry, and with no entry stay inside the buffer, unused).
+__device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
+                                float cellf, Sums& S) {
+  const int last = max(nb - 1, 0);
+  uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
+  float pxn = s.px[wn & (kBandCap - 1)], pyn = s.py[wn & (kBandCap - 1)];
+  Nb nn = nb_from_lds(s, wn & (kBandCap - 1));
+  for (int t = 0; t < nb; t++) {
+    const uint32_t w = wn;
+    const float pxj = pxn, pyj = pyn;
+    const Nb nj = nn;
+    wn = wnn;
+    wnn = my[min(t + 2, last) * kBlock];
+    const uint32_t kn = wn & (kBandCap - 1);
+    pxn = s.px[kn];
+    pyn = s.py[kn];
+    nn = nb_from_lds(s, kn);
+    const float xij = (pix - pxj) + (float)((int)((w >> 10) & 3u) - 1) * cellf;
+    const float yij = (piy - pyj) + (float)((int)(w >> 12) - 1) * cellf;
+    pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
   }
-  __syncthreads();
-  const int32_t tot = s_tot;
-  const uint32_t i = i0 + threadIdx.x;
-  // the epilogue's state loads are issued first: they complete while the
-  // bands are staged and the pairs evaluated
-  EpiIn ein;
-  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+}
+
+// One thread of the workgroup of sorted particles [i0, i0 + kBlock): the
+// three bands (slot = j + base[tx]; band tx ends at slot end[tx]); returns
+// the number of staged candidates, or -1 for the global path.
+__device__ inline int32_t band_setup(const UlArgs& a, uint32_t i0, uint32_t* base, uint32_t* end) {
+  const uint32_t i1 = min(a.n, i0 + kBlock) - 1;
+  const uint2 m0 = a.meta[i0], m1 = a.meta[i1];
+  if (m0.y != m1.y) return -1;
+  const SceneConst* __restrict__ c = a.sc + m0.y;
+  const SceneGeom g = a.geom[m0.y];
+  const int64_t kb = c->cell_offset, ke = kb + (int64_t)g.num_cell - 1, ny = g.ny;
+  uint32_t t = 0;
+  for (int b = 0; b < 3; b++) {
+    const int64_t ka = max(kb, (int64_t)m0.x + (1 - b) * ny - 1);
+    const int64_t kz = min(ke, (int64_t)m1.x + (1 - b) * ny + 1);
+    const uint32_t lo = ka <= kz ? a.cell_start[ka] : 0u;
+    const uint32_t len = ka <= kz ? a.cell_start[kz + 1] - lo : 0u;
+    base[b] = t - lo;
+    t += len;
+    end[b] = t;
+  }
+  return t <= (uint32_t)kBandCap ? (int32_t)t : -1;
+}
+
+// The whole workgroup: positions and records of the tot staged candidates.
+__device__ inline void band_fill(const UlArgs& a, BandLds& s, int32_t tot, const uint32_t* base,
+                                 const uint32_t* end) {
   for (int32_t k = threadIdx.x; k < tot; k += kBlock) {
-    const uint32_t b = (uint32_t)k < s_end[0] ? 0 : ((uint32_t)k < s_end[1] ? 1 : 2);
-    const uint32_t j = (uint32_t)k - s_base[b];
+    const uint32_t b = (uint32_t)k < end[0] ? 0 : ((uint32_t)k < end[1] ? 1 : 2);
+    const uint32_t j = (uint32_t)k - base[b];
     const float2 p = a.pos32[j];
     const PairRec32 r = a.rec32[j];
     s.px[k] = p.x;
@@ -870,6 +881,24 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_lds32(UlArgs a) {
     s.Rxy[k] = r.Rxy;
     s.Ryy[k] = r.Ryy;
   }
+}
+
+__global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
+  __shared__ BandLds s;
+  __shared__ uint16_t s_nb[kListBatch * kBlock];
+  __shared__ uint32_t s_base[3], s_end[3];
+  __shared__ int32_t s_tot;
+  const uint32_t i0 = xcd_tile_chunked(a.xcd_chunk) * kBlock;
+  if (i0 >= a.n) return;  // the whole workgroup
+  if (threadIdx.x == 0) s_tot = band_setup(a, i0, s_base, s_end);
+  __syncthreads();
+  const int32_t tot = s_tot;
+  const uint32_t i = i0 + threadIdx.x;
+  // the epilogue's state loads are issued first: they complete while the
+  // bands are staged and the pairs evaluated
+  EpiIn ein;
+  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+  band_fill(a, s, tot, s_base, s_end);
   __syncthreads();  // the last barrier: threads past the end leave below
   if (i >= a.n) return;
   if (tot < 0) {
@@ -908,26 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_lds32(UlArgs a) {
           my[nb++ * kBlock] = (uint16_t)(sl[t] | ((e[t] >> 18) & 0x3C00u));  // slot | tx << 10 | ty << 12
       }
     }
-    // the next entry's fields and the word after it are read while the
-    // current one is evaluated (the last reads repeat the last entry)
-    const int last = max(nb - 1, 0);  // with no entry the reads stay inside the buffer, unused
-    uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
-    float pxn = s.px[wn & (kBandCap - 1)], pyn = s.py[wn & (kBandCap - 1)];
-    Nb nn = nb_from_lds(s, wn & (kBandCap - 1));
-    for (int t = 0; t < nb; t++) {
-      const uint32_t w = wn;
-      const float pxj = pxn, pyj = pyn;
-      const Nb nj = nn;
-      wn = wnn;
-      wnn = my[min(t + 2, last) * kBlock];
-      const uint32_t kn = wn & (kBandCap - 1);
-      pxn = s.px[kn];
-      pyn = s.py[kn];
-      nn = nb_from_lds(s, kn);
-      const float xij = (pix - pxj) + (float)((int)((w >> 10) & 3u) - 1) * cellf;
-      const float yij = (piy - pyj) + (float)((int)(w >> 12) - 1) * cellf;
-      pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
-    }
+    eval_lds(s, my, nb, me, pix, piy, cellf, S);
   }
   const double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
                         acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
@@ -940,11 +950,9 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_lds32(UlArgs a) {
 // (slot-major), those inside the cutoff also to the LDS buffer, which is
 // evaluated as in k_ul_pairs32 (collective flushes before a column that may
 // overflow a lane's buffer).  The epilogue is the list stages' one.
-__global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
-  __shared__ uint32_t nb_lds[kNbCap * kBlock];
-  uint32_t* const my = nb_lds + threadIdx.x;
-  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+// Sorted particle i from global memory; my = this thread's buffer (kNbCap
+// words, stride kBlock).
+__device__ inline void list_first_global(const UlArgs& a, uint32_t i, uint32_t* my) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
   const SceneGeom g = a.geom[mt.y];
@@ -1024,9 +1032,117 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
   finish_list<1, false>(a, c, g, mt.x, i, 0, v, EpiIn{});
 }
 
-void launch_ul_list_first(hipStream_t st, const UlArgs& a) {
+__global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
+  __shared__ uint32_t nb_lds[kNbCap * kBlock];
+  const uint32_t i = xcd_tile_chunked(a.xcd_chunk) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  list_first_global(a, i, nb_lds + threadIdx.x);
+}
+
+// Stage 1 with the candidates staged in LDS: the workgroup's bands (as for
+// the list stages) hold every particle of the 3x3 cell blocks its particles
+// scan, so the scan reads positions from LDS and the pairs inside the cutoff
+// are evaluated from LDS.  Candidates are visited, listed and summed in the
+// global path's order: the same list and the same bits.
+static_assert(sizeof(BandLds) >= sizeof(uint32_t) * kNbCap * kBlock, "global path buffer aliases the bands");
+
+__global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
+  __shared__ BandLds s;
+  __shared__ uint16_t s_nb[kNbCap * kBlock];
+  __shared__ uint32_t s_base[3], s_end[3];
+  __shared__ int32_t s_tot;
+  const uint32_t i0 = xcd_tile_chunked(a.xcd_chunk) * kBlock;
+  if (i0 >= a.n) return;  // the whole workgroup
+  if (threadIdx.x == 0) s_tot = band_setup(a, i0, s_base, s_end);
+  __syncthreads();
+  const int32_t tot = s_tot;
+  const uint32_t i = i0 + threadIdx.x;
+  EpiIn ein;
+  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+  band_fill(a, s, tot, s_base, s_end);
+  __syncthreads();  // the last barrier: threads past the end leave below
+  if (i >= a.n) return;
+  if (tot < 0) {
+    list_first_global(a, i, reinterpret_cast<uint32_t*>(&s) + threadIdx.x);
+    return;
+  }
+  uint16_t* const my = s_nb + threadIdx.x;
+  const uint2 mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+  const float cellf = (float)g.cell;
+  const uint32_t li = i + s_base[1];  // the particle's own cell lies in band 1
+  const float pix = s.px[li], piy = s.py[li];
+  const Me me = make_me(c, nb_from_lds(s, li));
+  const float rl = (float)(2.0 * ein.h + g.skin);
+  const float rl2 = rl * rl * 1.0001f;
+  Sums S;
+  int nb = 0;
+  uint32_t deg = 0;
+  auto flush = [&]() {
+    eval_lds(s, my, nb, me, pix, piy, cellf, S);
+    nb = 0;
+  };
+  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
+    const uint32_t rb = c->cell_offset + ni * g.ny + lo_j;
+    const uint32_t nc = hi_j - lo_j;  // 1..3 consecutive cells
+    const uint32_t b0 = a.cell_start[rb], b3 = a.cell_start[rb + nc];
+    const uint32_t b1 = nc > 1 ? a.cell_start[rb + 1] : 0xFFFFFFFFu;
+    const uint32_t b2 = nc > 2 ? a.cell_start[rb + 2] : 0xFFFFFFFFu;
+    if (__any(nb + ((b3 - b0 + 2) >> 2) + kScanBatch > (uint32_t)kNbCap)) flush();
+    const float offx = (float)((int)gi - (int)ni) * cellf;
+    const uint32_t tx = gi - ni + 1;
+    const uint32_t base = tx == 0 ? s_base[0] : tx == 1 ? s_base[1] : s_base[2];
+    for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
+      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      uint32_t sl[kScanBatch];
+      float pxj[kScanBatch], pyj[kScanBatch];
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) {
+        sl[e] = (min(j0 + e, b3 - 1) + base) & (kBandCap - 1);
+        pxj[e] = s.px[sl[e]];
+        pyj[e] = s.py[sl[e]];
+      }
+#pragma unroll
+      for (int e = 0; e < kScanBatch; e++) {
+        const uint32_t j = j0 + e;
+        if (j < b3 && j != i) {
+          const uint32_t nj = lo_j + (j >= b1 ? 1u : 0u) + (j >= b2 ? 1u : 0u);
+          const float xij = (pix - pxj[e]) + offx;
+          const float yij = (piy - pyj[e]) + (float)((int)gj - (int)nj) * cellf;
+          const float r2 = xij * xij + yij * yij;
+          if (r2 <= rl2) {
+            const uint32_t ty = gj - nj + 1;
+            if (deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = j | (tx << 28) | (ty << 30);
+            deg++;
+            if (r2 <= me.cut2) my[nb++ * kBlock] = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
+          }
+        }
+      }
+    }
+  }
+  flush();
+  if (deg > a.nl_cap) {
+    raise_err(a.err, a.berr, mt.y, EB_NBH);
+    deg = a.nl_cap;
+  }
+  a.nl_deg[i] = deg;
+  const double v[12] = {acc_val(S.Sxx_x), acc_val(S.Sxy_y), acc_val(S.Sxy_x), acc_val(S.Syy_y),
+                        acc_val(S.vx_x),  acc_val(S.vx_y),  acc_val(S.vy_x),  acc_val(S.vy_y),
+                        acc_val(S.ax),    acc_val(S.ay),    acc_val(S.xs),    acc_val(S.ys)};
+  finish_list<1, true>(a, c, g, mt.x, i, 0, v, ein);
+}
+
+void launch_ul_list_first(hipStream_t st, const UlArgs& a, bool staged) {
   const dim3 grid(xcd_grid((a.n + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_ul_list_first32, grid, dim3(kBlock), 0, st, a);
+  if (staged)
+    hipLaunchKernelGGL(k_ul_list_first_lds32, grid, dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_ul_list_first32, grid, dim3(kBlock), 0, st, a);
 }
 
 void launch_ul_list(hipStream_t st, const UlArgs& a, int lanes, bool staged) {
