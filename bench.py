@@ -725,7 +725,7 @@ def main():
         return
     achieved = ALG_BYTES_PER_PARTICLE_STAGE * n / (head["pair_kernel_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                traffic=None, kernel="k_ul_list32<1>" if head is modes.get("mixed") else "k_ul_pairs64",
+                traffic=None, kernel="k_ul_list_lds32" if head is modes.get("mixed") else "k_ul_pairs64",
                 algorithmic_bytes_per_launch=ALG_BYTES_PER_PARTICLE_STAGE * n)
     tr = traffic_from_profiles(roof["kernel"], n)
     if tr:

@@ -160,6 +160,63 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
   acc_add(S.ys, -fx * dvy);
 }
 
+// pair_terms as one basic block, for the LDS-staged kernels: the kernel's
+// two pieces, the artificial stress and the viscosity are evaluated and
+// selected with pair_terms' own predicates, and a pair at q >= 2 (possible
+// only inside the cutoff's 1e-4 margin) adds zeros.  Adding +-0 to an FP64
+// sum that starts at +0 leaves it unchanged, so the sums are pair_terms'
+// bit for bit (for finite states), while the scheduler can overlap one
+// pair's square root, reciprocals and loads with the rest of its work.
+// AS: 0 no artificial stress (wdeltap <= 0), 1 exponent 4, 2 any exponent.
+template <int AS>
+__device__ inline void pair_terms_flat(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
+  const float rij = __builtin_amdgcn_sqrtf(r2);
+  const float q = rij * me.h1;
+  const bool inside = !(q >= 2.f);
+  const float t = 2.f - q;
+  const float w_o = me.fac * (0.25f * t * t * t);
+  const float der_o = -0.75f * t * t * me.h1;
+  const float w_i = me.fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+  const float der_i = -3.0f * q * (1.f - 0.75f * q) * me.h1;
+  const float w = inside ? (q >= 1.f ? w_o : w_i) : 0.f;
+  const float der = inside ? (q >= 1.f ? der_o : der_i) : 0.f;
+  const float gsc = rij > 1e-12f ? der * me.fac * __builtin_amdgcn_rcpf(rij) : 0.f;
+  const float wx = xij * gsc, wy = yij * gsc;
+  float Rxx = 0.f, Rxy = 0.f, Ryy = 0.f;
+  if (AS > 0) {
+    float fab = w * me.inv_wdp;
+    fab = AS == 1 ? (fab * fab) * (fab * fab) : powf(fab, me.corr_exp);
+    Rxx = fab * (me.Rxx + nj.Rxx);
+    Rxy = fab * (me.Rxy + nj.Rxy);
+    Ryy = fab * (me.Ryy + nj.Ryy);
+  }
+  const float mj = nj.m;
+  acc_add(S.Sxx_x, mj * (me.sxx + nj.sxx + Rxx) * wx);
+  acc_add(S.Sxy_y, mj * (me.sxy + nj.sxy + Rxy) * wy);
+  acc_add(S.Sxy_x, mj * (me.sxy + nj.sxy + Rxy) * wx);
+  acc_add(S.Syy_y, mj * (me.syy + nj.syy + Ryy) * wy);
+  const float dvx = nj.vx - me.vx, dvy = nj.vy - me.vy;
+  acc_add(S.vx_x, dvx * wx * nj.qw);
+  acc_add(S.vx_y, dvx * wy * nj.qw);
+  acc_add(S.vy_x, dvy * wx * nj.qw);
+  acc_add(S.vy_y, dvy * wy * nj.qw);
+  const float vp = -(dvx * xij + dvy * yij);
+  const float irhoij = __builtin_amdgcn_rcpf(0.5f * (me.rho + nj.rho));
+  const float cij = 0.5f * (me.c + nj.c);
+  const float hij = 0.5f * (me.h + nj.h);
+  const float muij = (hij * vp) * __builtin_amdgcn_rcpf(r2 + me.eta2 * hij * hij);
+  const float pi_c = (-me.alpha * cij * muij + me.beta * muij * muij) * irhoij;
+  const float piij = !(vp >= 0.f) ? pi_c : 0.f;
+  acc_add(S.ax, -mj * piij * wx);
+  acc_add(S.ay, -mj * piij * wy);
+  const float fx = -me.xeps_mi * w * irhoij;
+  acc_add(S.xs, -fx * dvx);
+  acc_add(S.ys, -fx * dvy);
+}
+
+// The artificial-stress variant of a particle's scene.
+__device__ inline int as_mode(const Me& me) { return !me.use_as ? 0 : me.pow4 ? 1 : 2; }
+
 __device__ inline void jaumann64(double G2, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx,
                                  double Sxy, double Syy, double& t00, double& t01, double& t11) {
   const double e01 = 0.5 * (vx_y + vy_x);
@@ -812,8 +869,9 @@ __device__ inline Nb nb_from_lds(const BandLds& s, uint32_t k) {
 // tx << 10 | ty << 12), in order.  The next entry's fields and the word after
 // it are read while the current one is evaluated (reads past the end repeat
 // the last entry, and with no entry stay inside the buffer, unused).
-__device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
-                                float cellf, Sums& S) {
+template <int AS>
+__device__ inline void eval_lds_as(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
+                                   float cellf, Sums& S) {
   const int last = max(nb - 1, 0);
   uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
   float pxn = s.px[wn & (kBandCap - 1)], pyn = s.py[wn & (kBandCap - 1)];
@@ -830,8 +888,19 @@ __device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, co
     nn = nb_from_lds(s, kn);
     const float xij = (pix - pxj) + (float)((int)((w >> 10) & 3u) - 1) * cellf;
     const float yij = (piy - pyj) + (float)((int)(w >> 12) - 1) * cellf;
-    pair_terms(me, nj, xij, yij, xij * xij + yij * yij, S);
+    pair_terms_flat<AS>(me, nj, xij, yij, xij * xij + yij * yij, S);
   }
+}
+
+__device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
+                                float cellf, Sums& S) {
+  const int as = as_mode(me);
+  if (as == 1)
+    eval_lds_as<1>(s, my, nb, me, pix, piy, cellf, S);
+  else if (as == 0)
+    eval_lds_as<0>(s, my, nb, me, pix, piy, cellf, S);
+  else
+    eval_lds_as<2>(s, my, nb, me, pix, piy, cellf, S);
 }
 
 // One thread of the workgroup of sorted particles [i0, i0 + kBlock): the
@@ -894,10 +963,17 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   __syncthreads();
   const int32_t tot = s_tot;
   const uint32_t i = i0 + threadIdx.x;
-  // the epilogue's state loads are issued first: they complete while the
-  // bands are staged and the pairs evaluated
+  // issued first, to arrive while the bands are staged: the epilogue's state
+  // and the first kListBatch words of the list (each read once)
   EpiIn ein;
-  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+  uint32_t deg = 0, ew[kListBatch];
+  const uint32_t* __restrict__ col = a.nl_col + i;
+  if (tot >= 0 && i < a.n) {
+    ein = epi_load<1>(a, i, 0);
+    deg = a.nl_deg[i];
+#pragma unroll
+    for (int t = 0; t < kListBatch; t++) ew[t] = (uint32_t)t < deg ? ld_nt(&col[(size_t)t * a.n]) : 0u;
+  }
   band_fill(a, s, tot, s_base, s_end);
   __syncthreads();  // the last barrier: threads past the end leave below
   if (i >= a.n) return;
@@ -914,28 +990,22 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   const uint32_t li = i + base1;  // the particle's own cell lies in band 1
   const float pix = s.px[li], piy = s.py[li];
   const Me me = make_me(c, nb_from_lds(s, li));
-  const uint32_t deg = a.nl_deg[i];
-  const uint32_t* __restrict__ col = a.nl_col + i;
   Sums S;
   for (uint32_t base = 0; base < deg; base += kListBatch) {
-    const uint32_t end = min(deg, base + kListBatch);
+    if (base > 0) {  // a list longer than one batch: its next words
+#pragma unroll
+      for (int t = 0; t < kListBatch; t++) ew[t] = base + t < deg ? ld_nt(&col[(size_t)(base + t) * a.n]) : 0u;
+    }
     int nb = 0;
-    for (uint32_t k0 = base; k0 < end; k0 += kScanBatch) {
-      uint32_t e[kScanBatch], sl[kScanBatch];
 #pragma unroll
-      for (int t = 0; t < kScanBatch; t++) {
-        const uint32_t k = k0 + t;
-        e[t] = ld_nt(&col[(size_t)(k < end ? k : base) * a.n]);  // each list entry is read once
-        const uint32_t tx = (e[t] >> 28) & 3u;
-        sl[t] = ((e[t] & kListMask) + (tx == 0 ? base0 : tx == 1 ? base1 : base2)) & (kBandCap - 1);
-      }
-#pragma unroll
-      for (int t = 0; t < kScanBatch; t++) {
-        const float xij = (pix - s.px[sl[t]]) + (float)((int)((e[t] >> 28) & 3u) - 1) * cellf;
-        const float yij = (piy - s.py[sl[t]]) + (float)((int)(e[t] >> 30) - 1) * cellf;
-        if (k0 + t < end && xij * xij + yij * yij <= me.cut2)
-          my[nb++ * kBlock] = (uint16_t)(sl[t] | ((e[t] >> 18) & 0x3C00u));  // slot | tx << 10 | ty << 12
-      }
+    for (int t = 0; t < kListBatch; t++) {
+      // words past the list's end are 0: an in-range slot, not kept
+      const uint32_t e = ew[t], tx = (e >> 28) & 3u;
+      const uint32_t sl = ((e & kListMask) + (tx == 0 ? base0 : tx == 1 ? base1 : base2)) & (kBandCap - 1);
+      const float xij = (pix - s.px[sl]) + (float)((int)tx - 1) * cellf;
+      const float yij = (piy - s.py[sl]) + (float)((int)(e >> 30) - 1) * cellf;
+      if (base + t < deg && xij * xij + yij * yij <= me.cut2)
+        my[nb++ * kBlock] = (uint16_t)(sl | ((e >> 18) & 0x3C00u));  // slot | tx << 10 | ty << 12
     }
     eval_lds(s, my, nb, me, pix, piy, cellf, S);
   }
