@@ -171,6 +171,7 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, hipMemset(c->count, 0, (size_t)N * sizeof(uint32_t)));  // the binning keeps it zero between calls
   HIPCHK(c, dalloc(c, &c->cell_start, N + 1));
   HIPCHK(c, dalloc(c, &c->perm, N));
+  HIPCHK(c, dalloc(c, &c->perm_tmp, N));
   HIPCHK(c, dalloc(c, &c->tsum, (N + kScanTile - 1) / kScanTile + 1));
   HIPCHK(c, dalloc(c, &c->meta, N));
   HIPCHK(c, dalloc(c, &c->tile_off, c->S + 1));
@@ -267,6 +268,7 @@ BinArgs bin_args(mfree_ctx* c, const double* x, const double* y, int b) {
   a.tsum = c->tsum;
   a.cell_start = c->cell_start;
   a.perm = c->perm;
+  a.perm_tmp = c->perm_tmp;
   a.err = c->errb;
   a.berr = tl_method(c->opt.method) ? nullptr : c->berr;
   a.tile_cells = c->tile_cells;

@@ -364,6 +364,38 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(uint32_t ncell, const uint
   }
 }
 
+// One thread per sorted slot after the scatter into perm_in: the entry's
+// rank among its cell's entries by global id (ties, which do not occur for
+// distinct particles, by slot) places it in perm, so that within a cell the
+// particles appear in idx order -- the order k_cell_sort produces.  Every
+// load is independent of the comparisons, where an insertion sort chains
+// dependent loads.  A body whose grid fell back to one cell (its step fails)
+// keeps the scatter's order: ranking that cell is quadratic in the body.
+__global__ __launch_bounds__(kBlock) void k_cell_rank(uint32_t n, const uint32_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ gid,
+                                                      const uint32_t* __restrict__ scene,
+                                                      const SceneGeom* __restrict__ geom,
+                                                      const uint32_t* __restrict__ perm_in,
+                                                      uint32_t* __restrict__ perm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = perm_in[i];
+  if (geom[scene[p]].err) {
+    perm[i] = p;
+    return;
+  }
+  const uint32_t k = key[p];
+  const uint32_t a = cell_start[k], b = cell_start[k + 1];
+  const uint32_t g = gid[p];
+  uint32_t r = 0;
+  for (uint32_t j = a; j < b; j++) {
+    const uint32_t gj = gid[perm_in[j]];
+    r += (gj < g || (gj == g && j < i)) ? 1u : 0u;
+  }
+  perm[a + r] = p;
+}
+
 // Plain streaming copy: the HBM calibration kernel (reads and writes n*8 bytes).
 __global__ __launch_bounds__(kBlock) void k_calib_copy(const double2* __restrict__ a, double2* __restrict__ b,
                                                        uint64_t n2) {
@@ -415,9 +447,13 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum, a.cell_start,
                      a.n, 1);
+  const bool ranked = a.deterministic && a.perm_tmp;
   hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
-                     a.cell_start, a.perm);
-  if (a.deterministic)
+                     a.cell_start, ranked ? a.perm_tmp : a.perm);
+  if (ranked)
+    hipLaunchKernelGGL(k_cell_rank, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.cell_start, a.gid,
+                       a.scene, a.geom, a.perm_tmp, a.perm);
+  else if (a.deterministic)
     hipLaunchKernelGGL(k_cell_sort, dim3(nblk(a.ncell_cap, kBlock)), dim3(kBlock), 0, st, a.ncell_cap,
                        a.cell_start, a.gid, a.scene, a.geom, a.perm);
 }

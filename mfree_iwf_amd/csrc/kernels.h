@@ -25,6 +25,7 @@ struct BinArgs {
   const uint32_t* gid;          // storage order
   double* part;                 // 5 per chunk
   uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
+  uint32_t* perm_tmp = nullptr;  // deterministic binning: scatter here, rank-sort into perm (null: in-place sort)
   uint32_t* err;
   uint32_t* berr = nullptr;     // per-body error words (null: context word only)
   uint32_t tile_cells;          // > 0: also enumerate pair tiles of this many cells
