@@ -1127,8 +1127,33 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
   __syncthreads();
   const int32_t tot = s_tot;
   const uint32_t i = i0 + threadIdx.x;
+  const bool live = tot >= 0 && i < a.n;
+  // issued first, to arrive while the bands are staged: the epilogue's state
+  // and the cell starts of the (up to) three rows of cells the scan visits
   EpiIn ein;
-  if (tot >= 0 && i < a.n) ein = epi_load<1>(a, i, 0);
+  uint2 mt = make_uint2(0u, 0u);
+  if (live) mt = a.meta[i];
+  const SceneConst* __restrict__ c = a.sc + mt.y;
+  const SceneGeom g = a.geom[mt.y];
+  const uint32_t local = mt.x - c->cell_offset;
+  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
+  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
+  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
+  const uint32_t nc = hi_j - lo_j;  // 1..3 consecutive cells per row
+  uint32_t cb[3][4];                // per row: starts of its cells (absent: ~0) and its end
+  if (live) {
+    ein = epi_load<1>(a, i, 0);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const uint32_t ni = lo_i + r;
+      const uint32_t rb = c->cell_offset + ni * g.ny + lo_j;
+      const bool row = ni < hi_i;
+      cb[r][0] = row ? a.cell_start[rb] : 0u;
+      cb[r][1] = row && nc > 1 ? a.cell_start[rb + 1] : 0xFFFFFFFFu;
+      cb[r][2] = row && nc > 2 ? a.cell_start[rb + 2] : 0xFFFFFFFFu;
+      cb[r][3] = row ? a.cell_start[rb + nc] : 0u;
+    }
+  }
   band_fill(a, s, tot, s_base, s_end);
   __syncthreads();  // the last barrier: threads past the end leave below
   if (i >= a.n) return;
@@ -1137,13 +1162,6 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
     return;
   }
   uint16_t* const my = s_nb + threadIdx.x;
-  const uint2 mt = a.meta[i];
-  const SceneConst* __restrict__ c = a.sc + mt.y;
-  const SceneGeom g = a.geom[mt.y];
-  const uint32_t local = mt.x - c->cell_offset;
-  const uint32_t gi = local / g.ny, gj = local - gi * g.ny;
-  const uint32_t lo_i = gi > 0 ? gi - 1 : 0, hi_i = g.err ? lo_i : min(gi + 2, g.nx);
-  const uint32_t lo_j = gj > 0 ? gj - 1 : 0, hi_j = min(gj + 2, g.ny);
   const float cellf = (float)g.cell;
   const uint32_t li = i + s_base[1];  // the particle's own cell lies in band 1
   const float pix = s.px[li], piy = s.py[li];
@@ -1157,12 +1175,10 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
     eval_lds(s, my, nb, me, pix, piy, cellf, S);
     nb = 0;
   };
-  for (uint32_t ni = lo_i; ni < hi_i; ni++) {
-    const uint32_t rb = c->cell_offset + ni * g.ny + lo_j;
-    const uint32_t nc = hi_j - lo_j;  // 1..3 consecutive cells
-    const uint32_t b0 = a.cell_start[rb], b3 = a.cell_start[rb + nc];
-    const uint32_t b1 = nc > 1 ? a.cell_start[rb + 1] : 0xFFFFFFFFu;
-    const uint32_t b2 = nc > 2 ? a.cell_start[rb + 2] : 0xFFFFFFFFu;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const uint32_t ni = lo_i + r;  // rows past hi_i have b0 = b3 = 0: nothing to scan
+    const uint32_t b0 = cb[r][0], b1 = cb[r][1], b2 = cb[r][2], b3 = cb[r][3];
     if (__any(nb + ((b3 - b0 + 2) >> 2) + kScanBatch > (uint32_t)kNbCap)) flush();
     const float offx = (float)((int)gi - (int)ni) * cellf;
     const uint32_t tx = gi - ni + 1;
