@@ -866,43 +866,30 @@ __device__ inline Nb nb_from_lds(const BandLds& s, uint32_t k) {
 }
 
 // Pair terms of the nb compacted entries my[0], my[kBlock], ... (LDS slot |
-// tx << 10 | ty << 12), in order, two per iteration in two register sets:
-// the fields of the pair after next and the words two ahead are read while
-// the current pair is evaluated (reads past the end repeat the last entry,
-// and with no entry stay inside the buffer, unused).  Scheduling barriers
-// keep the two halves apart, so only the loaded fields are doubled, not the
-// pair terms' temporaries.
+// tx << 10 | ty << 12), in order.  The next entry's fields and the word after
+// it are read while the current one is evaluated (reads past the end repeat
+// the last entry, and with no entry stay inside the buffer, unused).
 template <int AS>
 __device__ inline void eval_lds_as(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
                                    float cellf, Sums& S) {
   const int last = max(nb - 1, 0);
-  auto eval = [&](uint32_t w, float pxj, float pyj, const Nb& nj) {
+  uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
+  float pxn = s.px[wn & (kBandCap - 1)], pyn = s.py[wn & (kBandCap - 1)];
+  Nb nn = nb_from_lds(s, wn & (kBandCap - 1));
+  for (int t = 0; t < nb; t++) {
+    const uint32_t w = wn;
+    const float pxj = pxn, pyj = pyn;
+    const Nb nj = nn;
+    wn = wnn;
+    wnn = my[min(t + 2, last) * kBlock];
+    const uint32_t kn = wn & (kBandCap - 1);
+    pxn = s.px[kn];
+    pyn = s.py[kn];
+    nn = nb_from_lds(s, kn);
     const float xij = (pix - pxj) + (float)((int)((w >> 10) & 3u) - 1) * cellf;
     const float yij = (piy - pyj) + (float)((int)(w >> 12) - 1) * cellf;
     pair_terms_flat<AS>(me, nj, xij, yij, xij * xij + yij * yij, S);
-  };
-  uint32_t wa = my[0], wb = my[min(1, last) * kBlock];
-  float pxa = s.px[wa & (kBandCap - 1)], pya = s.py[wa & (kBandCap - 1)];
-  Nb na = nb_from_lds(s, wa & (kBandCap - 1));
-  float pxb = s.px[wb & (kBandCap - 1)], pyb = s.py[wb & (kBandCap - 1)];
-  Nb nbb = nb_from_lds(s, wb & (kBandCap - 1));
-  int t = 0;
-  for (; t + 1 < nb; t += 2) {
-    const uint32_t wa2 = my[min(t + 2, last) * kBlock], wb2 = my[min(t + 3, last) * kBlock];
-    eval(wa, pxa, pya, na);
-    wa = wa2;
-    pxa = s.px[wa & (kBandCap - 1)];
-    pya = s.py[wa & (kBandCap - 1)];
-    na = nb_from_lds(s, wa & (kBandCap - 1));
-    __builtin_amdgcn_sched_barrier(0);
-    eval(wb, pxb, pyb, nbb);
-    wb = wb2;
-    pxb = s.px[wb & (kBandCap - 1)];
-    pyb = s.py[wb & (kBandCap - 1)];
-    nbb = nb_from_lds(s, wb & (kBandCap - 1));
-    __builtin_amdgcn_sched_barrier(0);
   }
-  if (t < nb) eval(wa, pxa, pya, na);
 }
 
 __device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
