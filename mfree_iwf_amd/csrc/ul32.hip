@@ -1193,21 +1193,22 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
         pxj[e] = s.px[sl[e]];
         pyj[e] = s.py[sl[e]];
       }
+      // predicated rather than branched: the separation of every slot of the
+      // batch is computed, the compaction buffer is written at nb for every
+      // slot (nb <= kNbCap - 1 here) and nb, deg advance by the predicates
 #pragma unroll
       for (int e = 0; e < kScanBatch; e++) {
         const uint32_t j = j0 + e;
-        if (j < b3 && j != i) {
-          const uint32_t nj = lo_j + (j >= b1 ? 1u : 0u) + (j >= b2 ? 1u : 0u);
-          const float xij = (pix - pxj[e]) + offx;
-          const float yij = (piy - pyj[e]) + (float)((int)gj - (int)nj) * cellf;
-          const float r2 = xij * xij + yij * yij;
-          if (r2 <= rl2) {
-            const uint32_t ty = gj - nj + 1;
-            if (deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = j | (tx << 28) | (ty << 30);
-            deg++;
-            if (r2 <= me.cut2) my[nb++ * kBlock] = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
-          }
-        }
+        const uint32_t nj = lo_j + (j >= b1 ? 1u : 0u) + (j >= b2 ? 1u : 0u);
+        const float xij = (pix - pxj[e]) + offx;
+        const float yij = (piy - pyj[e]) + (float)((int)gj - (int)nj) * cellf;
+        const float r2 = xij * xij + yij * yij;
+        const bool listed = j < b3 && j != i && r2 <= rl2;
+        const uint32_t ty = gj - nj + 1;
+        if (listed && deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = j | (tx << 28) | (ty << 30);
+        deg += listed ? 1u : 0u;
+        my[nb * kBlock] = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
+        nb += (listed && r2 <= me.cut2) ? 1 : 0;
       }
     }
   }
