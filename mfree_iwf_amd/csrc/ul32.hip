@@ -1184,7 +1184,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
     const uint32_t tx = gi - ni + 1;
     const uint32_t base = tx == 0 ? s_base[0] : tx == 1 ? s_base[1] : s_base[2];
     for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
-      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      // guard: room for the batch; the lanes still scanning flush together,
+      // not each on its own with the others idle
+      if (__any(nb > kNbCap - kScanBatch)) flush();
       uint32_t sl[kScanBatch];
       float pxj[kScanBatch], pyj[kScanBatch];
 #pragma unroll
