@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_ul_pairs32(UlArgs a) {
     const float offx = (float)((int)gi - (int)ni) * cellf;
     const uint32_t tx = (gi - ni + 1) << 28;
     for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
-      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      if (__any(nb > kNbCap - kScanBatch)) flush();  // guard: room for the batch, taken together
       float2 pj[kScanBatch];
 #pragma unroll
       for (int e = 0; e < kScanBatch; e++) pj[e] = a.pos32[min(j0 + e, b3 - 1)];
@@ -1068,7 +1068,7 @@ __device__ inline void list_first_global(const UlArgs& a, uint32_t i, uint32_t* 
     const float offx = (float)((int)gi - (int)ni) * cellf;
     const uint32_t tx = (gi - ni + 1) << 28;
     for (uint32_t j0 = b0; j0 < b3; j0 += kScanBatch) {
-      if (nb > kNbCap - kScanBatch) flush();  // guard: room for the batch
+      if (__any(nb > kNbCap - kScanBatch)) flush();  // guard: room for the batch, taken together
       float2 pj[kScanBatch];
 #pragma unroll
       for (int e = 0; e < kScanBatch; e++) pj[e] = a.pos32[min(j0 + e, b3 - 1)];
