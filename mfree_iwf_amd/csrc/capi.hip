@@ -171,7 +171,7 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, hipMemset(c->count, 0, (size_t)N * sizeof(uint32_t)));  // the binning keeps it zero between calls
   HIPCHK(c, dalloc(c, &c->cell_start, N + 1));
   HIPCHK(c, dalloc(c, &c->perm, N));
-  HIPCHK(c, dalloc(c, &c->perm_tmp, N));
+  HIPCHK(c, dalloc(c, &c->perm_tmp, 3 * (size_t)N));
   HIPCHK(c, dalloc(c, &c->tsum, (N + kScanTile - 1) / kScanTile + 1));
   HIPCHK(c, dalloc(c, &c->meta, N));
   HIPCHK(c, dalloc(c, &c->tile_off, c->S + 1));

@@ -52,7 +52,7 @@ struct mfree_ctx {
   uint32_t *tsum = nullptr, *errb = nullptr;
   uint32_t* berr = nullptr;       // errb + 1: the bodies' error words
   uint32_t* perm_keep = nullptr;  // isolated bodies: the step's stage-1 permutation
-  uint32_t* perm_tmp = nullptr;   // UL binning: the scatter's order, before the rank sort
+  uint32_t* perm_tmp = nullptr;   // UL binning: the scatter's order (and ids, keys at its slots), 3N
   std::vector<uint32_t> body_bits;  // isolated bodies: error bits of each body (sticky)
   uint2* meta = nullptr;
   double2* pos64 = nullptr;

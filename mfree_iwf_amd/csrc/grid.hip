@@ -335,6 +335,27 @@ __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* 
   perm[cell_start[key[i]] + rank[i]] = i;
 }
 
+// The scatter of a deterministic binning with a rank sort: besides the
+// particle, its global id and cell key (bit 31: its body's grid fell back to
+// one cell) go to the particle's scatter slot, so that k_cell_rank reads a
+// slot's data and its cell-mates' ids without going through the particle.
+__global__ __launch_bounds__(kBlock) void k_scatter_slots(uint32_t n, const uint32_t* __restrict__ key,
+                                                          const uint32_t* __restrict__ rank,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ gid,
+                                                          const uint32_t* __restrict__ scene,
+                                                          const SceneGeom* __restrict__ geom,
+                                                          uint32_t* __restrict__ perm, uint32_t* __restrict__ gid_at,
+                                                          uint32_t* __restrict__ key_at) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = key[i];
+  const uint32_t t = cell_start[k] + rank[i];
+  perm[t] = i;
+  gid_at[t] = gid[i];
+  key_at[t] = k | (geom[scene[i]].err ? 0x80000000u : 0u);
+}
+
 // One thread per cell: insertion sort of the cell's entries by global id,
 // so that within a cell particles appear in idx order.  Entries arrive
 // nearly sorted (storage is in the previous step's cell order), so a cell
@@ -371,26 +392,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(uint32_t ncell, const uint
 // load is independent of the comparisons, where an insertion sort chains
 // dependent loads.  A body whose grid fell back to one cell (its step fails)
 // keeps the scatter's order: ranking that cell is quadratic in the body.
-__global__ __launch_bounds__(kBlock) void k_cell_rank(uint32_t n, const uint32_t* __restrict__ key,
-                                                      const uint32_t* __restrict__ cell_start,
-                                                      const uint32_t* __restrict__ gid,
-                                                      const uint32_t* __restrict__ scene,
-                                                      const SceneGeom* __restrict__ geom,
+__global__ __launch_bounds__(kBlock) void k_cell_rank(uint32_t n, const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ gid_at,
+                                                      const uint32_t* __restrict__ key_at,
                                                       const uint32_t* __restrict__ perm_in,
                                                       uint32_t* __restrict__ perm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t p = perm_in[i];
-  if (geom[scene[p]].err) {
+  const uint32_t p = perm_in[i], kf = key_at[i], g = gid_at[i];
+  if (kf & 0x80000000u) {
     perm[i] = p;
     return;
   }
-  const uint32_t k = key[p];
-  const uint32_t a = cell_start[k], b = cell_start[k + 1];
-  const uint32_t g = gid[p];
+  const uint32_t a = cell_start[kf], b = cell_start[kf + 1];
   uint32_t r = 0;
   for (uint32_t j = a; j < b; j++) {
-    const uint32_t gj = gid[perm_in[j]];
+    const uint32_t gj = gid_at[j];
     r += (gj < g || (gj == g && j < i)) ? 1u : 0u;
   }
   perm[a + r] = p;
@@ -448,12 +465,18 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, st, a.count, a.ncell_cap, a.tsum, a.cell_start,
                      a.n, 1);
   const bool ranked = a.deterministic && a.perm_tmp;
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
-                     a.cell_start, ranked ? a.perm_tmp : a.perm);
-  if (ranked)
-    hipLaunchKernelGGL(k_cell_rank, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.cell_start, a.gid,
-                       a.scene, a.geom, a.perm_tmp, a.perm);
-  else if (a.deterministic)
+  if (ranked) {
+    uint32_t* gid_at = a.perm_tmp + a.n;
+    uint32_t* key_at = a.perm_tmp + 2 * (size_t)a.n;
+    hipLaunchKernelGGL(k_scatter_slots, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
+                       a.cell_start, a.gid, a.scene, a.geom, a.perm_tmp, gid_at, key_at);
+    hipLaunchKernelGGL(k_cell_rank, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.cell_start, gid_at,
+                       key_at, a.perm_tmp, a.perm);
+  } else {
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, st, a.n, a.key, a.rank,
+                       a.cell_start, a.perm);
+  }
+  if (!ranked && a.deterministic)
     hipLaunchKernelGGL(k_cell_sort, dim3(nblk(a.ncell_cap, kBlock)), dim3(kBlock), 0, st, a.ncell_cap,
                        a.cell_start, a.gid, a.scene, a.geom, a.perm);
 }

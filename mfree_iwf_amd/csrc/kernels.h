@@ -25,7 +25,8 @@ struct BinArgs {
   const uint32_t* gid;          // storage order
   double* part;                 // 5 per chunk
   uint32_t *key, *rank, *count, *tsum, *cell_start, *perm;
-  uint32_t* perm_tmp = nullptr;  // deterministic binning: scatter here, rank-sort into perm (null: in-place sort)
+  uint32_t* perm_tmp = nullptr;  // deterministic binning: 3n words (scatter order, ids and keys at the scatter
+                                 // slots), rank-sorted into perm (null: in-place sort)
   uint32_t* err;
   uint32_t* berr = nullptr;     // per-body error words (null: context word only)
   uint32_t tile_cells;          // > 0: also enumerate pair tiles of this many cells
