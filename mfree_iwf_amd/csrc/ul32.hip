@@ -169,7 +169,7 @@ __device__ inline void pair_terms(const Me& me, const Nb& nj, float xij, float y
 // pair's square root, reciprocals and loads with the rest of its work.
 // AS: 0 no artificial stress (wdeltap <= 0), 1 exponent 4, 2 any exponent.
 template <int AS>
-__device__ inline void pair_terms_flat(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
+__device__ __attribute__((always_inline)) inline void pair_terms_flat(const Me& me, const Nb& nj, float xij, float yij, float r2, Sums& S) {
   const float rij = __builtin_amdgcn_sqrtf(r2);
   const float q = rij * me.h1;
   const bool inside = !(q >= 2.f);
@@ -870,7 +870,7 @@ __device__ inline Nb nb_from_lds(const BandLds& s, uint32_t k) {
 // it are read while the current one is evaluated (reads past the end repeat
 // the last entry, and with no entry stay inside the buffer, unused).
 template <int AS>
-__device__ inline void eval_lds_as(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
+__device__ __attribute__((always_inline)) inline void eval_lds_as(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
                                    float cellf, Sums& S) {
   const int last = max(nb - 1, 0);
   uint32_t wn = my[0], wnn = my[min(1, last) * kBlock];
@@ -892,7 +892,7 @@ __device__ inline void eval_lds_as(const BandLds& s, const uint16_t* my, int nb,
   }
 }
 
-__device__ inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
+__device__ __attribute__((always_inline)) inline void eval_lds(const BandLds& s, const uint16_t* my, int nb, const Me& me, float pix, float piy,
                                 float cellf, Sums& S) {
   const int as = as_mode(me);
   if (as == 1)
