@@ -127,6 +127,10 @@ mfree_status commit(mfree_ctx* c) {
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t N = c->N;
   const bool tl = tl_method(c->opt.method);
+  // the mixed pair passes pack a sorted position into the low 28 bits of a
+  // candidate word (the upper four hold the cell offsets)
+  if (!tl && c->opt.precision == MFREE_PRECISION_MIXED && N > (size_t)kListMask + 1)
+    return fail(c, MFREE_ERR_UNSUPPORTED, "mixed precision UL contexts hold at most 2^28 particles");
   for (int b = 0; b < 2; b++) {
     if (b == 1 && tl) break;  // TL storage is never reordered
     HIPCHK(c, alloc8(c, c->y[b], N));

@@ -36,6 +36,9 @@ constexpr uint32_t kXcdChunkDefault = 64;
 // Mach > 2 impacts), so 512 slots (2 KiB per particle) hold a 6-fold
 // compression before the list overflows (EB_NBH).
 constexpr uint32_t kListCap = 512;
+// mixed pair passes: a candidate word holds the sorted position in its low
+// 28 bits and the cell offsets (dx + 1, dy + 1) in bits 28-29 and 30-31
+constexpr uint32_t kListMask = 0x0FFFFFFFu;
 constexpr double kSkinMin = 0.25;
 constexpr double kSkinVfac = 4.0;
 // a skin above kSkinMax h_max (a particle moving h/2 in one step) is no

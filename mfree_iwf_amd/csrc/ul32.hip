@@ -636,7 +636,6 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
 // checks that the particle stays within half the skin of its stage-1
 // position (error bit EB_SKIN otherwise).  Sums run in list order.
 
-constexpr uint32_t kListMask = 0x0FFFFFFFu;
 
 // Epilogue of a list stage: FP64 derivatives of sorted particle i (storage
 // index i: the stage-1 prepass moved storage into sorted order), the RK4
