@@ -183,14 +183,21 @@ __device__ inline void make_rec32(const SceneConst* __restrict__ c, double cx, d
 // stages 1, 5, 4: k2 at y + dt/2 k1, k3 at y - dt k1 + 2 dt k2,
 // y_new = y + dt/6 (k1 + 4 k2 + k3).  acc holds the weighted sum of the
 // rates so far.
+// Streaming accesses: state read and written once per stage bypasses the
+// caches, which then keep the data that neighbour loads re-read.
+template <class T>
+__device__ inline T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __restrict__ c, uint32_t home,
                                        const double* d) {
   if (a.stage == 1) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home];
-      a.acc.f[k][home] = d[k];
-      a.K.f[k][home] = yk + d[k] * c->half_dt;
+      const double yk = ld_nt(&a.y.f[k][home]);
+      st_nt(&a.acc.f[k][home], d[k]);
+      st_nt(&a.K.f[k][home], yk + d[k] * c->half_dt);
       if (a.k1s.f[k]) a.k1s.f[k][home] = d[k];
     }
   } else if (a.stage == 2 || a.stage == 3) {
@@ -198,9 +205,9 @@ __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __rest
     const Fields8& ks = a.stage == 2 ? a.k2s : a.k3s;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home];
-      a.acc.f[k][home] = a.acc.f[k][home] + 2 * d[k];
-      a.K.f[k][home] = yk + d[k] * dts;
+      const double yk = ld_nt(&a.y.f[k][home]);
+      st_nt(&a.acc.f[k][home], ld_nt(&a.acc.f[k][home]) + 2 * d[k]);
+      st_nt(&a.K.f[k][home], yk + d[k] * dts);
       if (ks.f[k]) ks.f[k][home] = d[k];
     }
   } else if (a.stage == 5) {
@@ -214,8 +221,8 @@ __device__ inline void rk_stage_update(const UlArgs& a, const SceneConst* __rest
   } else {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const double yk = a.y.f[k][home], sum = a.acc.f[k][home] + d[k];
-      a.y.f[k][home] = yk + c->dt6 * sum;
+      const double yk = ld_nt(&a.y.f[k][home]), sum = ld_nt(&a.acc.f[k][home]) + d[k];
+      st_nt(&a.y.f[k][home], yk + c->dt6 * sum);
       if (a.rate_out) a.acc.f[k][home] = sum * (1. / 6.);
     }
   }

@@ -641,10 +641,6 @@ __global__ __launch_bounds__(kBlock) void k_ul_tiles32(UlArgs a) {
 // index i: the stage-1 prepass moved storage into sorted order), the RK4
 // stage update, and the record of the next stage.  The L lanes of the
 // particle hold the same sums; lane q stores the fields f with f % L == q.
-template <class T>
-__device__ inline T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
-template <class T>
-__device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
 // What the epilogue of sorted particle i reads from HBM: the stage's state,
 // the step's start state (stage 4: the fields of lane q only), the weighted
