@@ -538,6 +538,8 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
   ba.skin_vfac = kSkinVfac;
   ba.tile_cells = 0;
   ba.tile_off = nullptr;
+  const bool iso = c->opt.isolate_bodies != 0;
+  if (!iso) ba.advance = c->sc;
   launch_binning(st, ba);
   UlArgs u = ul_args(c, a);
   u.src = c->y[a];
@@ -579,7 +581,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
       evs->push_back({e0, e1});
     }
   }
-  end_of_step(c, a, b, true);
+  end_of_step(c, a, b, iso);
 }
 
 void enqueue_ul_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs) {

@@ -92,14 +92,15 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const Chunk* __restrict
 // One workgroup per body: combine the chunk partials and derive the cell
 // geometry with the reference's arithmetic (src/grid.h:100-114).
 __global__ __launch_bounds__(kBlock) void k_bbox_final(const uint32_t* __restrict__ chunk_first,
-                                                       const SceneConst* __restrict__ sc,
+                                                       const SceneConst* sc,
                                                        const double* __restrict__ part, double kernel_width,
                                                        double cell_min, uint32_t tile_cells,
                                                        SceneGeom* __restrict__ geom, uint32_t* __restrict__ err,
                                                        uint32_t* __restrict__ berr,
                                                        const double* __restrict__ vpart, double skin_min,
-                                                       double skin_vfac) {
+                                                       double skin_vfac, SceneConst* advance) {
   const uint32_t s = blockIdx.x;
+  if (advance && threadIdx.x == 0) advance[s].time = advance[s].time + advance[s].dt;
   double mnx = DBL_MAX, mny = DBL_MAX, mxx = -DBL_MAX, mxy = -DBL_MAX, mxh = -DBL_MAX, mxv = 0.0;
   for (uint32_t c = chunk_first[s] + threadIdx.x; c < chunk_first[s + 1]; c += blockDim.x) {
     const double* p = part + 5 * (size_t)c;
@@ -454,7 +455,7 @@ void launch_binning(hipStream_t st, const BinArgs& a) {
                      skin ? a.vx : nullptr, skin ? a.vy : nullptr, skin ? a.vpart : nullptr);
   hipLaunchKernelGGL(k_bbox_final, dim3(a.nscene), dim3(kBlock), 0, st, a.chunk_first, a.sc, a.part,
                      a.kernel_width, a.cell_min, a.tile_cells, a.geom, a.err, a.berr, skin ? a.vpart : nullptr,
-                     a.skin_min, a.skin_vfac);
+                     a.skin_min, a.skin_vfac, a.advance);
   if (a.tile_cells && a.tile_off)
     hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kBlock), 0, st, a.nscene, a.geom, a.tile_off);
   // a.count is zero here: zeroed at allocation and cleared again by the scan

@@ -38,6 +38,9 @@ struct BinArgs {
   const double* vy = nullptr;
   double* vpart = nullptr;      // max |v|^2 per chunk
   double skin_min = 0.0, skin_vfac = 0.0;
+  // non-null: the body's time advances by its dt (the step that bins here
+  // ends by then; nothing of the step reads the time)
+  SceneConst* advance = nullptr;
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint32_t n);

@@ -95,8 +95,8 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
   const SceneConst* __restrict__ c = a.sc + s;
   double v[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) v[k] = a.src.f[k][p];
-  const double m = a.m[p], h = a.h[p];
+  for (int k = 0; k < 8; k++) v[k] = ld_nt(&a.src.f[k][p]);
+  const double m = ld_nt(&a.m[p]), h = ld_nt(&a.h[p]);
   const double x = v[0], y = v[1], vx = v[2], vy = v[3], rho = v[4], Sxx = v[5], Sxy = v[6], Syy = v[7];
   const double qw = m / rho;
   const double pr = c->c0sq * (rho - c->rho0);
@@ -142,13 +142,13 @@ __global__ __launch_bounds__(kBlock) void k_ul_prepass(UlArgs a) {
     a.rec32[i] = r;
   }
   if (a.reorder) {
-    if (a.perm_keep) a.perm_keep[i] = p;
+    if (a.perm_keep) st_nt(&a.perm_keep[i], p);
 #pragma unroll
-    for (int k = 0; k < 8; k++) a.dst.f[k][i] = v[k];
-    a.m_dst[i] = m;
-    a.h_dst[i] = h;
-    a.scene_dst[i] = s;
-    a.gid_dst[i] = a.gid[p];
+    for (int k = 0; k < 8; k++) st_nt(&a.dst.f[k][i], v[k]);
+    st_nt(&a.m_dst[i], m);
+    st_nt(&a.h_dst[i], h);
+    st_nt(&a.scene_dst[i], s);
+    st_nt(&a.gid_dst[i], ld_nt(&a.gid[p]));
   }
 }
 
