@@ -112,7 +112,8 @@ struct UlArgs {
   uint32_t xcd_chunk;           // per-particle pair kernels: tiles per XCD chunk (xcd_tile_chunked)
   // neighbour-list path (mixed-precision RK4 steps): one binning and one list
   // per step, cells of side 2 h_max + skin; stages read the list
-  uint32_t* nl_col;             // slot-major: entry k of particle i at nl_col[k * n + i],
+  uint32_t* nl_col;             // groups of 4 entries, group-major: entry k of particle i at
+                                // nl_col[((k / 4) * n + i) * 4 + k % 4],
                                 // sorted position | (cell dx + 1) << 28 | (cell dy + 1) << 30
   uint32_t* nl_deg;             // entries per particle
   uint32_t nl_cap;              // slots per particle

@@ -746,6 +746,12 @@ __device__ inline void finish_list(const UlArgs& a, const SceneConst* __restrict
 // cutoff).  The FP64 partial sums of the L lanes are combined with
 // cross-lane shuffles.
 constexpr int kListBatch = 32;
+// Entry k of particle i in the list: groups of 4 entries, group-major, 16 B
+// per particle and group (a wave's 64 particles read a group as 1 KB).
+__device__ inline size_t list_at(uint32_t n, uint32_t i, uint32_t k) {
+  return ((size_t)(k >> 2) * n + i) * 4 + (k & 3u);
+}
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Sorted particle i, lane q of L, every neighbour gathered from global
 // memory; my = this thread's compaction buffer (kListBatch words, stride kBlock).
@@ -758,7 +764,6 @@ __device__ inline void list_stage_global(const UlArgs& a, uint32_t i, uint32_t q
   const float2 pi = a.pos32[i];
   const Me me = make_me(c, nb_from_rec(a.rec32[i]));
   const uint32_t deg = a.nl_deg[i];
-  const uint32_t* __restrict__ col = a.nl_col + i;
   Sums S;
   auto sep = [&](uint32_t e, float2 pj, float& xij, float& yij) {
     xij = (pi.x - pj.x) + (float)((int)((e >> 28) & 3u) - 1) * cellf;
@@ -773,7 +778,7 @@ __device__ inline void list_stage_global(const UlArgs& a, uint32_t i, uint32_t q
 #pragma unroll
       for (int t = 0; t < kScanBatch; t++) {
         const uint32_t k = k0 + t * L;
-        e[t] = ld_nt(&col[(size_t)(k < end ? k : base) * a.n]);  // each list entry is read once
+        e[t] = ld_nt(&a.nl_col[list_at(a.n, i, k < end ? k : base)]);  // each list entry is read once
       }
 #pragma unroll
       for (int t = 0; t < kScanBatch; t++) pj[t] = a.pos32[e[t] & kListMask];
@@ -959,15 +964,18 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   const int32_t tot = s_tot;
   const uint32_t i = i0 + threadIdx.x;
   // issued first, to arrive while the bands are staged: the epilogue's state
-  // and the first kListBatch words of the list (each read once)
+  // and the first kListBatch words of the list (each read once, 4 per load)
   EpiIn ein;
   uint32_t deg = 0, ew[kListBatch];
-  const uint32_t* __restrict__ col = a.nl_col + i;
+  const u32x4* __restrict__ col4 = reinterpret_cast<const u32x4*>(a.nl_col) + i;
   if (tot >= 0 && i < a.n) {
     ein = epi_load<1>(a, i, 0);
     deg = a.nl_deg[i];
 #pragma unroll
-    for (int t = 0; t < kListBatch; t++) ew[t] = (uint32_t)t < deg ? ld_nt(&col[(size_t)t * a.n]) : 0u;
+    for (int g = 0; g < kListBatch / 4; g++) {
+      const u32x4 w = (uint32_t)(4 * g) < deg ? ld_nt(&col4[(size_t)g * a.n]) : u32x4{0u, 0u, 0u, 0u};
+      ew[4 * g] = w.x; ew[4 * g + 1] = w.y; ew[4 * g + 2] = w.z; ew[4 * g + 3] = w.w;
+    }
   }
   band_fill(a, s, tot, s_base, s_end);
   __syncthreads();  // the last barrier: threads past the end leave below
@@ -989,12 +997,16 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   for (uint32_t base = 0; base < deg; base += kListBatch) {
     if (base > 0) {  // a list longer than one batch: its next words
 #pragma unroll
-      for (int t = 0; t < kListBatch; t++) ew[t] = base + t < deg ? ld_nt(&col[(size_t)(base + t) * a.n]) : 0u;
+      for (int g = 0; g < kListBatch / 4; g++) {
+        const u32x4 w = base + 4 * g < deg ? ld_nt(&col4[(size_t)((base >> 2) + g) * a.n]) : u32x4{0u, 0u, 0u, 0u};
+        ew[4 * g] = w.x; ew[4 * g + 1] = w.y; ew[4 * g + 2] = w.z; ew[4 * g + 3] = w.w;
+      }
     }
     int nb = 0;
 #pragma unroll
     for (int t = 0; t < kListBatch; t++) {
-      // words past the list's end are 0: an in-range slot, not kept
+      // words past the list's end (0, or the rest of its last group of 4) give
+      // an in-range slot through the mask and are not kept
       const uint32_t e = ew[t], tx = (e >> 28) & 3u;
       const uint32_t sl = ((e & kListMask) + (tx == 0 ? base0 : tx == 1 ? base1 : base2)) & (kBandCap - 1);
       const float xij = (pix - s.px[sl]) + (float)((int)tx - 1) * cellf;
@@ -1012,7 +1024,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
 
 // Stage 1 of a list step in one pass: the 3x3 block of the step's cells is
 // scanned once; every candidate within 2 h_i + skin goes to the step's list
-// (slot-major), those inside the cutoff also to the LDS buffer, which is
+// (groups of 4 entries, group-major), those inside the cutoff also to the LDS buffer, which is
 // evaluated as in k_ul_pairs32 (collective flushes before a column that may
 // overflow a lane's buffer).  The epilogue is the list stages' one.
 // Sorted particle i from global memory; my = this thread's buffer (kNbCap
@@ -1077,7 +1089,7 @@ __device__ inline void list_first_global(const UlArgs& a, uint32_t i, uint32_t* 
           const float r2 = xij * xij + yij * yij;
           if (r2 <= rl2) {
             const uint32_t word = j | tx | ((gj - nj + 1) << 30);
-            if (deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = word;
+            if (deg < a.nl_cap) a.nl_col[list_at(a.n, i, deg)] = word;
             deg++;
             if (r2 <= me.cut2) my[nb++ * kBlock] = word;
           }
@@ -1202,7 +1214,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
         const float r2 = xij * xij + yij * yij;
         const bool listed = j < b3 && j != i && r2 <= rl2;
         const uint32_t ty = gj - nj + 1;
-        if (listed && deg < a.nl_cap) a.nl_col[(size_t)deg * a.n + i] = j | (tx << 28) | (ty << 30);
+        // entries past the capacity overwrite the last slot: the step fails (EB_NBH)
+        if (listed) a.nl_col[list_at(a.n, i, min(deg, a.nl_cap - 1))] = j | (tx << 28) | (ty << 30);
         deg += listed ? 1u : 0u;
         my[nb * kBlock] = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
         nb += (listed && r2 <= me.cut2) ? 1 : 0;
