@@ -189,6 +189,7 @@ mfree_status commit(mfree_ctx* c) {
         c->list_lanes = c->opt.list_lanes ? c->opt.list_lanes : 1;
         c->list_staged = c->opt.list_staging >= 0;
         HIPCHK(c, dalloc(c, &c->nl_col, (size_t)kListCap * N));
+        if (c->list_staged && c->list_lanes == 1) HIPCHK(c, dalloc(c, &c->nl16, (size_t)kListCap * N));
         HIPCHK(c, dalloc(c, &c->nl_deg, N));
         HIPCHK(c, dalloc(c, &c->pos32b, N));
         HIPCHK(c, dalloc(c, &c->rec32b, N));
@@ -554,6 +555,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
   UlArgs p = ul_args(c, b);
   p.home_identity = 1;
   p.nl_col = c->nl_col;
+  p.nl16 = c->nl16;
   p.nl_deg = c->nl_deg;
   p.nl_cap = c->nl_cap;
   float2* pos[2] = {c->pos32, c->pos32b};
@@ -573,7 +575,7 @@ void enqueue_ul_list_step(mfree_ctx* c, std::vector<std::pair<hipEvent_t, hipEve
       if (e0) hipEventRecord(e0, st);
     }
     if (k == 1)
-      launch_ul_list_first(st, p, c->list_staged);
+      launch_ul_list_first(st, p, c->list_staged && c->list_lanes == 1);  // as the stages that read the list
     else
       launch_ul_list(st, p, c->list_lanes, c->list_staged);
     if (evs && e0 && e1) {

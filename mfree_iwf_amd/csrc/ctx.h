@@ -75,6 +75,7 @@ struct mfree_ctx {
   bool list_staged = true;  // list steps: neighbours staged in LDS (stage 1, and stages with one lane)
   uint32_t nl_cap = 0;
   uint32_t *nl_col = nullptr, *nl_deg = nullptr;
+  uint16_t* nl16 = nullptr;  // staged list stages: the list as band-slot words
   float2* pos32b = nullptr;      // second record buffer (records of the next stage)
   PairRec32* rec32b = nullptr;
   double* vpart = nullptr;       // max |v|^2 per binning chunk

@@ -117,6 +117,9 @@ struct UlArgs {
                                 // sorted position | (cell dx + 1) << 28 | (cell dy + 1) << 30
   uint32_t* nl_deg;             // entries per particle
   uint32_t nl_cap;              // slots per particle
+  uint16_t* nl16;               // workgroups whose candidates fit the LDS bands (staged stages):
+                                // the list as band-slot words slot | (dx + 1) << 10 | (dy + 1) << 12,
+                                // groups of 8, group-major: entry k at nl16[((k / 8) * n + i) * 8 + k % 8]
   float2* pos32_out;            // records of the next stage (stages 1-3)
   PairRec32* rec32_out;
 };

@@ -752,6 +752,10 @@ __device__ inline size_t list_at(uint32_t n, uint32_t i, uint32_t k) {
   return ((size_t)(k >> 2) * n + i) * 4 + (k & 3u);
 }
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// Entry k of particle i in the band-slot list: groups of 8 words, group-major.
+__device__ inline size_t list16_at(uint32_t n, uint32_t i, uint32_t k) {
+  return ((size_t)(k >> 3) * n + i) * 8 + (k & 7u);
+}
 
 // Sorted particle i, lane q of L, every neighbour gathered from global
 // memory; my = this thread's compaction buffer (kListBatch words, stride kBlock).
@@ -964,16 +968,17 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   const int32_t tot = s_tot;
   const uint32_t i = i0 + threadIdx.x;
   // issued first, to arrive while the bands are staged: the epilogue's state
-  // and the first kListBatch words of the list (each read once, 4 per load)
+  // and the first kListBatch band-slot words of the list (each read once, 8
+  // per load, two per register)
   EpiIn ein;
-  uint32_t deg = 0, ew[kListBatch];
-  const u32x4* __restrict__ col4 = reinterpret_cast<const u32x4*>(a.nl_col) + i;
+  uint32_t deg = 0, ew[kListBatch / 2];
+  const u32x4* __restrict__ col8 = reinterpret_cast<const u32x4*>(a.nl16) + i;
   if (tot >= 0 && i < a.n) {
     ein = epi_load<1>(a, i, 0);
     deg = a.nl_deg[i];
 #pragma unroll
-    for (int g = 0; g < kListBatch / 4; g++) {
-      const u32x4 w = (uint32_t)(4 * g) < deg ? ld_nt(&col4[(size_t)g * a.n]) : u32x4{0u, 0u, 0u, 0u};
+    for (int g = 0; g < kListBatch / 8; g++) {
+      const u32x4 w = (uint32_t)(8 * g) < deg ? ld_nt(&col8[(size_t)g * a.n]) : u32x4{0u, 0u, 0u, 0u};
       ew[4 * g] = w.x; ew[4 * g + 1] = w.y; ew[4 * g + 2] = w.z; ew[4 * g + 3] = w.w;
     }
   }
@@ -984,7 +989,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
     list_stage_global<1>(a, i, 0, reinterpret_cast<uint32_t*>(&s) + threadIdx.x);
     return;
   }
-  const uint32_t base0 = s_base[0], base1 = s_base[1], base2 = s_base[2];
+  const uint32_t base1 = s_base[1];
   uint16_t* const my = s_nb + threadIdx.x;
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -997,22 +1002,20 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   for (uint32_t base = 0; base < deg; base += kListBatch) {
     if (base > 0) {  // a list longer than one batch: its next words
 #pragma unroll
-      for (int g = 0; g < kListBatch / 4; g++) {
-        const u32x4 w = base + 4 * g < deg ? ld_nt(&col4[(size_t)((base >> 2) + g) * a.n]) : u32x4{0u, 0u, 0u, 0u};
+      for (int g = 0; g < kListBatch / 8; g++) {
+        const u32x4 w = base + 8 * g < deg ? ld_nt(&col8[(size_t)((base >> 3) + g) * a.n]) : u32x4{0u, 0u, 0u, 0u};
         ew[4 * g] = w.x; ew[4 * g + 1] = w.y; ew[4 * g + 2] = w.z; ew[4 * g + 3] = w.w;
       }
     }
     int nb = 0;
 #pragma unroll
     for (int t = 0; t < kListBatch; t++) {
-      // words past the list's end (0, or the rest of its last group of 4) give
-      // an in-range slot through the mask and are not kept
-      const uint32_t e = ew[t], tx = (e >> 28) & 3u;
-      const uint32_t sl = ((e & kListMask) + (tx == 0 ? base0 : tx == 1 ? base1 : base2)) & (kBandCap - 1);
-      const float xij = (pix - s.px[sl]) + (float)((int)tx - 1) * cellf;
-      const float yij = (piy - s.py[sl]) + (float)((int)(e >> 30) - 1) * cellf;
-      if (base + t < deg && xij * xij + yij * yij <= me.cut2)
-        my[nb++ * kBlock] = (uint16_t)(sl | ((e >> 18) & 0x3C00u));  // slot | tx << 10 | ty << 12
+      // slot | tx << 10 | ty << 12; words past the list's end give an in-range
+      // slot through the mask and are not kept
+      const uint32_t e = (ew[t >> 1] >> (16 * (t & 1))) & 0xFFFFu, sl = e & (kBandCap - 1);
+      const float xij = (pix - s.px[sl]) + (float)((int)((e >> 10) & 3u) - 1) * cellf;
+      const float yij = (piy - s.py[sl]) + (float)((int)(e >> 12) - 1) * cellf;
+      if (base + t < deg && xij * xij + yij * yij <= me.cut2) my[nb++ * kBlock] = (uint16_t)e;
     }
     eval_lds(s, my, nb, me, pix, piy, cellf, S);
   }
@@ -1024,7 +1027,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
 
 // Stage 1 of a list step in one pass: the 3x3 block of the step's cells is
 // scanned once; every candidate within 2 h_i + skin goes to the step's list
-// (groups of 4 entries, group-major), those inside the cutoff also to the LDS buffer, which is
+// (32-bit words, groups of 4, group-major), those inside the cutoff also to the LDS buffer, which is
 // evaluated as in k_ul_pairs32 (collective flushes before a column that may
 // overflow a lane's buffer).  The epilogue is the list stages' one.
 // Sorted particle i from global memory; my = this thread's buffer (kNbCap
@@ -1120,7 +1123,10 @@ __global__ __launch_bounds__(kBlock) void k_ul_list_first32(UlArgs a) {
 // the list stages) hold every particle of the 3x3 cell blocks its particles
 // scan, so the scan reads positions from LDS and the pairs inside the cutoff
 // are evaluated from LDS.  Candidates are visited, listed and summed in the
-// global path's order: the same list and the same bits.
+// global path's order: the same list and the same bits.  The list is written
+// as band-slot words (nl16), which the staged list stages read; a workgroup
+// whose bands do not fit takes the global path and its 32-bit words, in every
+// stage of the step alike (the bands depend on the step's binning only).
 static_assert(sizeof(BandLds) >= sizeof(uint32_t) * kNbCap * kBlock, "global path buffer aliases the bands");
 
 __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
@@ -1214,10 +1220,11 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_first_lds32(UlArgs a) {
         const float r2 = xij * xij + yij * yij;
         const bool listed = j < b3 && j != i && r2 <= rl2;
         const uint32_t ty = gj - nj + 1;
+        const uint16_t w = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
         // entries past the capacity overwrite the last slot: the step fails (EB_NBH)
-        if (listed) a.nl_col[list_at(a.n, i, min(deg, a.nl_cap - 1))] = j | (tx << 28) | (ty << 30);
+        if (listed) a.nl16[list16_at(a.n, i, min(deg, a.nl_cap - 1))] = w;
         deg += listed ? 1u : 0u;
-        my[nb * kBlock] = (uint16_t)(sl[e] | (tx << 10) | (ty << 12));
+        my[nb * kBlock] = w;
         nb += (listed && r2 <= me.cut2) ? 1 : 0;
       }
     }
