@@ -758,8 +758,9 @@ __device__ inline size_t list16_at(uint32_t n, uint32_t i, uint32_t k) {
 }
 
 // Sorted particle i, lane q of L, every neighbour gathered from global
-// memory; my = this thread's compaction buffer (kListBatch words, stride kBlock).
-template <int L>
+// memory; my = this thread's compaction buffer (kListBatch words, stride kBlock);
+// the positions of B entries are loaded together.
+template <int L, int B = kScanBatch>
 __device__ inline void list_stage_global(const UlArgs& a, uint32_t i, uint32_t q, uint32_t* my) {
   const uint2 mt = a.meta[i];
   const SceneConst* __restrict__ c = a.sc + mt.y;
@@ -776,18 +777,18 @@ __device__ inline void list_stage_global(const UlArgs& a, uint32_t i, uint32_t q
   for (uint32_t base = q; base < deg; base += kListBatch * L) {
     const uint32_t end = min(deg, base + kListBatch * L);
     int nb = 0;
-    for (uint32_t k0 = base; k0 < end; k0 += kScanBatch * L) {
-      uint32_t e[kScanBatch];
-      float2 pj[kScanBatch];
+    for (uint32_t k0 = base; k0 < end; k0 += B * L) {
+      uint32_t e[B];
+      float2 pj[B];
 #pragma unroll
-      for (int t = 0; t < kScanBatch; t++) {
+      for (int t = 0; t < B; t++) {
         const uint32_t k = k0 + t * L;
         e[t] = ld_nt(&a.nl_col[list_at(a.n, i, k < end ? k : base)]);  // each list entry is read once
       }
 #pragma unroll
-      for (int t = 0; t < kScanBatch; t++) pj[t] = a.pos32[e[t] & kListMask];
+      for (int t = 0; t < B; t++) pj[t] = a.pos32[e[t] & kListMask];
 #pragma unroll
-      for (int t = 0; t < kScanBatch; t++) {
+      for (int t = 0; t < B; t++) {
         float xij, yij;
         sep(e[t], pj[t], xij, yij);
         if (k0 + t * L < end && xij * xij + yij * yij <= me.cut2) my[nb++ * kBlock] = e[t];
@@ -986,7 +987,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_ul_list_lds32(UlArgs a) {
   __syncthreads();  // the last barrier: threads past the end leave below
   if (i >= a.n) return;
   if (tot < 0) {
-    list_stage_global<1>(a, i, 0, reinterpret_cast<uint32_t*>(&s) + threadIdx.x);
+    // batches of 8 here: the same bits, and the staged path above and below
+    // compiles to a faster schedule (0.177 -> 0.169 ms per stage) than with 4
+    list_stage_global<1, 8>(a, i, 0, reinterpret_cast<uint32_t*>(&s) + threadIdx.x);
     return;
   }
   const uint32_t base1 = s_base[1];
