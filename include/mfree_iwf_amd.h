@@ -134,7 +134,8 @@ typedef struct mfree_options {
                                evaluates its lists from there (workgroups whose runs exceed 1024
                                particles or that span two bodies gather from global memory);
                                -1 = off: every neighbour gathered from global memory.  Both give
-                               the same sums in the same order */
+                               the same sums in the same order.  Applies with list_lanes 1 (the
+                               default); with 2 or 4 lanes every stage gathers from global memory */
 } mfree_options;
 
 /* SoA host view of particles (src/particle.h:42-60), idx order.  X, Y
